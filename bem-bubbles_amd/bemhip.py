@@ -1,0 +1,309 @@
+"""Python mirror of the reference's simulation interface over libbemhip's C-ABI.
+
+The class `ColocSim` follows the method names and argument meaning of the
+reference's Bem::ColocSim / LinLinSim / Simulation (Simulation.hpp:29-167,
+LinLinSim.hpp:22-67): evolve_system_RK4, evolve_system, compute_psi,
+set_phi/get_phi, get_psi, set_V_0, set_time/get_time, get_volume,
+set_bcgstab, set_min_dt, set_dp_balance, position_t, pot_t_multi.  Errors are
+raised like the reference raises them (IndexError for the wall-mode
+"negative values" std::out_of_range) or as BemHipError with the library's
+status code.
+
+There is no CPU fallback: every call runs the gfx950 kernels of libbemhip.so,
+and loading fails loudly when the library is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbemhip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "bemhip.h")
+
+OK, ERR_ARG, ERR_BAD_MESH, ERR_NEG_X, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_NAN, ERR_OOM, ERR_HIP, ERR_NO_DEVICE = range(10)
+MIRROR_X0 = 0x1
+SOLVERS = {"lu64": 0, "lu32_ir64": 1, "bicgstab64": 2, "lu16x3_ir64": 3}
+
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+class BemHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bemhip status {code}: {msg}")
+        self.code = code
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double),
+                ("correction", C.c_double), ("ms_assemble", C.c_double), ("ms_factor", C.c_double),
+                ("ms_solve", C.c_double), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
+                ("gemm_launches", C.c_int32), ("pad_", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+class SimParams(C.Structure):
+    _fields_ = [("p_inf", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double), ("gamma", C.c_double),
+                ("V_0", C.c_double), ("field_amp", C.c_double), ("field_freq", C.c_double),
+                ("field_phase", C.c_double), ("dp_balance", C.c_double), ("min_dt", C.c_double),
+                ("wall_eps", C.c_double), ("tol", C.c_double), ("maxit", C.c_int32), ("solver", C.c_int32),
+                ("flags", C.c_uint32), ("wall_guards", C.c_int32), ("frozen_stage_time", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+_lib = None
+
+
+def declared_functions(header: str = HEADER):
+    """Names of the functions declared by include/bemhip.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(bemhip_\w+)\s*\(", text, flags=re.M)))
+
+
+def load(path: str = LIB_PATH):
+    """Load libbemhip.so (no device work).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libbemhip.so not built at {path}; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.bemhip_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.bemhip_destroy.argtypes = [vp]
+    L.bemhip_last_error.argtypes = [vp]
+    L.bemhip_last_error.restype = C.c_char_p
+    L.bemhip_quadrature.argtypes = [f64p]
+    L.bemhip_quadrature.restype = None
+    L.bemhip_assemble.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p, C.c_uint32, vp, vp, vp, vp]
+    L.bemhip_solve.argtypes = [vp, C.c_int64, vp, f64p, f64p, C.c_int, C.c_double, C.c_int32, C.POINTER(SolveStats)]
+    L.bemhip_compute_psi.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p, f64p, C.c_uint32, C.c_int,
+                                     C.c_double, C.c_int32, f64p, C.POINTER(SolveStats)]
+    L.bemhip_sim_default_params.argtypes = [C.POINTER(SimParams)]
+    L.bemhip_sim_default_params.restype = None
+    L.bemhip_sim_create.argtypes = [vp, C.POINTER(SimParams), C.c_int64, f64p, C.c_int64, u32p, C.POINTER(vp)]
+    L.bemhip_sim_destroy.argtypes = [vp]
+    L.bemhip_sim_set_phi.argtypes = [vp, f64p]
+    L.bemhip_sim_set_psi.argtypes = [vp, f64p]
+    L.bemhip_sim_set_V_0.argtypes = [vp, C.c_double]
+    L.bemhip_sim_set_time.argtypes = [vp, C.c_double]
+    L.bemhip_sim_set_solver.argtypes = [vp, C.c_int, C.c_double, C.c_int32]
+    L.bemhip_sim_get_state.argtypes = [vp, vp, vp, vp, C.POINTER(C.c_double)]
+    L.bemhip_sim_get_volume.argtypes = [vp, C.POINTER(C.c_double)]
+    L.bemhip_sim_position_t.argtypes = [vp, vp, vp]
+    L.bemhip_sim_pot_t_multi.argtypes = [vp, f64p, C.c_double, f64p]
+    L.bemhip_sim_compute_psi.argtypes = [vp]
+    L.bemhip_sim_evolve_rk4.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.bemhip_sim_evolve_euler.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.bemhip_sim_last_stats.argtypes = [vp, C.POINTER(SolveStats)]
+    L.bemhip_sim_accum_stats.argtypes = [vp, C.POINTER(SolveStats), C.c_int]
+    _lib = L
+    return L
+
+
+def quadrature():
+    out = np.zeros(35)
+    load().bemhip_quadrature(out)
+    return {"gauss_x": out[0:7], "gauss_w": out[7:14], "tri_x": out[14:21], "tri_y": out[21:28], "tri_w": out[28:35]}
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _solver_id(s):
+    return SOLVERS[s] if isinstance(s, str) else int(s)
+
+
+class Context:
+    """A device, stream and workspace (bemhip_create / bemhip_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.bemhip_create(device, C.byref(h))
+        if rc != OK:
+            raise BemHipError(rc, "bemhip_create failed (no gfx950 device?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bemhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != OK:
+            raise BemHipError(rc, self.L.bemhip_last_error(self.h).decode())
+
+    # ColocSim::assemble_matrices (ColocSim.cpp:21-110)
+    def assemble(self, verts, tris, mirror=False, want_H=True, phi=None):
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        N, M = v.shape[0], t.shape[0]
+        G = np.zeros((N, N), order="F")
+        H = np.zeros((N, N), order="F") if want_H else None
+        ph = np.ascontiguousarray(phi, dtype=np.float64) if phi is not None else None
+        hphi = np.zeros(N) if phi is not None else None
+        self.check(self.L.bemhip_assemble(self.h, N, v.reshape(-1), M, t.reshape(-1),
+                                          MIRROR_X0 if mirror else 0, _ptr(G), _ptr(H), _ptr(ph), _ptr(hphi)))
+        return G, H, hphi
+
+    # Simulation::solve_system (Simulation.cpp:40-71)
+    def solve(self, G, b, solver="lu64", tol=0.0, maxit=0):
+        Gf = np.asfortranarray(G, dtype=np.float64)
+        x = np.zeros(Gf.shape[0])
+        st = SolveStats()
+        self.check(self.L.bemhip_solve(self.h, Gf.shape[0], _ptr(Gf), np.ascontiguousarray(b, dtype=np.float64),
+                                       x, _solver_id(solver), tol, maxit, C.byref(st)))
+        return x, st.as_dict()
+
+    # Simulation::compute_psi (Simulation.hpp:73-77)
+    def compute_psi(self, verts, tris, phi, solver="lu64", mirror=False, tol=0.0, maxit=0):
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        psi = np.zeros(v.shape[0])
+        st = SolveStats()
+        self.check(self.L.bemhip_compute_psi(self.h, v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1),
+                                             np.ascontiguousarray(phi, dtype=np.float64),
+                                             MIRROR_X0 if mirror else 0, _solver_id(solver), tol, maxit, psi,
+                                             C.byref(st)))
+        return psi, st.as_dict()
+
+
+class ColocSim:
+    """Device-resident drop-in for Bem::ColocSim (ColocSim.hpp:16-22)."""
+
+    def __init__(self, ctx: Context, verts, tris, p_inf=1.0, epsilon=1.0, sigma=0.0, gamma=1.0,
+                 field_amp=0.0, field_freq=0.0, field_phase=0.0, mirror=False, wall_guards=None,
+                 frozen_stage_time=True, solver="bicgstab64", tol=0.0, maxit=0, V_0=None):
+        self.ctx = ctx
+        L = ctx.L
+        p = SimParams()
+        L.bemhip_sim_default_params(C.byref(p))
+        p.p_inf, p.epsilon, p.sigma, p.gamma = p_inf, epsilon, sigma, gamma
+        p.field_amp, p.field_freq, p.field_phase = field_amp, field_freq, field_phase
+        p.flags = MIRROR_X0 if mirror else 0
+        p.wall_guards = int(mirror if wall_guards is None else wall_guards)
+        p.frozen_stage_time = int(frozen_stage_time)
+        p.solver, p.tol, p.maxit = _solver_id(solver), tol, maxit
+        if V_0 is not None:
+            p.V_0 = V_0
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        self.N, self.M = v.shape[0], t.shape[0]
+        h = C.c_void_p()
+        ctx.check(L.bemhip_sim_create(ctx.h, C.byref(p), self.N, v.reshape(-1), self.M, t.reshape(-1), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.bemhip_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc == ERR_NEG_X:
+            raise IndexError("negative values")   # LinLinSim.cpp:157 throws std::out_of_range
+        self.ctx.check(rc)
+
+    # setters / getters (Simulation.hpp:92-167)
+    def set_phi(self, phi):
+        phi = np.ascontiguousarray(np.broadcast_to(np.asarray(phi, dtype=np.float64), (self.N,)))
+        self._check(self.ctx.L.bemhip_sim_set_phi(self.h, phi))
+
+    def set_psi(self, psi):
+        self._check(self.ctx.L.bemhip_sim_set_psi(self.h, np.ascontiguousarray(psi, dtype=np.float64)))
+
+    def set_V_0(self, v):
+        self._check(self.ctx.L.bemhip_sim_set_V_0(self.h, float(v)))
+
+    def set_time(self, t):
+        self._check(self.ctx.L.bemhip_sim_set_time(self.h, float(t)))
+
+    def set_solver(self, solver, tol=0.0, maxit=0):
+        self._check(self.ctx.L.bemhip_sim_set_solver(self.h, _solver_id(solver), tol, maxit))
+
+    def set_bcgstab(self, value: bool):   # Simulation.hpp:153-155
+        self.set_solver("bicgstab64" if value else "lu64")
+
+    def state(self):
+        x = np.zeros((self.N, 3))
+        phi = np.zeros(self.N)
+        psi = np.zeros(self.N)
+        t = C.c_double()
+        self._check(self.ctx.L.bemhip_sim_get_state(self.h, _ptr(x), _ptr(phi), _ptr(psi), C.byref(t)))
+        return x, phi, psi, t.value
+
+    def get_vertices(self):
+        return self.state()[0]
+
+    def get_phi(self):
+        return self.state()[1]
+
+    def get_psi(self):
+        return self.state()[2]
+
+    def get_time(self):
+        return self.state()[3]
+
+    def get_volume(self):
+        v = C.c_double()
+        self._check(self.ctx.L.bemhip_sim_get_volume(self.h, C.byref(v)))
+        return v.value
+
+    # hot path
+    def position_t(self):
+        vel = np.zeros((self.N, 3))
+        psi = np.zeros(self.N)
+        self._check(self.ctx.L.bemhip_sim_position_t(self.h, _ptr(vel), _ptr(psi)))
+        return vel, psi
+
+    def pot_t_multi(self, vel, t):
+        out = np.zeros(self.N)
+        self._check(self.ctx.L.bemhip_sim_pot_t_multi(self.h, np.ascontiguousarray(vel, dtype=np.float64).reshape(-1),
+                                                      float(t), out))
+        return out
+
+    def compute_psi(self):
+        self._check(self.ctx.L.bemhip_sim_compute_psi(self.h))
+        return self.get_psi()
+
+    def evolve_system_RK4(self, dp, fixdt=False):
+        dt = C.c_double()
+        self._check(self.ctx.L.bemhip_sim_evolve_rk4(self.h, float(dp), int(fixdt), C.byref(dt)))
+        return dt.value
+
+    def evolve_system(self, dp, fixdt=False):
+        dt = C.c_double()
+        self._check(self.ctx.L.bemhip_sim_evolve_euler(self.h, float(dp), int(fixdt), C.byref(dt)))
+        return dt.value
+
+    def last_stats(self):
+        st = SolveStats()
+        self._check(self.ctx.L.bemhip_sim_last_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def accum_stats(self, reset=False):
+        st = SolveStats()
+        self._check(self.ctx.L.bemhip_sim_accum_stats(self.h, C.byref(st), int(reset)))
+        return st.as_dict()

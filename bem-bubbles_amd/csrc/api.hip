@@ -1,0 +1,542 @@
+// C-ABI of libbemhip and the device-resident simulation handle that mirrors
+// the reference's LinLinSim / ColocSim state machine (LinLinSim.cpp:26-284,
+// Simulation.hpp:29-167).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+#include "internal.hpp"
+#include "quad_rules.hpp"
+
+using namespace bemhip;
+
+namespace bemhip {
+__global__ void k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+}  // namespace bemhip
+
+struct bemhip_sim {
+    bemhip_ctx* ctx = nullptr;
+    bemhip_sim_params P{};
+    Topology th;
+    DevTopology topo;
+    int64_t N = 0, M = 0;
+    double time = 0.0;
+    DevBuf x, phi, psi, x1, p1, xs, ps, kx, kp, avg, vn, tnorm, tgrad, tvol, pvol, tk, hphi, psil;
+    bemhip_solve_stats last{};
+    bemhip_solve_stats accum{};
+};
+
+namespace {
+
+template <class F> int guarded(bemhip_ctx* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return BEMHIP_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->err = "host allocation failed";
+        return BEMHIP_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->err = e.what();
+        return BEMHIP_ERR_HIP;
+    }
+}
+
+void check_finite(const double* p, int64_t n, const char* what) {
+    if (!p) return;
+    for (int64_t i = 0; i < n; ++i)
+        BH_CHECK(std::isfinite(p[i]), BEMHIP_ERR_NAN, std::string("non-finite value in ") + what);
+}
+
+template <class T> T* dev(DevBuf& b, size_t n) { return static_cast<T*>(b.ensure(sizeof(T) * std::max<size_t>(n, 1))); }
+
+void h2d(void* d, const void* h, size_t bytes, hipStream_t s) {
+    BH_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+}
+void d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
+    BH_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+    BH_HIP(hipStreamSynchronize(s));
+}
+
+bool mixed(int solver) { return solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64; }
+
+float event_ms(hipEvent_t a, hipEvent_t b) {
+    BH_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// assembly + H phi + solve on device-resident positions / potential
+void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi, uint32_t flags,
+                int solver, double tol, int maxit, double* d_hphi, double* d_psi, bemhip_solve_stats* st) {
+    const int64_t N = topo.N;
+    BH_CHECK(N <= 65536, BEMHIP_ERR_ARG, "dense solve supports N <= 65536");
+    const size_t nn = (size_t)N * (size_t)N;
+    double* G64 = dev<double>(ctx->G64, nn);
+    float* G32 = mixed(solver) ? dev<float>(ctx->G32, nn) : nullptr;
+    hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+    BH_HIP(hipEventRecord(e0, ctx->stream));
+    assemble_device(ctx, topo, d_xyz, d_phi, flags, AsmOut{G64, G32, nullptr, d_hphi});
+    BH_HIP(hipEventRecord(e1, ctx->stream));
+    const float ams = event_ms(e0, e1);
+    solve_device(ctx, (int)N, G64, G32, d_hphi, d_psi, solver, tol, maxit, st);
+    st->ms_assemble = ams;
+}
+
+void accumulate(bemhip_solve_stats& a, const bemhip_solve_stats& s) {
+    a.iterations += s.iterations;
+    a.ms_assemble += s.ms_assemble;
+    a.ms_factor += s.ms_factor;
+    a.ms_solve += s.ms_solve;
+    a.gemm_ms += s.gemm_ms;
+    a.gemm_flops += s.gemm_flops;
+    a.gemm_launches += s.gemm_launches;
+    a.residual = std::max(a.residual, s.residual);
+    a.correction = std::max(a.correction, s.correction);
+    const bool first = a.pad_ == 0;   // pad_ marks a non-empty accumulation
+    a.converged = first ? s.converged : (a.converged && s.converged);
+    a.pad_ = 1;
+}
+
+StageParams stage_params(const bemhip_sim* s) {
+    StageParams sp;
+    sp.p_inf = s->P.p_inf; sp.epsilon = s->P.epsilon; sp.sigma = s->P.sigma; sp.gamma = s->P.gamma;
+    sp.V_0 = s->P.V_0;
+    sp.field_amp = s->P.field_amp; sp.field_freq = s->P.field_freq; sp.field_phase = s->P.field_phase;
+    return sp;
+}
+
+// LinLinSim::position_t (LinLinSim.cpp:26-83) followed by pot_t_multi
+// (LinLinSim.cpp:98-120) for one stage: positions xs, potential ps ->
+// velocities kx, Dphi/Dt kp
+void stage(bemhip_sim* s, const double* xs, const double* ps, double t, double* kx, double* kp) {
+    bemhip_ctx* ctx = s->ctx;
+    const int64_t N = s->N, M = s->M;
+    double* hphi = dev<double>(s->hphi, N);
+    double* psil = dev<double>(s->psil, N);
+    psi_device(ctx, s->topo, xs, ps, s->P.flags, s->P.solver, s->P.tol, s->P.maxit, hphi, psil, &s->last);
+    accumulate(s->accum, s->last);
+    double* tnorm = dev<double>(s->tnorm, 3 * M);
+    double* tgrad = dev<double>(s->tgrad, 3 * M);
+    double* tvol = dev<double>(s->tvol, M);
+    tri_geometry(ctx, s->topo, xs, ps, tnorm, tgrad, tvol);
+    vertex_velocity(ctx, s->topo, tnorm, tgrad, psil, kx);
+    if (!kp) return;
+    double* pvol = dev<double>(s->pvol, s->topo.nparts);
+    part_volumes(ctx, s->topo, tvol, pvol);
+    double* tk = nullptr;
+    if (s->P.sigma != 0.0) {
+        double* vn = dev<double>(s->vn, 3 * N);
+        tk = dev<double>(s->tk, M);
+        vertex_normals(ctx, s->topo, xs, vn);
+        tri_curvature(ctx, s->topo, xs, vn, tk);
+    }
+    vertex_dpot(ctx, s->topo, stage_params(s), xs, kx, tk, pvol, t, kp);
+}
+
+double get_dt(bemhip_sim* s, double dp, const double* kx, const double* kp) {  // Simulation.cpp:77-92
+    double m[2];
+    dt_reduce(s->ctx, s->N, kx, kp, m);
+    double dt = dp / (m[0] + s->P.dp_balance * std::sqrt(m[1]));
+    if (s->P.min_dt > 0.0) dt = std::min(dt, s->P.min_dt);
+    return dt;
+}
+
+}  // namespace
+
+// =============================================================== C-ABI ==
+extern "C" {
+
+int bemhip_create(int device, bemhip_ctx** out) {
+    if (!out) return BEMHIP_ERR_ARG;
+    *out = nullptr;
+    auto ctx = std::make_unique<bemhip_ctx>();
+    int rc = guarded(ctx.get(), [&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+            throw Error(BEMHIP_ERR_NO_DEVICE, "no HIP device " + std::to_string(device));
+        BH_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        BH_HIP(hipGetDeviceProperties(&prop, device));
+        BH_CHECK(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, BEMHIP_ERR_NO_DEVICE,
+                 std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        ctx->device = device;
+        ctx->cus = prop.multiProcessorCount;
+        BH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        BH_BLAS(rocblas_create_handle(&ctx->blas));
+        BH_BLAS(rocblas_set_stream(ctx->blas, ctx->stream));
+        BH_HIP(hipHostMalloc((void**)&ctx->host_scal, 64 * sizeof(double), hipHostMallocDefault));
+        for (auto& e : ctx->ev) BH_HIP(hipEventCreate(&e));
+    });
+    if (rc != BEMHIP_OK) {
+        // keep the context alive just long enough to report; callers get no handle
+        return rc;
+    }
+    *out = ctx.release();
+    return BEMHIP_OK;
+}
+
+int bemhip_destroy(bemhip_ctx* ctx) {
+    if (!ctx) return BEMHIP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+    if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BEMHIP_OK;
+}
+
+const char* bemhip_last_error(const bemhip_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void bemhip_quadrature(double* o) {
+    for (int q = 0; q < 7; ++q) {
+        o[q] = GQ_X[q];
+        o[7 + q] = GQ_W[q];
+        o[14 + q] = TQ_X[q];
+        o[21 + q] = TQ_Y[q];
+        o[28 + q] = TQ_W[q];
+    }
+}
+
+int bemhip_assemble(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri, uint32_t flags,
+                    double* G, double* H, const double* phi, double* Hphi) {
+    if (!ctx) return BEMHIP_ERR_ARG;
+    return guarded(ctx, [&] {
+        BH_CHECK(xyz && tri && G, BEMHIP_ERR_ARG, "null argument");
+        BH_CHECK(N <= 46340, BEMHIP_ERR_ARG, "host G output limited to N <= 46340");
+        BH_CHECK(!(flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown assembly flag");
+        check_finite(phi, N, "phi");
+        Topology th = build_topology(N, xyz, M, tri);
+        DevTopology topo;
+        topo.upload(th, ctx->stream);
+        double* dx = dev<double>(ctx->xyz, 3 * N);
+        h2d(dx, xyz, sizeof(double) * 3 * N, ctx->stream);
+        double* dphi = nullptr;
+        double* dh = nullptr;
+        if (phi && Hphi) {
+            dphi = dev<double>(ctx->phi, N);
+            h2d(dphi, phi, sizeof(double) * N, ctx->stream);
+            dh = dev<double>(ctx->hphi, N);
+        }
+        const size_t nn = (size_t)N * N;
+        double* dG = dev<double>(ctx->G64, nn);
+        double* dH = H ? dev<double>(ctx->H, nn) : nullptr;
+        assemble_device(ctx, topo, dx, dphi, flags, AsmOut{dG, nullptr, dH, dh});
+        d2h(G, dG, sizeof(double) * nn, ctx->stream);
+        if (H) d2h(H, dH, sizeof(double) * nn, ctx->stream);
+        if (dh) d2h(Hphi, dh, sizeof(double) * N, ctx->stream);
+    });
+}
+
+int bemhip_solve(bemhip_ctx* ctx, int64_t N, const double* G, const double* b, double* x, int solver, double tol,
+                 int32_t maxit, bemhip_solve_stats* stats) {
+    if (!ctx) return BEMHIP_ERR_ARG;
+    return guarded(ctx, [&] {
+        BH_CHECK(G && b && x && N > 0 && N <= 46340, BEMHIP_ERR_ARG, "bad arguments");
+        check_finite(b, N, "b");
+        const size_t nn = (size_t)N * N;
+        double* dG = dev<double>(ctx->G64, nn);
+        h2d(dG, G, sizeof(double) * nn, ctx->stream);
+        float* dG32 = nullptr;
+        if (mixed(solver)) {
+            dG32 = dev<float>(ctx->G32, nn);
+            k_d2f<<<4096, 256, 0, ctx->stream>>>(nn, dG, dG32);
+            BH_HIP(hipGetLastError());
+        }
+        double* db = dev<double>(ctx->vec[10], N);
+        double* dx = dev<double>(ctx->vec[11], N);
+        h2d(db, b, sizeof(double) * N, ctx->stream);
+        bemhip_solve_stats st{};
+        solve_device(ctx, (int)N, dG, dG32, db, dx, solver, tol, maxit, &st);
+        d2h(x, dx, sizeof(double) * N, ctx->stream);
+        if (stats) *stats = st;
+        check_finite(x, N, "solution");
+        if (!st.converged) throw Error(BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+    });
+}
+
+int bemhip_compute_psi(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                       const double* phi, uint32_t flags, int solver, double tol, int32_t maxit, double* psi,
+                       bemhip_solve_stats* stats) {
+    if (!ctx) return BEMHIP_ERR_ARG;
+    return guarded(ctx, [&] {
+        BH_CHECK(xyz && tri && phi && psi, BEMHIP_ERR_ARG, "null argument");
+        check_finite(phi, N, "phi");
+        Topology th = build_topology(N, xyz, M, tri);
+        DevTopology topo;
+        topo.upload(th, ctx->stream);
+        double* dx = dev<double>(ctx->xyz, 3 * N);
+        double* dphi = dev<double>(ctx->phi, N);
+        double* dh = dev<double>(ctx->hphi, N);
+        double* dpsi = dev<double>(ctx->vec[11], N);
+        h2d(dx, xyz, sizeof(double) * 3 * N, ctx->stream);
+        h2d(dphi, phi, sizeof(double) * N, ctx->stream);
+        bemhip_solve_stats st{};
+        psi_device(ctx, topo, dx, dphi, flags, solver, tol, maxit, dh, dpsi, &st);
+        d2h(psi, dpsi, sizeof(double) * N, ctx->stream);
+        if (stats) *stats = st;
+        check_finite(psi, N, "psi");
+        if (!st.converged) throw Error(BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+    });
+}
+
+void bemhip_sim_default_params(bemhip_sim_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->p_inf = 1.0; p->epsilon = 1.0; p->sigma = 0.0; p->gamma = 1.0;   // Simulation.hpp:29
+    p->V_0 = std::numeric_limits<double>::quiet_NaN();
+    p->dp_balance = 3.0;                                                // Simulation.hpp:37
+    p->min_dt = -1.0;                                                   // Simulation.hpp:36
+    p->wall_eps = 1e-2;                                                 // LinLinSim.hpp:24
+    p->tol = 0.0;
+    p->maxit = 0;
+    p->solver = BEMHIP_SOLVE_BICGSTAB64;                                // Simulation.hpp:38
+    p->flags = 0;
+    p->wall_guards = 0;
+    p->frozen_stage_time = 1;
+}
+
+int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, const double* xyz, int64_t M,
+                      const uint32_t* tri, bemhip_sim** out) {
+    if (!ctx || !out) return BEMHIP_ERR_ARG;
+    *out = nullptr;
+    auto s = std::make_unique<bemhip_sim>();
+    int rc = guarded(ctx, [&] {
+        BH_CHECK(p && xyz && tri, BEMHIP_ERR_ARG, "null argument");
+        BH_CHECK(!(p->flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown flag");
+        BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
+        s->ctx = ctx;
+        s->P = *p;
+        s->N = N;
+        s->M = M;
+        s->th = build_topology(N, xyz, M, tri);
+        s->topo.upload(s->th, ctx->stream);
+        double* x = dev<double>(s->x, 3 * N);
+        h2d(x, xyz, sizeof(double) * 3 * N, ctx->stream);
+        BH_HIP(hipMemsetAsync(dev<double>(s->phi, N), 0, sizeof(double) * N, ctx->stream));   // LinLinSim.hpp:28-29
+        BH_HIP(hipMemsetAsync(dev<double>(s->psi, N), 0, sizeof(double) * N, ctx->stream));
+        if (std::isnan(s->P.V_0)) {   // Simulation.hpp:42: total volume of the initial mesh
+            double* tvol = dev<double>(s->tvol, M);
+            double* pvol = dev<double>(s->pvol, s->topo.nparts);
+            tri_geometry(ctx, s->topo, x, nullptr, nullptr, nullptr, tvol);
+            part_volumes(ctx, s->topo, tvol, pvol);
+            std::vector<double> pv(s->topo.nparts);
+            d2h(pv.data(), pvol, sizeof(double) * pv.size(), ctx->stream);
+            double v = 0.0;
+            for (double a : pv) v += a;
+            s->P.V_0 = v;
+        }
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (rc != BEMHIP_OK) return rc;
+    *out = s.release();
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_destroy(bemhip_sim* s) {
+    if (!s) return BEMHIP_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_set_phi(bemhip_sim* s, const double* phi) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(phi, BEMHIP_ERR_ARG, "null phi");
+        check_finite(phi, s->N, "phi");
+        h2d(s->phi.p, phi, sizeof(double) * s->N, s->ctx->stream);
+        BH_HIP(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+int bemhip_sim_set_psi(bemhip_sim* s, const double* psi) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(psi, BEMHIP_ERR_ARG, "null psi");
+        h2d(s->psi.p, psi, sizeof(double) * s->N, s->ctx->stream);
+        BH_HIP(hipStreamSynchronize(s->ctx->stream));
+    });
+}
+
+int bemhip_sim_set_V_0(bemhip_sim* s, double v) {
+    if (!s) return BEMHIP_ERR_ARG;
+    s->P.V_0 = v;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_set_time(bemhip_sim* s, double t) {
+    if (!s) return BEMHIP_ERR_ARG;
+    s->time = t;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_set_solver(bemhip_sim* s, int solver, double tol, int32_t maxit) {
+    if (!s) return BEMHIP_ERR_ARG;
+    if (solver < 0 || solver > 3) { s->ctx->err = "unknown solver"; return BEMHIP_ERR_ARG; }
+    s->P.solver = solver;
+    s->P.tol = tol;
+    s->P.maxit = maxit;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_get_state(bemhip_sim* s, double* xyz, double* phi, double* psi, double* time) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        if (xyz) d2h(xyz, s->x.p, sizeof(double) * 3 * s->N, s->ctx->stream);
+        if (phi) d2h(phi, s->phi.p, sizeof(double) * s->N, s->ctx->stream);
+        if (psi) d2h(psi, s->psi.p, sizeof(double) * s->N, s->ctx->stream);
+        if (time) *time = s->time;
+    });
+}
+
+int bemhip_sim_get_volume(bemhip_sim* s, double* vol) {
+    if (!s || !vol) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        double* tvol = dev<double>(s->tvol, s->M);
+        double* pvol = dev<double>(s->pvol, s->topo.nparts);
+        tri_geometry(s->ctx, s->topo, s->x.as<double>(), nullptr, nullptr, nullptr, tvol);
+        part_volumes(s->ctx, s->topo, tvol, pvol);
+        std::vector<double> pv(s->topo.nparts);
+        d2h(pv.data(), pvol, sizeof(double) * pv.size(), s->ctx->stream);
+        double v = 0.0;
+        for (double a : pv) v += a;
+        *vol = v;
+    });
+}
+
+int bemhip_sim_position_t(bemhip_sim* s, double* vel, double* psi) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        double* kx = dev<double>(s->kx, 12 * s->N);
+        stage(s, s->x.as<double>(), s->phi.as<double>(), s->time, kx, nullptr);
+        if (vel) d2h(vel, kx, sizeof(double) * 3 * s->N, s->ctx->stream);
+        if (psi) d2h(psi, s->psil.p, sizeof(double) * s->N, s->ctx->stream);
+        BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+    });
+}
+
+int bemhip_sim_pot_t_multi(bemhip_sim* s, const double* vel, double t, double* dpot) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(vel && dpot, BEMHIP_ERR_ARG, "null argument");
+        const int64_t N = s->N, M = s->M;
+        double* kx = dev<double>(s->kx, 12 * N);
+        double* kp = dev<double>(s->kp, 4 * N);
+        h2d(kx, vel, sizeof(double) * 3 * N, s->ctx->stream);
+        double* tvol = dev<double>(s->tvol, M);
+        double* pvol = dev<double>(s->pvol, s->topo.nparts);
+        tri_geometry(s->ctx, s->topo, s->x.as<double>(), nullptr, nullptr, nullptr, tvol);
+        part_volumes(s->ctx, s->topo, tvol, pvol);
+        double* tk = nullptr;
+        if (s->P.sigma != 0.0) {
+            double* vn = dev<double>(s->vn, 3 * N);
+            tk = dev<double>(s->tk, M);
+            vertex_normals(s->ctx, s->topo, s->x.as<double>(), vn);
+            tri_curvature(s->ctx, s->topo, s->x.as<double>(), vn, tk);
+        }
+        vertex_dpot(s->ctx, s->topo, stage_params(s), s->x.as<double>(), kx, tk, pvol, t, kp);
+        d2h(dpot, kp, sizeof(double) * N, s->ctx->stream);
+    });
+}
+
+int bemhip_sim_compute_psi(bemhip_sim* s) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        double* hphi = dev<double>(s->hphi, s->N);
+        psi_device(s->ctx, s->topo, s->x.as<double>(), s->phi.as<double>(), s->P.flags, s->P.solver, s->P.tol,
+                   s->P.maxit, hphi, s->psi.as<double>(), &s->last);
+        accumulate(s->accum, s->last);
+        BH_HIP(hipStreamSynchronize(s->ctx->stream));
+        BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+    });
+}
+
+int bemhip_sim_evolve_rk4(bemhip_sim* s, double dp, int fixdt, double* dt_out) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        bemhip_ctx* ctx = s->ctx;
+        const int64_t N = s->N;
+        const size_t n3 = 3 * (size_t)N;
+        if (s->P.wall_guards && any_negative_x(ctx, N, s->x.as<double>()))   // LinLinSim.cpp:198
+            throw Error(BEMHIP_ERR_NEG_X_IN_MIRROR, "negative values");
+        double* x = s->x.as<double>();
+        double* phi = s->phi.as<double>();
+        double* x1 = dev<double>(s->x1, n3);
+        double* p1 = dev<double>(s->p1, N);
+        double* xs = dev<double>(s->xs, n3);
+        double* ps = dev<double>(s->ps, N);
+        double* kx = dev<double>(s->kx, 4 * n3);
+        double* kp = dev<double>(s->kp, 4 * (size_t)N);
+        BH_HIP(hipMemcpyAsync(x1, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+        BH_HIP(hipMemcpyAsync(p1, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+        const double t0 = s->time;
+        bool conv = true;
+        stage(s, x1, p1, t0, kx, kp);
+        conv = conv && s->last.converged;
+        const double dt = fixdt ? dp : get_dt(s, dp, kx, kp);
+        const bool frozen = s->P.frozen_stage_time != 0;
+        const double cs[3] = {0.5 * dt, 0.5 * dt, dt};
+        const double ts[3] = {frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + dt};
+        for (int k = 1; k < 4; ++k) {
+            axpy_state(ctx, N, cs[k - 1], kx + (k - 1) * n3, x1, xs, kp + (size_t)(k - 1) * N, p1, ps);
+            stage(s, xs, ps, ts[k - 1], kx + k * n3, kp + (size_t)k * N);
+            conv = conv && s->last.converged;
+        }
+        double* avg = dev<double>(s->avg, n3);
+        rk4_final(ctx, N, dt, kx, kp, x1, x, phi, avg, s->P.wall_guards, s->P.wall_eps);
+        double* vn = dev<double>(s->vn, n3);
+        vertex_normals(ctx, s->topo, x, vn);
+        psi_project(ctx, N, avg, vn, s->psi.as<double>());
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        s->time = t0 + dt;
+        if (dt_out) *dt_out = dt;
+        BH_CHECK(conv, BEMHIP_ERR_NOT_CONVERGED, "a stage solve stopped above tolerance");
+        if (s->P.wall_guards && any_negative_x(ctx, N, x))   // LinLinSim.cpp:256
+            throw Error(BEMHIP_ERR_NEG_X_IN_MIRROR, "negative values");
+    });
+}
+
+int bemhip_sim_evolve_euler(bemhip_sim* s, double dp, int fixdt, double* dt_out) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        bemhip_ctx* ctx = s->ctx;
+        const int64_t N = s->N;
+        double* kx = dev<double>(s->kx, 12 * (size_t)N);
+        double* kp = dev<double>(s->kp, 4 * (size_t)N);
+        stage(s, s->x.as<double>(), s->phi.as<double>(), s->time, kx, kp);
+        const double dt = fixdt ? dp : get_dt(s, dp, kx, kp);
+        euler_update(ctx, N, dt, kx, kp, s->x.as<double>(), s->phi.as<double>());
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        s->time += dt;
+        if (dt_out) *dt_out = dt;
+        BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+    });
+}
+
+int bemhip_sim_last_stats(bemhip_sim* s, bemhip_solve_stats* st) {
+    if (!s || !st) return BEMHIP_ERR_ARG;
+    *st = s->last;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_accum_stats(bemhip_sim* s, bemhip_solve_stats* st, int reset) {
+    if (!s) return BEMHIP_ERR_ARG;
+    if (st) *st = s->accum;
+    if (reset) s->accum = bemhip_solve_stats{};
+    return BEMHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,289 @@
+// Collocation assembly of the single-layer (G) and double-layer (H) matrices
+// for linear elements on flat triangles, gfx950.
+//
+// Reference semantics: ColocSim::assemble_matrices (ColocSim.cpp:21-110) with
+// integrate_Lin_coloc_local[_mir] (Integrator.cpp:237-283): for every
+// collocation vertex i and triangle T, 3 G and 3 H panel integrals are added to
+// the columns of T's vertices; i in T uses the polar self-panel rule for G and
+// H = 0 (plus the image panel in wall mode); then every H row is closed by the
+// 4-pi rule H(i,i) = -4 pi - sum_{j != i} H(i,j).
+//
+// Device design:
+//   k_tri_prep   one thread per triangle: the 7 quadrature points, unit normal,
+//                Jacobian (|ab x bc|) and first vertex, 32 doubles per triangle.
+//   k_far        work-group = 128 collocation rows (one per lane) x a group of
+//                column patches.  A patch is 32 mesh-adjacent vertices; the
+//                work-group streams the triangles incident to the patch (wave-
+//                uniform, scalar loads) and accumulates each lane's 32 G (and
+//                optionally H) columns in LDS, private per lane, so no atomics
+//                and a fixed summation order.  A flushed patch column is a
+//                coalesced 1 KiB store per wave.  H is never stored unless
+//                asked: H phi is reduced on the fly as
+//                sum_k H_k (phi_k - phi_i), which equals the 4-pi-closed
+//                product (H phi)_i + 4 pi phi_i exactly in real arithmetic.
+//   k_self       one thread per vertex: adds the polar-rule self panels to its
+//                own row, finishes H phi (and H's diagonal when stored).
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+#include "quad_rules.hpp"
+
+namespace bemhip {
+
+constexpr int TQ_STRIDE = 32;  // doubles per triangle record
+
+struct QW {
+    double c[7][3];
+};
+
+// w_q * phi_k(q), phi = (1 - x - y, x, y): basis of get_linear_elements
+// (ResultTypes.hpp:111-117) at u = x + y, v = y.
+__host__ __device__ constexpr double basis_w(int q, int k) {
+    return TQ_W[q] * (k == 0 ? (1.0 - (TQ_X[q] + TQ_Y[q])) : (k == 1 ? TQ_X[q] : TQ_Y[q]));
+}
+
+__global__ __launch_bounds__(256) void k_tri_prep(int M, const double* __restrict__ xyz,
+                                                  const uint32_t* __restrict__ tri,
+                                                  double* __restrict__ triq) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const uint32_t ia = tri[3 * j], ib = tri[3 * j + 1], ic = tri[3 * j + 2];
+    const double ax = xyz[3 * ia], ay = xyz[3 * ia + 1], az = xyz[3 * ia + 2];
+    const double bx = xyz[3 * ib], by = xyz[3 * ib + 1], bz = xyz[3 * ib + 2];
+    const double cx = xyz[3 * ic], cy = xyz[3 * ic + 1], cz = xyz[3 * ic + 2];
+    // Interpolator (Interpolator.hpp:15-21): ab, bc, n = ab x bc, area_ = |n|
+    const double abx = bx - ax, aby = by - ay, abz = bz - az;
+    const double bcx = cx - bx, bcy = cy - by, bcz = cz - bz;
+    double nx = aby * bcz - abz * bcy, ny = abz * bcx - abx * bcz, nz = abx * bcy - aby * bcx;
+    const double J = sqrt(nx * nx + ny * ny + nz * nz);
+    const double rj = 1.0 / J;
+    nx *= rj; ny *= rj; nz *= rj;
+    double* o = triq + (size_t)TQ_STRIDE * j;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const double u = TQ_X[q] + TQ_Y[q], v = TQ_Y[q];
+        o[3 * q + 0] = ax + (u * abx + v * bcx);
+        o[3 * q + 1] = ay + (u * aby + v * bcy);
+        o[3 * q + 2] = az + (u * abz + v * bcz);
+    }
+    o[21] = nx; o[22] = ny; o[23] = nz;
+    o[24] = J;
+    o[25] = ax; o[26] = ay; o[27] = az;
+    o[28] = 0.0; o[29] = 0.0; o[30] = 0.0; o[31] = 0.0;
+}
+
+template <bool MIRROR, bool STORE_H, bool WRITE32>
+__global__ __launch_bounds__(ASM_ROWS) void k_far(
+    int N, int npatch, int patches_per_group, const double* __restrict__ xyz, const double* __restrict__ phi,
+    const double* __restrict__ triq, const uint32_t* __restrict__ tri, const int32_t* __restrict__ pptr,
+    const uint32_t* __restrict__ ptri, const uint32_t* __restrict__ pslots, const int32_t* __restrict__ pcol,
+    double* __restrict__ G, float* __restrict__ G32, double* __restrict__ H, double* __restrict__ part_hphi,
+    double* __restrict__ part_hrow) {
+    __shared__ double accG[PATCH][ASM_ROWS];
+    __shared__ double accH[STORE_H ? PATCH : 1][ASM_ROWS];
+    const int tid = threadIdx.x;
+    const int row = blockIdx.x * ASM_ROWS + tid;
+    const bool valid = row < N;
+    const int rr = valid ? row : N - 1;
+    const double xi = xyz[3 * rr], yi = xyz[3 * rr + 1], zi = xyz[3 * rr + 2];
+    const double phii = phi ? phi[rr] : 0.0;
+    double hphi = 0.0, hrow = 0.0;
+
+    const int p0 = blockIdx.y * patches_per_group;
+    const int p1 = min(npatch, p0 + patches_per_group);
+    for (int p = p0; p < p1; ++p) {
+#pragma unroll
+        for (int s = 0; s < PATCH; ++s) {
+            accG[s][tid] = 0.0;
+            if constexpr (STORE_H) accH[s][tid] = 0.0;
+        }
+        const int e0 = pptr[p], e1 = pptr[p + 1];
+        for (int e = e0; e < e1; ++e) {
+            const uint32_t t = ptri[e];
+            const uint32_t sl = pslots[e];
+            const double* __restrict__ q = triq + (size_t)TQ_STRIDE * t;
+            const uint32_t v0 = tri[3 * t], v1 = tri[3 * t + 1], v2 = tri[3 * t + 2];
+            const double nx = q[21], ny = q[22], nz = q[23], J = q[24];
+            const double dy = q[26] - yi, dz = q[27] - zi;
+            const double zn = (q[25] - xi) * nx + dy * ny + dz * nz;
+            double g0 = 0, g1 = 0, g2 = 0, s0 = 0, s1 = 0, s2 = 0;
+            double gm0 = 0, gm1 = 0, gm2 = 0, sm0 = 0, sm1 = 0, sm2 = 0;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const double zx = q[3 * k] - xi, zy = q[3 * k + 1] - yi, zz = q[3 * k + 2] - zi;
+                const double yz2 = zy * zy + zz * zz;
+                const double inv = rsqrt(zx * zx + yz2);
+                const double inv3 = inv * inv * inv;
+                g0 += basis_w(k, 0) * inv; g1 += basis_w(k, 1) * inv; g2 += basis_w(k, 2) * inv;
+                s0 += basis_w(k, 0) * inv3; s1 += basis_w(k, 1) * inv3; s2 += basis_w(k, 2) * inv3;
+                if constexpr (MIRROR) {
+                    const double zxm = -q[3 * k] - xi;
+                    const double invm = rsqrt(zxm * zxm + yz2);
+                    const double invm3 = invm * invm * invm;
+                    gm0 += basis_w(k, 0) * invm; gm1 += basis_w(k, 1) * invm; gm2 += basis_w(k, 2) * invm;
+                    sm0 += basis_w(k, 0) * invm3; sm1 += basis_w(k, 1) * invm3; sm2 += basis_w(k, 2) * invm3;
+                }
+            }
+            // i in T: the direct part is the self panel (added by k_self), H = 0
+            const bool far = ((uint32_t)row != v0) & ((uint32_t)row != v1) & ((uint32_t)row != v2);
+            double G0 = far ? g0 : 0.0, G1 = far ? g1 : 0.0, G2 = far ? g2 : 0.0;
+            double H0 = far ? -zn * s0 : 0.0, H1 = far ? -zn * s1 : 0.0, H2 = far ? -zn * s2 : 0.0;
+            if constexpr (MIRROR) {
+                // image point R y (R: x -> -x) with normal R n: (Ry - x).(Rn) = (a - Rx).n
+                const double znm = (q[25] + xi) * nx + dy * ny + dz * nz;
+                G0 += gm0; G1 += gm1; G2 += gm2;
+                H0 += -znm * sm0; H1 += -znm * sm1; H2 += -znm * sm2;
+            }
+            G0 *= J; G1 *= J; G2 *= J;
+            H0 *= J; H1 *= J; H2 *= J;
+            const uint32_t sa = sl & 0xFFu, sb = (sl >> 8) & 0xFFu, sc = (sl >> 16) & 0xFFu;
+            if (sa != 0xFFu) {
+                accG[sa][tid] += G0;
+                if constexpr (STORE_H) accH[sa][tid] += H0;
+                hphi += H0 * (phi ? phi[v0] - phii : 0.0);
+                hrow += H0;
+            }
+            if (sb != 0xFFu) {
+                accG[sb][tid] += G1;
+                if constexpr (STORE_H) accH[sb][tid] += H1;
+                hphi += H1 * (phi ? phi[v1] - phii : 0.0);
+                hrow += H1;
+            }
+            if (sc != 0xFFu) {
+                accG[sc][tid] += G2;
+                if constexpr (STORE_H) accH[sc][tid] += H2;
+                hphi += H2 * (phi ? phi[v2] - phii : 0.0);
+                hrow += H2;
+            }
+        }
+        if (valid) {
+            const int32_t* cols = pcol + (size_t)p * PATCH;
+#pragma unroll 4
+            for (int s = 0; s < PATCH; ++s) {
+                const int32_t c = cols[s];
+                if (c < 0) continue;
+                const size_t off = (size_t)c * (size_t)N + (size_t)row;
+                const double gv = accG[s][tid];
+                if (G) G[off] = gv;
+                if constexpr (WRITE32) G32[off] = (float)gv;
+                if constexpr (STORE_H) H[off] = accH[s][tid];
+            }
+        }
+    }
+    if (valid) {
+        part_hphi[(size_t)blockIdx.y * N + row] = hphi;
+        part_hrow[(size_t)blockIdx.y * N + row] = hrow;
+    }
+}
+
+// integrate_identical_coloc<LinElm> (Integrator.cpp:12-41): x = a, polar rule
+// theta in (0, pi/4) with the 7-point Gauss rule.
+__device__ inline void self_panel(double abx, double aby, double abz, double bcx, double bcy, double bcz,
+                                  double J, double& t0, double& t1, double& t2) {
+    const double jac = 0.5 * J * (PI / 4.0);
+    t0 = t1 = t2 = 0.0;
+#pragma unroll
+    for (int p = 0; p < 7; ++p) {
+        const double th = GQ_X[p] * (PI / 4.0);
+        const double c = cos(th), s = sin(th);
+        const double dx = c * abx + s * bcx, dy = c * aby + s * bcy, dz = c * abz + s * bcz;
+        const double f = jac * GQ_W[p] * rsqrt(dx * dx + dy * dy + dz * dz);
+        const double fac = 1.0 / (c * c);
+        t0 += f * c * fac;
+        t1 += f * (c - s) * fac;
+        t2 += f * s * fac;
+    }
+}
+
+template <bool STORE_H, bool WRITE32>
+__global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* __restrict__ xyz,
+                                              const double* __restrict__ phi, const uint32_t* __restrict__ tri,
+                                              const int32_t* __restrict__ vptr, const int32_t* __restrict__ vtri,
+                                              double* __restrict__ G, float* __restrict__ G32,
+                                              double* __restrict__ H, const double* __restrict__ part_hphi,
+                                              const double* __restrict__ part_hrow, double* __restrict__ hphi) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const double ax = xyz[3 * v], ay = xyz[3 * v + 1], az = xyz[3 * v + 2];
+    for (int e = vptr[v]; e < vptr[v + 1]; ++e) {
+        const uint32_t* t = tri + 3 * (size_t)vtri[e];
+        // cyclic_reorder (Triplet.hpp:55-72): v first
+        const int s = (t[0] == (uint32_t)v) ? 0 : ((t[1] == (uint32_t)v) ? 1 : 2);
+        const uint32_t ib = t[(s + 1) % 3], ic = t[(s + 2) % 3];
+        const double bx = xyz[3 * ib], by = xyz[3 * ib + 1], bz = xyz[3 * ib + 2];
+        const double cx = xyz[3 * ic], cy = xyz[3 * ic + 1], cz = xyz[3 * ic + 2];
+        const double abx = bx - ax, aby = by - ay, abz = bz - az;
+        const double bcx = cx - bx, bcy = cy - by, bcz = cz - bz;
+        const double nx = aby * bcz - abz * bcy, ny = abz * bcx - abx * bcz, nz = abx * bcy - aby * bcx;
+        const double J = sqrt(nx * nx + ny * ny + nz * nz);
+        double g0, g1, g2;
+        self_panel(abx, aby, abz, bcx, bcy, bcz, J, g0, g1, g2);
+        const uint32_t cols[3] = {(uint32_t)v, ib, ic};
+        const double gv[3] = {g0, g1, g2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const size_t off = (size_t)cols[k] * (size_t)N + (size_t)v;
+            double val = gv[k];
+            if (G) { val += G[off]; G[off] = val; }
+            else val += (double)G32[off];
+            if constexpr (WRITE32) G32[off] = (float)val;
+        }
+    }
+    double hp = 0.0, hr = 0.0;
+    for (int g = 0; g < ngroups; ++g) {
+        hp += part_hphi[(size_t)g * N + v];
+        hr += part_hrow[(size_t)g * N + v];
+    }
+    const double four_pi = 4.0 * PI;
+    if (hphi) hphi[v] = hp - four_pi * (phi ? phi[v] : 0.0);
+    if constexpr (STORE_H) {
+        const size_t d = (size_t)v * (size_t)N + (size_t)v;
+        H[d] = H[d] - four_pi - hr;  // ColocSim.cpp:96 with val_H = -sum_j H(i,j)
+    }
+}
+
+void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi,
+                     uint32_t flags, const AsmOut& out) {
+    const int N = (int)topo.N, M = (int)topo.M;
+    BH_CHECK(out.G64 || out.G32, BEMHIP_ERR_ARG, "assembly needs a G output");
+    hipStream_t st = ctx->stream;
+    double* triq = ctx->triq.ensure(sizeof(double) * TQ_STRIDE * (size_t)M) ? ctx->triq.as<double>() : nullptr;
+    k_tri_prep<<<(M + 255) / 256, 256, 0, st>>>(M, d_xyz, topo.tri.as<uint32_t>(), triq);
+    BH_HIP(hipGetLastError());
+
+    const int rowblocks = (N + ASM_ROWS - 1) / ASM_ROWS;
+    int groups = (4096 + rowblocks - 1) / rowblocks;
+    groups = std::max(1, std::min(groups, topo.npatch));
+    const int ppg = (topo.npatch + groups - 1) / groups;
+    groups = (topo.npatch + ppg - 1) / ppg;
+    double* ph = static_cast<double*>(ctx->part_hphi.ensure(sizeof(double) * (size_t)groups * N));
+    double* hr = static_cast<double*>(ctx->part_hrow.ensure(sizeof(double) * (size_t)groups * N));
+    const bool mirror = flags & BEMHIP_MIRROR_X0;
+    const bool storeH = out.H != nullptr;
+    const bool w32 = out.G32 != nullptr;
+    dim3 grid(rowblocks, groups);
+#define BH_FAR(MI, SH, W3)                                                                              \
+    k_far<MI, SH, W3><<<grid, ASM_ROWS, 0, st>>>(                                                      \
+        N, topo.npatch, ppg, d_xyz, d_phi, triq, topo.tri.as<uint32_t>(), topo.patch_ptr.as<int32_t>(), \
+        topo.patch_tri.as<uint32_t>(), topo.patch_slots.as<uint32_t>(), topo.patch_col.as<int32_t>(),   \
+        out.G64, out.G32, out.H, ph, hr)
+    if (mirror) {
+        if (storeH) { if (w32) BH_FAR(true, true, true); else BH_FAR(true, true, false); }
+        else { if (w32) BH_FAR(true, false, true); else BH_FAR(true, false, false); }
+    } else {
+        if (storeH) { if (w32) BH_FAR(false, true, true); else BH_FAR(false, true, false); }
+        else { if (w32) BH_FAR(false, false, true); else BH_FAR(false, false, false); }
+    }
+#undef BH_FAR
+    BH_HIP(hipGetLastError());
+#define BH_SELF(SH, W3)                                                                             \
+    k_self<SH, W3><<<(N + 255) / 256, 256, 0, st>>>(N, groups, d_xyz, d_phi, topo.tri.as<uint32_t>(), \
+                                                    topo.vptr.as<int32_t>(), topo.vtri.as<int32_t>(), \
+                                                    out.G64, out.G32, out.H, ph, hr, out.hphi)
+    if (storeH) { if (w32) BH_SELF(true, true); else BH_SELF(true, false); }
+    else { if (w32) BH_SELF(false, true); else BH_SELF(false, false); }
+#undef BH_SELF
+    BH_HIP(hipGetLastError());
+}
+
+}  // namespace bemhip

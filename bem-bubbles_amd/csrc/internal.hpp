@@ -1,0 +1,179 @@
+// Internal declarations of libbemhip (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bemhip.h"
+
+namespace bemhip {
+
+// ---------------------------------------------------------------- errors --
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define BH_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::bemhip::Error(e_ == hipErrorOutOfMemory ? BEMHIP_ERR_OOM : BEMHIP_ERR_HIP, \
+                                  std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+#define BH_BLAS(expr)                                                                 \
+    do {                                                                              \
+        rocblas_status s_ = (expr);                                                   \
+        if (s_ != rocblas_status_success)                                             \
+            throw ::bemhip::Error(BEMHIP_ERR_HIP, std::string(#expr) + ": " +         \
+                                                      rocblas_status_to_string(s_));  \
+    } while (0)
+
+#define BH_CHECK(cond, code, msg)                          \
+    do {                                                   \
+        if (!(cond)) throw ::bemhip::Error((code), (msg)); \
+    } while (0)
+
+// ------------------------------------------------------- device buffers --
+// Grow-only device allocation (contents are not preserved on growth).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void* ensure(size_t n) {
+        if (n <= bytes && p) return p;
+        release();
+        if (n == 0) n = 256;
+        BH_HIP(hipMalloc(&p, n));
+        bytes = n;
+        return p;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ----------------------------------------------------- mesh topology --
+// Host-built, topology-only tables (rebuilt when the connectivity changes).
+constexpr int PATCH = 32;          // columns per assembly patch
+constexpr int ASM_ROWS = 128;      // collocation rows per assembly work-group
+
+struct Topology {
+    int64_t N = 0, M = 0;
+    int npatch = 0;
+    std::vector<uint32_t> tri;             // M x 3
+    // assembly patches: vertex -> (patch, slot); per patch, the incident triangles
+    std::vector<int32_t> patch_col;        // npatch * PATCH vertex ids (-1 = empty slot)
+    std::vector<int32_t> patch_ptr;        // npatch + 1
+    std::vector<uint32_t> patch_tri;       // entries: triangle id
+    std::vector<uint32_t> patch_slots;     // entries: 3 x 8-bit slot (0xFF = other patch)
+    // vertex -> incident triangles (ascending), Mesh.cpp:75-84
+    std::vector<int32_t> vptr, vtri;
+    // loose parts (Mesh.cpp:382-440): per vertex part id; per part triangle list
+    int nparts = 0;
+    std::vector<int32_t> vpart;
+    std::vector<int32_t> part_tptr, part_tri;
+    double redundancy = 0.0;               // patch entries / M
+};
+
+Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri);
+
+// Device copies of the topology tables
+struct DevTopology {
+    int64_t N = 0, M = 0;
+    int npatch = 0, nparts = 0;
+    DevBuf tri, patch_col, patch_ptr, patch_tri, patch_slots, vptr, vtri, vpart, part_tptr, part_tri;
+    void upload(const Topology& t, hipStream_t s);
+};
+
+// ------------------------------------------------------------- context --
+struct LuWork {
+    DevBuf ipiv, panel_ws, flag;
+};
+
+}  // namespace bemhip
+
+struct bemhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+    int cus = 0;
+    // workspace
+    bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H, vec[12], scal, gemv_part;
+    bemhip::LuWork lu;
+    bemhip::DevBuf split_a, split_b, split_mx, perm;
+    std::vector<int> h_ipiv, h_perm;
+    double* host_scal = nullptr;  // pinned
+    // instrumentation of trailing updates
+    bool timing = true;
+    hipEvent_t ev[4] = {};
+    double gemm_ms = 0.0, gemm_flops = 0.0;
+    int gemm_launches = 0;
+};
+
+namespace bemhip {
+
+// ------------------------------------------------------------ assembly --
+struct AsmOut {
+    double* G64;     // may be null only when G32 is given
+    float* G32;      // optional FP32 copy
+    double* H;       // optional
+    double* hphi;    // optional (needs phi)
+};
+void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi,
+                     uint32_t flags, const AsmOut& out);
+
+// ---------------------------------------------------------- dense LA --
+// LU with partial pivoting (column-major, lda = N), pivots as absolute rows.
+template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16 = false);
+// x (double, device) <- (LU)^-1 P x with LU in precision T
+template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int* ipiv, double* x);
+// r = b - A x (A FP64 column-major), or y = A x when b == nullptr
+void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const double* b, double* r);
+// solver front end on a device-resident FP64 G (destroyed by LU64) and rhs b.
+void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double* b, double* x,
+                  int solver, double tol, int maxit, bemhip_solve_stats* st);
+// device reductions
+double dev_absmax(bemhip_ctx* ctx, int n, const double* x);
+double dev_dot(bemhip_ctx* ctx, int n, const double* x, const double* y);
+
+// ------------------------------------------------------------- stage --
+struct StageParams {
+    double p_inf, epsilon, sigma, gamma, V_0;
+    double field_amp, field_freq, field_phase;
+};
+void tri_geometry(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, const double* pot,
+                  double* tnorm, double* tgrad, double* tvol);
+void vertex_normals(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, double* vn);
+void tri_curvature(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, const double* vn,
+                   double* tkappa);
+void part_volumes(bemhip_ctx* ctx, const DevTopology& topo, const double* tvol, double* pvol);
+void vertex_velocity(bemhip_ctx* ctx, const DevTopology& topo, const double* tnorm, const double* tgrad,
+                     const double* psi, double* vel);
+void vertex_dpot(bemhip_ctx* ctx, const DevTopology& topo, const StageParams& sp, const double* xyz,
+                 const double* vel, const double* tkappa, const double* pvol, double t, double* dpot);
+void axpy_state(bemhip_ctx* ctx, int64_t N, double c, const double* kx, const double* x1, double* xs,
+                const double* kp, const double* p1, double* ps);
+void rk4_final(bemhip_ctx* ctx, int64_t N, double dt, const double* kx, const double* kp,
+               const double* x1, double* x, double* phi, double* avg, int wall_guards, double wall_eps);
+void psi_project(bemhip_ctx* ctx, int64_t N, const double* avg, const double* vn, double* psi);
+void euler_update(bemhip_ctx* ctx, int64_t N, double dt, const double* vel, const double* dpot,
+                  double* x, double* phi);
+void dt_reduce(bemhip_ctx* ctx, int64_t N, const double* vel, const double* dpot, double* out2);
+int any_negative_x(bemhip_ctx* ctx, int64_t N, const double* xyz);
+
+}  // namespace bemhip

@@ -1,0 +1,194 @@
+// Host-side topology tables of a mesh: assembly patches, vertex incidence and
+// loose parts.  They depend on connectivity only (positions merely seed the
+// patch ordering), so they are built once per mesh and reused every stage.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace bemhip {
+
+static uint32_t spread10(uint32_t v) {
+    v &= 0x3FF;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri) {
+    BH_CHECK(N > 0 && M > 0, BEMHIP_ERR_ARG, "empty mesh");
+    BH_CHECK(N < (int64_t)1 << 31 && 3 * M < (int64_t)1 << 31, BEMHIP_ERR_ARG, "mesh too large");
+    Topology T;
+    T.N = N;
+    T.M = M;
+    T.tri.assign(tri, tri + 3 * M);
+    for (int64_t j = 0; j < M; ++j) {
+        const uint32_t a = tri[3 * j], b = tri[3 * j + 1], c = tri[3 * j + 2];
+        BH_CHECK(a < N && b < N && c < N, BEMHIP_ERR_BAD_MESH,
+                 "triangle " + std::to_string(j) + " has a vertex index out of range");
+        BH_CHECK(a != b && b != c && a != c, BEMHIP_ERR_BAD_MESH,
+                 "triangle " + std::to_string(j) + " repeats a vertex");
+    }
+    for (int64_t i = 0; i < 3 * N; ++i)
+        BH_CHECK(std::isfinite(xyz[i]), BEMHIP_ERR_NAN, "non-finite vertex coordinate");
+
+    // vertex -> triangles, ascending (Mesh.cpp:75-84)
+    T.vptr.assign(N + 1, 0);
+    for (int64_t j = 0; j < 3 * M; ++j) T.vptr[tri[j] + 1]++;
+    for (int64_t i = 0; i < N; ++i) T.vptr[i + 1] += T.vptr[i];
+    T.vtri.resize(3 * M);
+    {
+        std::vector<int32_t> fill(T.vptr.begin(), T.vptr.end() - 1);
+        for (int64_t j = 0; j < M; ++j)
+            for (int k = 0; k < 3; ++k) T.vtri[fill[tri[3 * j + k]]++] = (int32_t)j;
+    }
+    for (int64_t i = 0; i < N; ++i)
+        BH_CHECK(T.vptr[i + 1] > T.vptr[i], BEMHIP_ERR_BAD_MESH,
+                 "vertex " + std::to_string(i) + " belongs to no triangle");
+
+    // loose parts by union-find; part ids ordered by their smallest vertex
+    {
+        std::vector<int32_t> p(N);
+        std::iota(p.begin(), p.end(), 0);
+        auto find = [&](int32_t a) {
+            while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
+            return a;
+        };
+        for (int64_t j = 0; j < M; ++j)
+            for (int k = 1; k < 3; ++k) {
+                int32_t a = find(tri[3 * j]), b = find(tri[3 * j + k]);
+                if (a != b) { if (a < b) p[b] = a; else p[a] = b; }
+            }
+        std::vector<int32_t> label(N, -1);
+        T.vpart.resize(N);
+        int n = 0;
+        for (int64_t i = 0; i < N; ++i) {
+            int32_t r = find((int32_t)i);
+            if (label[r] < 0) label[r] = n++;
+            T.vpart[i] = label[r];
+        }
+        T.nparts = n;
+        T.part_tptr.assign(n + 1, 0);
+        for (int64_t j = 0; j < M; ++j) T.part_tptr[T.vpart[tri[3 * j]] + 1]++;
+        for (int c = 0; c < n; ++c) T.part_tptr[c + 1] += T.part_tptr[c];
+        T.part_tri.resize(M);
+        std::vector<int32_t> fill(T.part_tptr.begin(), T.part_tptr.end() - 1);
+        for (int64_t j = 0; j < M; ++j) T.part_tri[fill[T.vpart[tri[3 * j]]]++] = (int32_t)j;
+    }
+
+    // assembly patches: compact groups of PATCH vertices grown breadth-first
+    // from seeds taken in Morton order, so a patch touches few triangles that
+    // also feed another patch.
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], xyz[3 * i + d]);
+                hi[d] = std::max(hi[d], xyz[3 * i + d]);
+            }
+        std::vector<std::pair<uint32_t, int32_t>> code(N);
+        for (int64_t i = 0; i < N; ++i) {
+            uint32_t q[3];
+            for (int d = 0; d < 3; ++d) {
+                const double span = hi[d] - lo[d];
+                double u = span > 0 ? (xyz[3 * i + d] - lo[d]) / span : 0.0;
+                q[d] = (uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0));
+            }
+            code[i] = {spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2), (int32_t)i};
+        }
+        std::sort(code.begin(), code.end());
+        std::vector<int32_t> pid(N, -1), slot(N, -1);
+        std::vector<int32_t> queue;
+        queue.reserve(N);
+        int np = 0;
+        for (int64_t s = 0; s < N; ++s) {
+            const int32_t seed = code[s].second;
+            if (pid[seed] >= 0) continue;
+            int filled = 0;
+            queue.clear();
+            queue.push_back(seed);
+            pid[seed] = np;
+            size_t head = 0;
+            while (head < queue.size() && filled < PATCH) {
+                const int32_t v = queue[head++];
+                slot[v] = filled++;
+                T.patch_col.push_back(v);
+                if (filled == PATCH) break;
+                for (int32_t e = T.vptr[v]; e < T.vptr[v + 1]; ++e) {
+                    const uint32_t* t = tri + 3 * (int64_t)T.vtri[e];
+                    for (int k = 0; k < 3; ++k) {
+                        const int32_t w = (int32_t)t[k];
+                        if (pid[w] < 0) { pid[w] = np; queue.push_back(w); }
+                    }
+                }
+            }
+            // vertices queued but not placed go back to the pool
+            for (size_t q = head; q < queue.size(); ++q)
+                if (slot[queue[q]] < 0) pid[queue[q]] = -1;
+            while (filled < PATCH) { T.patch_col.push_back(-1); ++filled; }
+            ++np;
+        }
+        T.npatch = np;
+        T.patch_ptr.assign(np + 1, 0);
+        for (int64_t j = 0; j < M; ++j) {
+            int32_t ps[3] = {pid[tri[3 * j]], pid[tri[3 * j + 1]], pid[tri[3 * j + 2]]};
+            for (int k = 0; k < 3; ++k) {
+                bool first = true;
+                for (int l = 0; l < k; ++l) if (ps[l] == ps[k]) first = false;
+                if (first) T.patch_ptr[ps[k] + 1]++;
+            }
+        }
+        for (int p = 0; p < np; ++p) T.patch_ptr[p + 1] += T.patch_ptr[p];
+        const int64_t ne = T.patch_ptr[np];
+        T.patch_tri.resize(ne);
+        T.patch_slots.resize(ne);
+        std::vector<int32_t> fill(T.patch_ptr.begin(), T.patch_ptr.end() - 1);
+        for (int64_t j = 0; j < M; ++j) {
+            const uint32_t* t = tri + 3 * j;
+            int32_t ps[3] = {pid[t[0]], pid[t[1]], pid[t[2]]};
+            for (int k = 0; k < 3; ++k) {
+                bool first = true;
+                for (int l = 0; l < k; ++l) if (ps[l] == ps[k]) first = false;
+                if (!first) continue;
+                uint32_t packed = 0;
+                for (int l = 0; l < 3; ++l) {
+                    const uint32_t sl = (ps[l] == ps[k]) ? (uint32_t)slot[t[l]] : 0xFFu;
+                    packed |= sl << (8 * l);
+                }
+                const int32_t e = fill[ps[k]]++;
+                T.patch_tri[e] = (uint32_t)j;
+                T.patch_slots[e] = packed;
+            }
+        }
+        T.redundancy = (double)ne / (double)M;
+    }
+    return T;
+}
+
+template <class V> static void up(DevBuf& b, const std::vector<V>& v, hipStream_t s) {
+    b.ensure(sizeof(V) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) BH_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, s));
+}
+
+void DevTopology::upload(const Topology& t, hipStream_t s) {
+    N = t.N;
+    M = t.M;
+    npatch = t.npatch;
+    nparts = t.nparts;
+    up(tri, t.tri, s);
+    up(patch_col, t.patch_col, s);
+    up(patch_ptr, t.patch_ptr, s);
+    up(patch_tri, t.patch_tri, s);
+    up(patch_slots, t.patch_slots, s);
+    up(vptr, t.vptr, s);
+    up(vtri, t.vtri, s);
+    up(vpart, t.vpart, s);
+    up(part_tptr, t.part_tptr, s);
+    up(part_tri, t.part_tri, s);
+    BH_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace bemhip

@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-RK4-stage BEM hot path on one MI355X.
+
+Workload: one sphere, icosphere of frequency nu = 64 (N = 40962 vertices,
+M = 81920 triangles), phi = -1 + 0.02 * seeded real spherical harmonics of
+degree 1..8, fixed RK4 step dt = 1e-3, free-space (no wall), sigma = 0.
+
+A bench "step" is one RK4 step (four stages, each = assembly of G with the
+fused H phi + dense solve + stage update), all state resident in HBM.  The
+reported value is ms per RK4 stage of the mixed-precision path (split-fp16 x3
+MFMA LU with FP32 accumulation + FP64 iterative refinement); the plain FP64
+path (FP64 LU) is timed on the same state for the speed ratio, and the psi of
+both paths is compared (target <= 1e-8 relative).
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bem-bubbles_amd"))
+
+import bemhip  # noqa: E402
+import meshgen  # noqa: E402
+
+METRIC = "ms/RK4 stage (assembly+solve) at N=40962; mixed-prec vs FP64 GPU @1e-8 rel"
+PEAK = {"lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(threads):
+    """Oracle (FP64 C/OpenMP restatement of the reference's loop structure)
+    timed on a bounded sample: one compute_psi stage at nu = 32 (N = 10242),
+    assembly scaled by (40962/10242)^2 and the LU by (40962/10242)^3."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    oracle.build()
+    v, t = meshgen.icosphere(32)
+    n = len(v)
+    t0 = time.perf_counter()
+    G, H = oracle.assemble(v, t, nthreads=threads)
+    t1 = time.perf_counter()
+    phi = -1.0 + 0.02 * meshgen.random_sh_field(v, 1, 8, 0)
+    b = oracle.gemv(H, phi, nthreads=threads)
+    del H
+    t2 = time.perf_counter()
+    LU, piv, _ = oracle.lu_factor(G, nthreads=threads)
+    oracle.lu_solve(LU, piv, b)
+    t3 = time.perf_counter()
+    del G, LU
+    s = 40962.0 / n
+    est_ms = ((t1 - t0) + (t2 - t1)) * s * s * 1e3 + (t3 - t2) * s ** 3 * 1e3
+    return {"value": round(est_ms, 1), "unit": "ms/stage (extrapolated to N=40962)", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle compute_psi at nu=32 (N={n}): assembly {t1 - t0:.2f}s, H*phi {t2 - t1:.2f}s, "
+                      f"LU+solve {t3 - t2:.2f}s on {threads} threads; assembly/GEMV scaled by (N/n)^2, "
+                      f"LU by (N/n)^3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nu", type=int, default=64)
+    ap.add_argument("--solver", default="lu16x3_ir64", choices=sorted(bemhip.SOLVERS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
+              f"design-only in this build", file=sys.stderr)
+        sys.exit(2)
+
+    case = meshgen.config_large(seed=0, nu=args.nu)
+    N, M = len(case.verts), len(case.tris)
+    log(f"[bench] nu={args.nu} N={N} M={M} solver={args.solver}")
+    ctx = bemhip.Context(0)
+    ph = case.physics
+    sim = bemhip.ColocSim(ctx, case.verts, case.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma,
+                          solver=args.solver)
+    sim.set_phi(case.phi)
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        sim.evolve_system_RK4(case.dt, True)
+        log(f"[bench] warmup step {w} {time.perf_counter() - t0:.3f}s")
+    sim.accum_stats(reset=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.evolve_system_RK4(case.dt, True)
+        log(f"[bench] step {k} done {time.perf_counter() - t0:.3f}s")
+    wall = time.perf_counter() - t0
+    acc = sim.accum_stats(reset=True)
+    stages = 4 * args.steps
+    ms_stage = wall * 1e3 / stages
+    log(f"[bench] mixed {ms_stage:.2f} ms/stage {acc}")
+
+    out = {
+        "metric": METRIC, "value": round(ms_stage, 3), "unit": "ms/stage", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-lu(split-f16 mfma)+f64-ir" if args.solver == "lu16x3_ir64" else
+                 ("f32-lu+f64-ir" if args.solver == "lu32_ir64" else "f64"),
+        "data": "synthetic",
+        "config": {"workload": f"one sphere, nu={args.nu} icosphere (N={N}, M={M}), phi=-1+0.02*SH(l=1..8) seed 0, "
+                               f"RK4 dt={case.dt}, one bench step = one RK4 step = 4 stages",
+                   "N": N, "M": M, "solver": args.solver, "stages_timed": stages},
+        "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
+        "refinement_sweeps_per_stage": acc["iterations"] / stages,
+    }
+    gf, gms = acc["gemm_flops"], acc["gemm_ms"]
+    if gms > 0:
+        achieved = gf / (gms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "mfma", "kernel": "LU trailing-update GEMM", "achieved": round(achieved, 2),
+                           "peak": round(PEAK[args.solver], 1), "unit": "TFLOP/s",
+                           "frac": round(achieved / PEAK[args.solver], 4), "traffic": None,
+                           "launches": acc["gemm_launches"], "avg_launch_ms": round(gms / max(1, acc["gemm_launches"]), 4)}
+
+    # plain FP64 path on the same state (timing + accuracy of the mixed psi)
+    if not args.no_fp64:
+        sim.compute_psi()
+        psi_mixed = sim.get_psi()
+        sim.set_solver("lu64")
+        sim.compute_psi()
+        psi64 = sim.get_psi()
+        err = float(np.max(np.abs(psi_mixed - psi64)) / np.max(np.abs(psi64)))
+        x, phi, psi, tm = sim.state()
+        sim.accum_stats(reset=True)
+        t0 = time.perf_counter()
+        sim.evolve_system_RK4(case.dt, True)
+        fp64_ms = (time.perf_counter() - t0) * 1e3 / 4
+        a64 = sim.accum_stats(reset=True)
+        log(f"[bench] fp64 {fp64_ms:.2f} ms/stage err={err:.3e} {a64}")
+        out["fp64_ms_per_stage"] = round(fp64_ms, 3)
+        out["speedup_vs_fp64"] = round(fp64_ms / ms_stage, 3)
+        out["psi_rel_err_vs_fp64"] = err
+        if a64["gemm_ms"] > 0:
+            out["fp64_gemm_tflops"] = round(a64["gemm_flops"] / (a64["gemm_ms"] * 1e-3) / 1e12, 2)
+    sim.close()
+    ctx.close()
+
+    if not args.no_cpu:
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * bemhip.h -- C-ABI of the MI355X (gfx950) per-RK4-stage BEM path.
+ *
+ * Plain C types only (pointers, sizes, POD structs); no exceptions cross this
+ * boundary.  Every entry point returns a bemhip_status; the message of the last
+ * failure on a context is available from bemhip_last_error().
+ *
+ * Layout conventions (identical to the reference's, Bem/basic/Bem.hpp:10-16):
+ *   xyz  N x 3 doubles, array of structs {x, y, z} (std::vector<vec3>)
+ *   tri  M x 3 uint32 vertex indices, right-handed (outward) order
+ *        (std::vector<Triplet> holds size_t; the host C++ class narrows them)
+ *   G, H N x N doubles, column-major (Eigen::MatrixXd), G(i,j) = G[j*N + i]
+ *
+ * Host buffers passed in are caller-owned and copied; device state is owned by
+ * the context / simulation handle.  One host thread per context; calls are
+ * issued on the context's stream and synchronous at return.
+ */
+#ifndef BEMHIP_H
+#define BEMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BEMHIP_OK = 0,
+    BEMHIP_ERR_ARG = 1,             /* invalid argument (sizes, null pointers, flags)  */
+    BEMHIP_ERR_BAD_MESH = 2,        /* index out of range, degenerate triangle          */
+    BEMHIP_ERR_NEG_X_IN_MIRROR = 3, /* wall mode with a vertex at x < 0 (LinLinSim.cpp:148-159 throws) */
+    BEMHIP_ERR_SINGULAR = 4,        /* exact zero pivot in the LU                       */
+    BEMHIP_ERR_NOT_CONVERGED = 5,   /* refinement / BiCGSTAB stopped above tolerance    */
+    BEMHIP_ERR_NAN = 6,             /* non-finite value in inputs or results            */
+    BEMHIP_ERR_OOM = 7,             /* device allocation failed                         */
+    BEMHIP_ERR_HIP = 8,             /* HIP / rocBLAS runtime error                      */
+    BEMHIP_ERR_NO_DEVICE = 9        /* no usable gfx950 device                          */
+} bemhip_status;
+
+/* assembly flags */
+#define BEMHIP_MIRROR_X0 0x1u   /* image across the wall x = 0 (MIRROR_MESH, LinLinSim.hpp:13) */
+
+/* dense solvers for G psi = H phi */
+typedef enum {
+    BEMHIP_SOLVE_LU64 = 0,        /* FP64 blocked GEPP (Eigen::PartialPivLU semantics)          */
+    BEMHIP_SOLVE_LU32_IR64 = 1,   /* FP32 GEPP factor + FP64-residual iterative refinement       */
+    BEMHIP_SOLVE_BICGSTAB64 = 2,  /* FP64 BiCGSTAB, diagonal preconditioner (Eigen::BiCGSTAB)    */
+    BEMHIP_SOLVE_LU16X3_IR64 = 3  /* FP32 GEPP whose trailing updates run as split-fp16 x3 MFMA
+                                     products with FP32 accumulation, + FP64 refinement          */
+} bemhip_solver;
+
+typedef struct bemhip_ctx bemhip_ctx;
+typedef struct bemhip_sim bemhip_sim;
+
+typedef struct {
+    int32_t iterations;      /* refinement sweeps or BiCGSTAB iterations           */
+    int32_t converged;       /* 1 when the stopping test was met                   */
+    double residual;         /* |G psi - H phi|_2 / |H phi|_2 in FP64              */
+    double correction;       /* last refinement |d|_inf / |psi|_inf                */
+    double ms_assemble;      /* device time of the assembly kernels                */
+    double ms_factor;        /* device time of the factorisation                   */
+    double ms_solve;         /* device time of solves / refinement / iterations    */
+    double gemm_ms;          /* device time inside trailing-update GEMMs           */
+    double gemm_flops;       /* algorithmic flops of those GEMMs (2 m n k each)    */
+    int32_t gemm_launches;   /* number of trailing-update GEMM launches            */
+    int32_t pad_;
+} bemhip_solve_stats;
+
+/* physics and flags of a simulation (Simulation.hpp:29-45, LinLinSim.hpp:22-27) */
+typedef struct {
+    double p_inf, epsilon, sigma, gamma;
+    double V_0;                 /* NaN: total initial volume (Simulation.hpp:42) */
+    double field_amp, field_freq, field_phase;  /* p_field = amp sin(2 pi freq t + phase) */
+    double dp_balance;          /* get_dt balance (Simulation.hpp:37), default 3 */
+    double min_dt;              /* <= 0: unused (Simulation.cpp:86) */
+    double wall_eps;            /* nopenetration eps (LinLinSim.hpp:24), default 1e-2 */
+    double tol;                 /* solver tolerance (<= 0: method default) */
+    int32_t maxit;              /* solver iteration cap (<= 0: method default) */
+    int32_t solver;             /* bemhip_solver; the reference default is BiCGSTAB */
+    uint32_t flags;             /* BEMHIP_MIRROR_X0 */
+    int32_t wall_guards;        /* test_negative + nopenetration (reference RK4 in wall mode) */
+    int32_t frozen_stage_time;  /* 1: every stage uses the step-start time (LinLinSim.cpp:208-238) */
+    int32_t pad_;
+} bemhip_sim_params;
+
+/* ---- context ------------------------------------------------------------ */
+int bemhip_create(int device, bemhip_ctx** out);
+int bemhip_destroy(bemhip_ctx* ctx);
+const char* bemhip_last_error(const bemhip_ctx* ctx);
+/* quadrature constants the kernels use: gauss7 x,w (7+7, on (0,1)) then the
+ * triangle rule x,y,w (7+7+7); 35 doubles (quadrature.hpp:30-38, 88-96).
+ * Needs no device. */
+void bemhip_quadrature(double* out35);
+
+/* ---- ColocSim::assemble_matrices (ColocSim.cpp:21-110) ------------------
+ * G (and H when non-null) receive the column-major N x N matrices of the
+ * reference, H including its 4-pi diagonal.  When phi and Hphi are both
+ * non-null, Hphi = H phi is formed inside the assembly without storing H. */
+int bemhip_assemble(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                    uint32_t flags, double* G, double* H, const double* phi, double* Hphi);
+
+/* ---- Simulation::solve_system (Simulation.cpp:40-71) --------------------
+ * Solves G x = b for a column-major FP64 G given on the host. */
+int bemhip_solve(bemhip_ctx* ctx, int64_t N, const double* G, const double* b, double* x,
+                 int solver, double tol, int32_t maxit, bemhip_solve_stats* stats);
+
+/* ---- Simulation::compute_psi (Simulation.hpp:73-77) ---------------------
+ * Assembly + H phi + solve, device resident; psi receives dphi/dn. */
+int bemhip_compute_psi(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                       const double* phi, uint32_t flags, int solver, double tol, int32_t maxit,
+                       double* psi, bemhip_solve_stats* stats);
+
+/* ---- simulation handle (LinLinSim / ColocSim state, device resident) ---- */
+void bemhip_sim_default_params(bemhip_sim_params* p);
+int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, const double* xyz,
+                      int64_t M, const uint32_t* tri, bemhip_sim** out);
+int bemhip_sim_destroy(bemhip_sim* sim);
+int bemhip_sim_set_phi(bemhip_sim* sim, const double* phi);            /* Simulation.hpp:92-95   */
+int bemhip_sim_set_psi(bemhip_sim* sim, const double* psi);            /* Simulation.hpp:97-100  */
+int bemhip_sim_set_V_0(bemhip_sim* sim, double V_0);                   /* Simulation.hpp:106-108 */
+int bemhip_sim_set_time(bemhip_sim* sim, double t);                    /* Simulation.hpp:120-122 */
+int bemhip_sim_set_solver(bemhip_sim* sim, int solver, double tol, int32_t maxit);
+int bemhip_sim_get_state(bemhip_sim* sim, double* xyz, double* phi, double* psi, double* time);
+int bemhip_sim_get_volume(bemhip_sim* sim, double* vol);               /* Simulation.hpp:128-130 */
+/* LinLinSim::position_t (LinLinSim.cpp:26-83) on the current state: vel N x 3
+ * and psi N (either may be null) */
+int bemhip_sim_position_t(bemhip_sim* sim, double* vel, double* psi);
+/* LinLinSim::pot_t_multi (LinLinSim.cpp:98-120) for given velocities */
+int bemhip_sim_pot_t_multi(bemhip_sim* sim, const double* vel, double t, double* dpot);
+/* Simulation::compute_psi on the current state */
+int bemhip_sim_compute_psi(bemhip_sim* sim);
+/* LinLinSim::evolve_system_RK4 (LinLinSim.cpp:194-257); *dt_out may be null */
+int bemhip_sim_evolve_rk4(bemhip_sim* sim, double dp, int fixdt, double* dt_out);
+/* LinLinSim::evolve_system, Euler (LinLinSim.cpp:260-284) */
+int bemhip_sim_evolve_euler(bemhip_sim* sim, double dp, int fixdt, double* dt_out);
+/* statistics of the last stage's solve */
+int bemhip_sim_last_stats(bemhip_sim* sim, bemhip_solve_stats* stats);
+/* sums of the timing fields (and iteration counts) over all stages since the
+ * last reset; residual / correction hold the maxima */
+int bemhip_sim_accum_stats(bemhip_sim* sim, bemhip_solve_stats* stats, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

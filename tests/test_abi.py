@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports what include/bemhip.h declares (CPU)."""
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+import bemhip
+
+
+def test_library_exports_every_declared_symbol():
+    names = bemhip.declared_functions()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(bemhip.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_kernel_quadrature_constants_match_closed_forms(oracle_lib):
+    q = bemhip.quadrature()
+    o = oracle_lib.quadrature()
+    for k in q:
+        assert np.max(np.abs(q[k] - o[k])) < 3e-16, k
+    s = math.sqrt(15.0)
+    assert abs(q["tri_x"][1] - (9 + 2 * s) / 21) < 2e-16
+    assert abs(q["tri_w"].sum() - 0.5) < 1e-15 and abs(q["gauss_w"].sum() - 1.0) < 1e-15
+
+
+def test_no_silent_fallback_without_device():
+    """On a host without a gfx950 device the context refuses to start."""
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        bemhip.Context(0)
+    except bemhip.BemHipError as e:
+        assert e.code == bemhip.ERR_NO_DEVICE
+    else:
+        raise AssertionError("context created without a device")
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    try:
+        bemhip._lib_saved = bemhip._lib
+        bemhip._lib = None
+        bemhip.load(str(tmp_path / "nope.so"))
+    except RuntimeError as e:
+        assert "not built" in str(e)
+    else:
+        raise AssertionError("load of a missing library did not raise")
+    finally:
+        bemhip._lib = bemhip._lib_saved
