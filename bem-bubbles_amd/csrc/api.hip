@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -175,8 +176,14 @@ int bemhip_create(int device, bemhip_ctx** out) {
         BH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         BH_BLAS(rocblas_create_handle(&ctx->blas));
         BH_BLAS(rocblas_set_stream(ctx->blas, ctx->stream));
+        BH_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+        BH_BLAS(rocblas_create_handle(&ctx->blas2));
+        BH_BLAS(rocblas_set_stream(ctx->blas2, ctx->stream2));
         BH_HIP(hipHostMalloc((void**)&ctx->host_scal, 64 * sizeof(double), hipHostMallocDefault));
         for (auto& e : ctx->ev) BH_HIP(hipEventCreate(&e));
+        for (auto& e : ctx->ev_la) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const char* la = std::getenv("BEMHIP_LU_LOOKAHEAD");
+        if (la && la[0] == '0') ctx->lookahead = false;
     });
     if (rc != BEMHIP_OK) {
         // keep the context alive just long enough to report; callers get no handle
@@ -190,7 +197,12 @@ int bemhip_destroy(bemhip_ctx* ctx) {
     if (!ctx) return BEMHIP_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->ev_la) if (e) (void)hipEventDestroy(e);
+    if (ctx->blas2) (void)rocblas_destroy_handle(ctx->blas2);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+    for (auto& e : ctx->gemm_ev) if (e) (void)hipEventDestroy(e);
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -390,6 +402,35 @@ int bemhip_sim_set_solver(bemhip_sim* s, int solver, double tol, int32_t maxit) 
     s->P.tol = tol;
     s->P.maxit = maxit;
     return BEMHIP_OK;
+}
+
+int bemhip_sim_set_params(bemhip_sim* s, const bemhip_sim_params* p) {
+    if (!s || !p) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(!(p->flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown flag");
+        BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
+        const double v0 = s->P.V_0;
+        s->P = *p;
+        if (std::isnan(p->V_0)) s->P.V_0 = v0;
+    });
+}
+
+int bemhip_sim_set_mesh(bemhip_sim* s, int64_t N, const double* xyz, int64_t M, const uint32_t* tri) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(xyz && tri, BEMHIP_ERR_ARG, "null argument");
+        bemhip_ctx* ctx = s->ctx;
+        Topology th = build_topology(N, xyz, M, tri);
+        s->th = std::move(th);
+        s->topo.upload(s->th, ctx->stream);
+        s->N = N;
+        s->M = M;
+        double* x = dev<double>(s->x, 3 * N);
+        h2d(x, xyz, sizeof(double) * 3 * N, ctx->stream);
+        BH_HIP(hipMemsetAsync(dev<double>(s->phi, N), 0, sizeof(double) * N, ctx->stream));
+        BH_HIP(hipMemsetAsync(dev<double>(s->psi, N), 0, sizeof(double) * N, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+    });
 }
 
 int bemhip_sim_get_state(bemhip_sim* s, double* xyz, double* phi, double* psi, double* time) {

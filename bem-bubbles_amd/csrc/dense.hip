@@ -68,7 +68,7 @@ __device__ inline void grid_barrier(unsigned int* bar, unsigned int target, unsi
         while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target &&
                !(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u)) {
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1u << 22)) {
+            if (++spins > (1u << 25)) {
                 __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -186,26 +186,26 @@ __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
 }
 
 // ------------------------------------------------------------ rocBLAS glue --
-static void trsm_llnu(bemhip_ctx* ctx, int m, int n, const double* A, int lda, double* B, int ldb) {
+static void trsm_llnu(rocblas_handle h, int m, int n, const double* A, int lda, double* B, int ldb) {
     const double one = 1.0;
-    BH_BLAS(rocblas_dtrsm(ctx->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+    BH_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                           rocblas_diagonal_unit, m, n, &one, A, lda, B, ldb));
 }
-static void trsm_llnu(bemhip_ctx* ctx, int m, int n, const float* A, int lda, float* B, int ldb) {
+static void trsm_llnu(rocblas_handle h, int m, int n, const float* A, int lda, float* B, int ldb) {
     const float one = 1.0f;
-    BH_BLAS(rocblas_strsm(ctx->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+    BH_BLAS(rocblas_strsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                           rocblas_diagonal_unit, m, n, &one, A, lda, B, ldb));
 }
-static void gemm_minus(bemhip_ctx* ctx, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+static void gemm_minus(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                        double* C, int ldc) {
     const double alpha = -1.0, beta = 1.0;
-    BH_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B,
+    BH_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B,
                           ldb, &beta, C, ldc));
 }
-static void gemm_minus(bemhip_ctx* ctx, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
+static void gemm_minus(rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
                        float* C, int ldc) {
     const float alpha = -1.0f, beta = 1.0f;
-    BH_BLAS(rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B,
+    BH_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B,
                           ldb, &beta, C, ldc));
 }
 
@@ -303,84 +303,132 @@ template <class T>
 static void trailing_update(bemhip_ctx* ctx, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C,
                             int ldc, bool split16) {
     if (m <= 0 || n <= 0 || k <= 0) return;
-    if (ctx->timing) BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        // event pairs are recorded now and read once after the factorisation
+        const size_t i = ctx->gemm_ev_used;
+        while (ctx->gemm_ev.size() < 2 * (i + 1)) {
+            hipEvent_t e;
+            BH_HIP(hipEventCreate(&e));
+            ctx->gemm_ev.push_back(e);
+        }
+        e0 = ctx->gemm_ev[2 * i];
+        e1 = ctx->gemm_ev[2 * i + 1];
+        ctx->gemm_fl.resize(i + 1);
+        ctx->gemm_fl[i] = 2.0 * (double)m * (double)n * (double)k;
+        ctx->gemm_ev_used = i + 1;
+        BH_HIP(hipEventRecord(e0, ctx->stream));
+    }
     if constexpr (std::is_same<T, float>::value) {
         if (split16) gemm_minus_split16(ctx, m, n, k, A, lda, B, ldb, C, ldc);
-        else gemm_minus(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+        else gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
-        gemm_minus(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+        gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
     }
-    if (ctx->timing) {
-        BH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        BH_HIP(hipEventSynchronize(ctx->ev[1]));
+    if (ctx->timing) BH_HIP(hipEventRecord(e1, ctx->stream));
+}
+
+// sums the recorded trailing-update timings (after the stream has drained)
+static void collect_gemm_timing(bemhip_ctx* ctx) {
+    for (size_t i = 0; i < ctx->gemm_ev_used; ++i) {
         float ms = 0.0f;
-        BH_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        BH_HIP(hipEventElapsedTime(&ms, ctx->gemm_ev[2 * i], ctx->gemm_ev[2 * i + 1]));
         ctx->gemm_ms += ms;
-        ctx->gemm_flops += 2.0 * (double)m * (double)n * (double)k;
+        ctx->gemm_flops += ctx->gemm_fl[i];
         ctx->gemm_launches += 1;
     }
+    ctx->gemm_ev_used = 0;
 }
 
 // ------------------------------------------------------------ factorise --
+// A stream with the rocBLAS handle bound to it.
+struct Lane {
+    hipStream_t s;
+    rocblas_handle h;
+};
+
 template <class T>
-static void leaf(bemhip_ctx* ctx, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
+static void leaf(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
     const int P = (m + PROWS - 1) / PROWS;
     BH_CHECK(P <= MAXP, BEMHIP_ERR_ARG, "matrix too large for the panel kernel (N > 65536)");
     char* ws = ctx->lu.panel_ws.as<char>();
-    unsigned int b = base;
-    void* args[] = {&m, &n, &A, (void*)&lda, &ipiv, &roff, &ws, &b};
-    BH_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(P), dim3(PROWS), args, 0, ctx->stream));
+    // P <= 256 work-groups of 256 threads always fit on the device at once (1-2 per
+    // CU by LDS), so a plain launch is co-resident once any concurrent kernel on
+    // the other stream retires work-groups; the barrier's bounded spin reports
+    // (never hangs) if that assumption were broken.
+    k_panel<T><<<P, PROWS, 0, L.s>>>(m, n, A, lda, ipiv, roff, ws, base);
+    BH_HIP(hipGetLastError());
     base += (unsigned)n * (unsigned)P;
 }
 
 template <class T>
-static void laswp(bemhip_ctx* ctx, int ncols, T* A, int lda, const int* ipiv, int k1, int k2, int roff) {
+static void laswp(Lane L, int ncols, T* A, int lda, const int* ipiv, int k1, int k2, int roff) {
     if (ncols <= 0 || k2 <= k1) return;
-    k_laswp<T><<<(ncols + 255) / 256, 256, 0, ctx->stream>>>(ncols, A, lda, ipiv, k1, k2, roff);
+    k_laswp<T><<<(ncols + 255) / 256, 256, 0, L.s>>>(ncols, A, lda, ipiv, k1, k2, roff);
     BH_HIP(hipGetLastError());
 }
 
 template <class T>
-static void panel_rec(bemhip_ctx* ctx, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
-    if (n <= LEAF) { leaf(ctx, m, n, A, lda, ipiv, roff, base); return; }
+static void panel_rec(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
+    if (n <= LEAF) { leaf(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
     int n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
     if (n1 >= n) n1 = n - LEAF;
     const int n2 = n - n1;
-    panel_rec(ctx, m, n1, A, lda, ipiv, roff, base);
+    panel_rec(ctx, L, m, n1, A, lda, ipiv, roff, base);
     T* A12 = A + (size_t)n1 * lda;
-    laswp(ctx, n2, A12, lda, ipiv, 0, n1, roff);
-    trsm_llnu(ctx, n1, n2, A, lda, A12, lda);
+    laswp(L, n2, A12, lda, ipiv, 0, n1, roff);
+    trsm_llnu(L.h, n1, n2, A, lda, A12, lda);
     if (m > n1) {
-        gemm_minus(ctx, m - n1, n2, n1, A + n1, lda, A12, lda, A12 + n1, lda);
-        panel_rec(ctx, m - n1, n2, A12 + n1, lda, ipiv + n1, roff + n1, base);
-        laswp(ctx, n1, A + n1, lda, ipiv + n1, 0, n2, roff + n1);
+        gemm_minus(L.h, m - n1, n2, n1, A + n1, lda, A12, lda, A12 + n1, lda);
+        panel_rec(ctx, L, m - n1, n2, A12 + n1, lda, ipiv + n1, roff + n1, base);
+        laswp(L, n1, A + n1, lda, ipiv + n1, 0, n2, roff + n1);
     }
 }
 
+// Blocked right-looking GEPP with one block of look-ahead: while the main
+// stream applies step k's update to the trailing matrix, the panel of step
+// k+1 (whose columns were updated first) is factored on the second stream.
 template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16) {
     ctx->lu.panel_ws.ensure(PanelWs::BYTES);
     char* ws = ctx->lu.panel_ws.as<char>();
-    k_fill_u32<<<1, 64, 0, ctx->stream>>>(reinterpret_cast<unsigned int*>(ws), 16, 0u);
+    const Lane main{ctx->stream, ctx->blas};
+    const Lane side = ctx->lookahead ? Lane{ctx->stream2, ctx->blas2} : main;
+    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1];
+    k_fill_u32<<<1, 64, 0, main.s>>>(reinterpret_cast<unsigned int*>(ws), 16, 0u);
     BH_HIP(hipGetLastError());
     unsigned int base = 0;
     const int lda = N;
+    // panel 0
+    BH_HIP(hipEventRecord(ev_ready, main.s));
+    BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+    panel_rec(ctx, side, N, std::min(LU_NB, N), A, lda, ipiv, 0, base);
+    BH_HIP(hipEventRecord(ev_panel, side.s));
     for (int k = 0; k < N; k += LU_NB) {
         const int jb = std::min(LU_NB, N - k);
         T* Akk = A + (size_t)k * lda + k;
-        panel_rec(ctx, N - k, jb, Akk, lda, ipiv + k, k, base);
-        laswp(ctx, k, A + k, lda, ipiv + k, 0, jb, k);
+        BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));   // panel k factored
+        laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
         const int rest = N - k - jb;
-        if (rest > 0) {
-            T* A12 = A + (size_t)(k + jb) * lda + k;
-            laswp(ctx, rest, A12, lda, ipiv + k, 0, jb, k);
-            trsm_llnu(ctx, jb, rest, Akk, lda, A12, lda);
-            trailing_update<T>(ctx, rest, rest, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16);
-        }
-        // a barrier counter wrap is impossible below 2^32 column-work-group steps
+        if (rest <= 0) break;
+        T* A12 = A + (size_t)(k + jb) * lda + k;
+        laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
+        trsm_llnu(main.h, jb, rest, Akk, lda, A12, lda);
+        const int jb2 = std::min(LU_NB, rest);
+        // next panel's columns first, then hand them to the side stream
+        trailing_update<T>(ctx, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16);
+        BH_HIP(hipEventRecord(ev_ready, main.s));
+        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+        panel_rec(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        BH_HIP(hipEventRecord(ev_panel, side.s));
+        if (rest > jb2)
+            trailing_update<T>(ctx, rest, rest - jb2, jb, Akk + jb, lda, A12 + (size_t)jb2 * lda, lda,
+                               A12 + (size_t)jb2 * lda + jb, lda, split16);
     }
+    BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     unsigned int flags[3] = {0, 0, 0};
-    BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, ctx->stream));
-    BH_HIP(hipStreamSynchronize(ctx->stream));
+    BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, main.s));
+    BH_HIP(hipStreamSynchronize(main.s));
+    collect_gemm_timing(ctx);
     BH_CHECK(!(flags[1] & 1u), BEMHIP_ERR_HIP, "panel grid barrier timed out");
     if (flags[1] & 2u) return (int)flags[2];  // 1-based first zero pivot
     return 0;
@@ -507,7 +555,7 @@ void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const doub
     BH_HIP(hipGetLastError());
 }
 
-// deterministic two-pass reductions: op 0 = max |x|, 1 = dot(x, y)
+// deterministic two-pass reductions: op 0 = max |x|, 1 = dot(x, y), 2 = sum(x)
 template <int OP>
 __global__ __launch_bounds__(256) void k_reduce_part(int n, const double* __restrict__ x, const double* __restrict__ y,
                                                      double* __restrict__ part) {
@@ -515,7 +563,8 @@ __global__ __launch_bounds__(256) void k_reduce_part(int n, const double* __rest
     double acc = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         if (OP == 0) acc = fmax(acc, fabs(x[i]));
-        else acc += x[i] * y[i];
+        else if (OP == 1) acc += x[i] * y[i];
+        else acc += x[i];
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -531,7 +580,7 @@ static double reduce(bemhip_ctx* ctx, int n, const double* x, const double* y) {
     const int nb = 256;
     double* part = static_cast<double*>(ctx->scal.ensure(sizeof(double) * 1024));
     k_reduce_part<OP><<<nb, 256, 0, ctx->stream>>>(n, x, y, part);
-    k_reduce_part<OP><<<1, 256, 0, ctx->stream>>>(nb, part, part, part + 512);
+    k_reduce_part<OP == 0 ? 0 : 2><<<1, 256, 0, ctx->stream>>>(nb, part, nullptr, part + 512);
     BH_HIP(hipMemcpyAsync(ctx->host_scal, part + 512, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
     return ctx->host_scal[0];

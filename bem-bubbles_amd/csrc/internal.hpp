@@ -110,6 +110,10 @@ struct bemhip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
+    hipStream_t stream2 = nullptr;      // LU look-ahead: panels of the next block column
+    rocblas_handle blas2 = nullptr;
+    hipEvent_t ev_la[2] = {};
+    bool lookahead = true;
     std::string err;
     int cus = 0;
     // workspace
@@ -121,6 +125,9 @@ struct bemhip_ctx {
     // instrumentation of trailing updates
     bool timing = true;
     hipEvent_t ev[4] = {};
+    std::vector<hipEvent_t> gemm_ev;   // start/stop pairs of the trailing updates of one factorisation
+    std::vector<double> gemm_fl;
+    size_t gemm_ev_used = 0;
     double gemm_ms = 0.0, gemm_flops = 0.0;
     int gemm_launches = 0;
 };

@@ -120,6 +120,13 @@ int bemhip_sim_set_psi(bemhip_sim* sim, const double* psi);            /* Simula
 int bemhip_sim_set_V_0(bemhip_sim* sim, double V_0);                   /* Simulation.hpp:106-108 */
 int bemhip_sim_set_time(bemhip_sim* sim, double t);                    /* Simulation.hpp:120-122 */
 int bemhip_sim_set_solver(bemhip_sim* sim, int solver, double tol, int32_t maxit);
+/* replace the physics / solver / flag parameters (set_min_dt, set_dp_balance,
+ * set_bcgstab, set_V_0 of Simulation.hpp:106-159); a NaN V_0 keeps the current one */
+int bemhip_sim_set_params(bemhip_sim* sim, const bemhip_sim_params* p);
+/* new mesh after remeshing (LinLinSim::remesh, LinLinSim.cpp:385-438): the
+ * topology tables are rebuilt, phi and psi are reset to zero and must be set
+ * again; time and V_0 are kept */
+int bemhip_sim_set_mesh(bemhip_sim* sim, int64_t N, const double* xyz, int64_t M, const uint32_t* tri);
 int bemhip_sim_get_state(bemhip_sim* sim, double* xyz, double* phi, double* psi, double* time);
 int bemhip_sim_get_volume(bemhip_sim* sim, double* vol);               /* Simulation.hpp:128-130 */
 /* LinLinSim::position_t (LinLinSim.cpp:26-83) on the current state: vel N x 3

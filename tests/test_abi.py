@@ -51,3 +51,22 @@ def test_missing_library_fails_loudly(tmp_path):
         raise AssertionError("load of a missing library did not raise")
     finally:
         bemhip._lib = bemhip._lib_saved
+
+
+def test_cpp_surface_compiles_and_links(tmp_path):
+    """include/bemhip.hpp (the ColocSim-shaped C++ surface) builds against the library."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "bemhip.hpp"\n'
+                   "int main() {\n"
+                   "  bemhip::Context c(0);\n"
+                   "  std::vector<bemhip::vec3d> v{{0,0,0},{1,0,0},{0,1,0},{0,0,1}};\n"
+                   "  std::vector<bemhip::tri3> t{{0,2,1},{0,1,3},{0,3,2},{1,2,3}};\n"
+                   "  bemhip::ColocSim s(c, v, t, 1.0, 1.0, 0.0, 1.0);\n"
+                   "  s.set_bcgstab(false); s.set_phi(1.0); s.evolve_system_RK4(1e-3, true);\n"
+                   "  return s.get_psi().size() == 4 ? 0 : 1;\n}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"), str(src),
+                        "-o", str(tmp_path / "use"), "-L", os.path.dirname(bemhip.LIB_PATH), "-lbemhip"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
