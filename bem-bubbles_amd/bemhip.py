@@ -44,7 +44,8 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double),
                 ("correction", C.c_double), ("ms_assemble", C.c_double), ("ms_factor", C.c_double),
                 ("ms_solve", C.c_double), ("gemm_ms", C.c_double), ("gemm_flops", C.c_double),
-                ("gemm_launches", C.c_int32), ("pad_", C.c_int32)]
+                ("gemm_launches", C.c_int32), ("pad_", C.c_int32), ("panels", C.c_int32),
+                ("panels_pivoted", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
@@ -106,6 +107,8 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_evolve_euler.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.bemhip_sim_last_stats.argtypes = [vp, C.POINTER(SolveStats)]
     L.bemhip_sim_accum_stats.argtypes = [vp, C.POINTER(SolveStats), C.c_int]
+    L.bemhip_sim_set_params.argtypes = [vp, C.POINTER(SimParams)]
+    L.bemhip_sim_set_mesh.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p]
     _lib = L
     return L
 

@@ -102,6 +102,8 @@ void accumulate(bemhip_solve_stats& a, const bemhip_solve_stats& s) {
     a.gemm_ms += s.gemm_ms;
     a.gemm_flops += s.gemm_flops;
     a.gemm_launches += s.gemm_launches;
+    a.panels += s.panels;
+    a.panels_pivoted += s.panels_pivoted;
     a.residual = std::max(a.residual, s.residual);
     a.correction = std::max(a.correction, s.correction);
     const bool first = a.pad_ == 0;   // pad_ marks a non-empty accumulation
@@ -184,6 +186,13 @@ int bemhip_create(int device, bemhip_ctx** out) {
         for (auto& e : ctx->ev_la) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         const char* la = std::getenv("BEMHIP_LU_LOOKAHEAD");
         if (la && la[0] == '0') ctx->lookahead = false;
+        const char* sp = std::getenv("BEMHIP_LU_SPECULATE");
+        if (sp && sp[0] == '0') ctx->speculate = false;
+        if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
+        const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
+        if (tp && tp[0] == '0') ctx->trsv_persistent = false;
     });
     if (rc != BEMHIP_OK) {
         // keep the context alive just long enough to report; callers get no handle

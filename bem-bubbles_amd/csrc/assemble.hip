@@ -9,8 +9,10 @@
 // 4-pi rule H(i,i) = -4 pi - sum_{j != i} H(i,j).
 //
 // Device design:
-//   k_tri_prep   one thread per triangle: the 7 quadrature points, unit normal,
-//                Jacobian (|ab x bc|) and first vertex, 32 doubles per triangle.
+//   k_entry_prep one thread per patch entry (a triangle as seen from one
+//                column patch): the 7 quadrature points, unit normal, Jacobian
+//                (|ab x bc|), the potential at its vertices, vertex ids and
+//                patch slots -- one contiguous 256-byte record.
 //   k_far        work-group = 128 collocation rows (one per lane) x a group of
 //                column patches.  A patch is 32 mesh-adjacent vertices; the
 //                work-group streams the triangles incident to the patch (wave-
@@ -32,21 +34,23 @@ namespace bemhip {
 
 constexpr int TQ_STRIDE = 32;  // doubles per triangle record
 
-struct QW {
-    double c[7][3];
-};
-
 // w_q * phi_k(q), phi = (1 - x - y, x, y): basis of get_linear_elements
 // (ResultTypes.hpp:111-117) at u = x + y, v = y.
 __host__ __device__ constexpr double basis_w(int q, int k) {
     return TQ_W[q] * (k == 0 ? (1.0 - (TQ_X[q] + TQ_Y[q])) : (k == 1 ? TQ_X[q] : TQ_Y[q]));
 }
 
-__global__ __launch_bounds__(256) void k_tri_prep(int M, const double* __restrict__ xyz,
-                                                  const uint32_t* __restrict__ tri,
-                                                  double* __restrict__ triq) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
+// One record per patch entry (a triangle as seen from one patch), so the
+// assembly streams records contiguously with a single scalar load per
+// triangle: the 7 quadrature points, unit normal, Jacobian, the potential at
+// the 3 vertices, the vertex ids and the vertex slots in this patch.
+__global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __restrict__ xyz,
+                                                    const double* __restrict__ phi, const uint32_t* __restrict__ tri,
+                                                    const uint32_t* __restrict__ ptri,
+                                                    const uint32_t* __restrict__ pslots, double* __restrict__ rec) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    const uint32_t j = ptri[e];
     const uint32_t ia = tri[3 * j], ib = tri[3 * j + 1], ic = tri[3 * j + 2];
     const double ax = xyz[3 * ia], ay = xyz[3 * ia + 1], az = xyz[3 * ia + 2];
     const double bx = xyz[3 * ib], by = xyz[3 * ib + 1], bz = xyz[3 * ib + 2];
@@ -58,7 +62,7 @@ __global__ __launch_bounds__(256) void k_tri_prep(int M, const double* __restric
     const double J = sqrt(nx * nx + ny * ny + nz * nz);
     const double rj = 1.0 / J;
     nx *= rj; ny *= rj; nz *= rj;
-    double* o = triq + (size_t)TQ_STRIDE * j;
+    double* o = rec + (size_t)TQ_STRIDE * e;
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
         const double u = TQ_X[q] + TQ_Y[q], v = TQ_Y[q];
@@ -68,15 +72,18 @@ __global__ __launch_bounds__(256) void k_tri_prep(int M, const double* __restric
     }
     o[21] = nx; o[22] = ny; o[23] = nz;
     o[24] = J;
-    o[25] = ax; o[26] = ay; o[27] = az;
-    o[28] = 0.0; o[29] = 0.0; o[30] = 0.0; o[31] = 0.0;
+    o[25] = phi ? phi[ia] : 0.0;
+    o[26] = phi ? phi[ib] : 0.0;
+    o[27] = phi ? phi[ic] : 0.0;
+    uint32_t* w = reinterpret_cast<uint32_t*>(o + 28);
+    w[0] = ia; w[1] = ib; w[2] = ic; w[3] = pslots[e];
+    o[30] = 0.0; o[31] = 0.0;
 }
 
 template <bool MIRROR, bool STORE_H, bool WRITE32>
 __global__ __launch_bounds__(ASM_ROWS) void k_far(
     int N, int npatch, int patches_per_group, const double* __restrict__ xyz, const double* __restrict__ phi,
-    const double* __restrict__ triq, const uint32_t* __restrict__ tri, const int32_t* __restrict__ pptr,
-    const uint32_t* __restrict__ ptri, const uint32_t* __restrict__ pslots, const int32_t* __restrict__ pcol,
+    const double* __restrict__ rec, const int32_t* __restrict__ pptr, const int32_t* __restrict__ pcol,
     double* __restrict__ G, float* __restrict__ G32, double* __restrict__ H, double* __restrict__ part_hphi,
     double* __restrict__ part_hrow) {
     __shared__ double accG[PATCH][ASM_ROWS];
@@ -99,13 +106,13 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
         }
         const int e0 = pptr[p], e1 = pptr[p + 1];
         for (int e = e0; e < e1; ++e) {
-            const uint32_t t = ptri[e];
-            const uint32_t sl = pslots[e];
-            const double* __restrict__ q = triq + (size_t)TQ_STRIDE * t;
-            const uint32_t v0 = tri[3 * t], v1 = tri[3 * t + 1], v2 = tri[3 * t + 2];
+            const double* __restrict__ q = rec + (size_t)TQ_STRIDE * e;
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(q + 28);
+            const uint32_t v0 = w[0], v1 = w[1], v2 = w[2], sl = w[3];
             const double nx = q[21], ny = q[22], nz = q[23], J = q[24];
-            const double dy = q[26] - yi, dz = q[27] - zi;
-            const double zn = (q[25] - xi) * nx + dy * ny + dz * nz;
+            // every quadrature point lies in the triangle's plane: (y_q - x).n is one number
+            const double dy = q[1] - yi, dz = q[2] - zi;
+            const double zn = (q[0] - xi) * nx + dy * ny + dz * nz;
             double g0 = 0, g1 = 0, g2 = 0, s0 = 0, s1 = 0, s2 = 0;
             double gm0 = 0, gm1 = 0, gm2 = 0, sm0 = 0, sm1 = 0, sm2 = 0;
 #pragma unroll
@@ -129,8 +136,8 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
             double G0 = far ? g0 : 0.0, G1 = far ? g1 : 0.0, G2 = far ? g2 : 0.0;
             double H0 = far ? -zn * s0 : 0.0, H1 = far ? -zn * s1 : 0.0, H2 = far ? -zn * s2 : 0.0;
             if constexpr (MIRROR) {
-                // image point R y (R: x -> -x) with normal R n: (Ry - x).(Rn) = (a - Rx).n
-                const double znm = (q[25] + xi) * nx + dy * ny + dz * nz;
+                // image point R y (R: x -> -x) with normal R n: (Ry - x).(Rn) = (y - Rx).n
+                const double znm = (q[0] + xi) * nx + dy * ny + dz * nz;
                 G0 += gm0; G1 += gm1; G2 += gm2;
                 H0 += -znm * sm0; H1 += -znm * sm1; H2 += -znm * sm2;
             }
@@ -140,19 +147,19 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
             if (sa != 0xFFu) {
                 accG[sa][tid] += G0;
                 if constexpr (STORE_H) accH[sa][tid] += H0;
-                hphi += H0 * (phi ? phi[v0] - phii : 0.0);
+                hphi += H0 * (q[25] - phii);
                 hrow += H0;
             }
             if (sb != 0xFFu) {
                 accG[sb][tid] += G1;
                 if constexpr (STORE_H) accH[sb][tid] += H1;
-                hphi += H1 * (phi ? phi[v1] - phii : 0.0);
+                hphi += H1 * (q[26] - phii);
                 hrow += H1;
             }
             if (sc != 0xFFu) {
                 accG[sc][tid] += G2;
                 if constexpr (STORE_H) accH[sc][tid] += H2;
-                hphi += H2 * (phi ? phi[v2] - phii : 0.0);
+                hphi += H2 * (q[27] - phii);
                 hrow += H2;
             }
         }
@@ -244,11 +251,14 @@ __global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* 
 
 void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi,
                      uint32_t flags, const AsmOut& out) {
-    const int N = (int)topo.N, M = (int)topo.M;
+    const int N = (int)topo.N;
     BH_CHECK(out.G64 || out.G32, BEMHIP_ERR_ARG, "assembly needs a G output");
     hipStream_t st = ctx->stream;
-    double* triq = ctx->triq.ensure(sizeof(double) * TQ_STRIDE * (size_t)M) ? ctx->triq.as<double>() : nullptr;
-    k_tri_prep<<<(M + 255) / 256, 256, 0, st>>>(M, d_xyz, topo.tri.as<uint32_t>(), triq);
+    const int nent = topo.nentries;
+    double* rec = static_cast<double*>(ctx->triq.ensure(sizeof(double) * TQ_STRIDE * (size_t)nent));
+    k_entry_prep<<<(nent + 255) / 256, 256, 0, st>>>(nent, d_xyz, d_phi, topo.tri.as<uint32_t>(),
+                                                      topo.patch_tri.as<uint32_t>(), topo.patch_slots.as<uint32_t>(),
+                                                      rec);
     BH_HIP(hipGetLastError());
 
     const int rowblocks = (N + ASM_ROWS - 1) / ASM_ROWS;
@@ -264,8 +274,7 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
     dim3 grid(rowblocks, groups);
 #define BH_FAR(MI, SH, W3)                                                                              \
     k_far<MI, SH, W3><<<grid, ASM_ROWS, 0, st>>>(                                                      \
-        N, topo.npatch, ppg, d_xyz, d_phi, triq, topo.tri.as<uint32_t>(), topo.patch_ptr.as<int32_t>(), \
-        topo.patch_tri.as<uint32_t>(), topo.patch_slots.as<uint32_t>(), topo.patch_col.as<int32_t>(),   \
+        N, topo.npatch, ppg, d_xyz, d_phi, rec, topo.patch_ptr.as<int32_t>(), topo.patch_col.as<int32_t>(), \
         out.G64, out.G32, out.H, ph, hr)
     if (mirror) {
         if (storeH) { if (w32) BH_FAR(true, true, true); else BH_FAR(true, true, false); }

@@ -26,7 +26,6 @@
 
 namespace bemhip {
 
-constexpr int LU_NB = 256;   // outer block
 constexpr int LEAF = 32;     // cooperative leaf width
 constexpr int PROWS = 256;   // rows per panel work-group (one per thread)
 constexpr int MAXP = 256;    // max panel work-groups -> N <= 65536
@@ -180,6 +179,82 @@ __global__ __launch_bounds__(256) void k_laswp(int ncols, T* __restrict__ A, int
     }
 }
 
+// Speculative leaf without row interchanges: every work-group factors the
+// leaf's top n x n block in LDS (redundantly) and eliminates its own rows
+// against it, with no inter-work-group synchronisation.  max |l| over all
+// multipliers goes to *maxl (bits of a non-negative double; NaN sorts above
+// every number).  If max |l| <= 1, partial pivoting with first-maximum ties
+// (Eigen::PartialPivLU, LAPACK) would have chosen the diagonal at every column,
+// i.e. this is exactly its elimination.
+// The factored top block goes to `topout` (n x n, column-major), never into
+// A: other work-groups may still be reading A's original top rows, so
+// k_copy_top writes it into place after this kernel.
+template <class T>
+__global__ __launch_bounds__(256) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
+                                                 unsigned long long* __restrict__ maxl, T* __restrict__ topout) {
+    __shared__ T top[LEAF][LEAF + 1];
+    __shared__ T as[LEAF][256];
+    __shared__ unsigned long long red[4];
+    const int tid = threadIdx.x;
+    const int r = blockIdx.x * 256 + tid;
+    for (int e = tid; e < n * n; e += 256) {
+        const int i = e % n, j = e / n;
+        top[i][j] = A[(size_t)j * lda + i];
+    }
+    __syncthreads();
+    double lmax = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (tid > j && tid < n) {
+            const T l = top[tid][j] / top[j][j];
+            top[tid][j] = l;
+            const double a = fabs((double)l);
+            if (!(a <= lmax)) lmax = a;   // NaN sticks
+            for (int k = j + 1; k < n; ++k) top[tid][k] -= l * top[j][k];
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x != 0) lmax = 0.0;   // the top block is reported once
+    if (r >= n && r < m) {
+        for (int j = 0; j < n; ++j) as[j][tid] = A[(size_t)j * lda + r];
+        for (int j = 0; j < n; ++j) {
+            const T l = as[j][tid] / top[j][j];
+            as[j][tid] = l;
+            const double a = fabs((double)l);
+            if (!(a <= lmax)) lmax = a;
+            for (int k = j + 1; k < n; ++k) as[k][tid] -= l * top[j][k];
+        }
+        for (int j = 0; j < n; ++j) A[(size_t)j * lda + r] = as[j][tid];
+    }
+    if (blockIdx.x == 0 && tid < n)
+        for (int j = 0; j < n; ++j) topout[(size_t)j * n + tid] = top[tid][j];
+    unsigned long long bits = (unsigned long long)__double_as_longlong(lmax);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = bits;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long b = red[0];
+        for (int w = 1; w < 4; ++w) b = red[w] > b ? red[w] : b;
+        atomicMax(maxl, b);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_copy_top(int n, const T* __restrict__ top, T* __restrict__ A, int lda) {
+    for (int e = threadIdx.x; e < n * n; e += 256) {
+        const int i = e % n, j = e / n;
+        A[(size_t)j * lda + i] = top[(size_t)j * n + i];
+    }
+}
+
+__global__ void k_iota(int* p, int n, int base) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = base + i;
+}
+
 __global__ void k_fill_u32(unsigned int* p, int n, unsigned int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -210,21 +285,21 @@ static void gemm_minus(rocblas_handle h, int m, int n, int k, const float* A, in
 }
 
 // ---------------------------------------------------- split-fp16 update --
-// max |x| over a column-major block (rows x cols, ld) -> bits of a float in *out
+// max |x| over a column-major block (rows x cols, ld) -> bits of a float in *out.
+// grid.x: 256-row tiles, grid.y: 64-column chunks.
 __global__ __launch_bounds__(256) void k_block_absmax(int rows, int cols, const float* __restrict__ X, int ld,
                                                       unsigned int* __restrict__ out) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * 64, c1 = min(cols, c0 + 64);
     float m = 0.0f;
-    const size_t total = (size_t)rows * cols;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = e / rows, r = e % rows;
-        m = fmaxf(m, fabsf(X[c * ld + r]));
-    }
+    if (r < rows)
+        for (int c = c0; c < c1; ++c) m = fmaxf(m, fabsf(X[(size_t)c * ld + r]));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));  // non-negative floats order as uints
 }
 
-// scale exponents: A and B scaled so that max |.| maps to about 2^12
+// scale exponents: A and B scaled so that max |.| maps to [2^11, 2^12)
 __device__ inline int scale_exp(unsigned int bits) {
     const float m = __uint_as_float(bits);
     if (!(m > 0.0f)) return 0;
@@ -236,72 +311,80 @@ __device__ inline int scale_exp(unsigned int bits) {
 __global__ void k_split_alpha(const unsigned int* __restrict__ mx, float* __restrict__ alpha) {
     const int ea = scale_exp(mx[0]), eb = scale_exp(mx[1]);
     alpha[0] = -ldexpf(1.0f, -(ea + eb));
+    alpha[1] = 1.0f;
 }
 
 // Acat (rows x 3K, ld rows) = [hi | hi | lo] of 2^ea A
 __global__ __launch_bounds__(256) void k_split_a(int rows, int K, const float* __restrict__ A, int lda,
                                                  const unsigned int* __restrict__ mx, __half* __restrict__ out) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
     const int ea = scale_exp(mx[0]);
-    const size_t total = (size_t)rows * K;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = e / rows, r = e % rows;
-        const float v = ldexpf(A[c * lda + r], ea);
+    const int c0 = blockIdx.y * 64, c1 = min(K, c0 + 64);
+    for (int c = c0; c < c1; ++c) {
+        const float v = ldexpf(A[(size_t)c * lda + r], ea);
         const __half h = __float2half_rn(v);
         const __half l = __float2half_rn(v - __half2float(h));
-        out[c * rows + r] = h;
-        out[(c + K) * rows + r] = h;
-        out[(c + 2 * K) * rows + r] = l;
+        out[(size_t)c * rows + r] = h;
+        out[(size_t)(c + K) * rows + r] = h;
+        out[(size_t)(c + 2 * K) * rows + r] = l;
     }
 }
 
-// Bcat (3K x cols, ld 3K) = [hi; lo; hi] of 2^eb B
+// Bcat (3K x cols, ld 3K) = [hi; lo; hi] of 2^eb B (K rows)
 __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* __restrict__ B, int ldb,
                                                  const unsigned int* __restrict__ mx, __half* __restrict__ out) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= K) return;
     const int eb = scale_exp(mx[1]);
-    const size_t total = (size_t)K * cols;
     const size_t K3 = 3 * (size_t)K;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = e / K, r = e % K;
-        const float v = ldexpf(B[c * ldb + r], eb);
+    const int c0 = blockIdx.y * 64, c1 = min(cols, c0 + 64);
+    for (int c = c0; c < c1; ++c) {
+        const float v = ldexpf(B[(size_t)c * ldb + r], eb);
         const __half h = __float2half_rn(v);
         const __half l = __float2half_rn(v - __half2float(h));
-        out[c * K3 + r] = h;
-        out[c * K3 + K + r] = l;
-        out[c * K3 + 2 * K + r] = h;
+        out[(size_t)c * K3 + r] = h;
+        out[(size_t)c * K3 + K + r] = l;
+        out[(size_t)c * K3 + 2 * K + r] = h;
     }
 }
 
-static void gemm_minus_split16(bemhip_ctx* ctx, int m, int n, int k, const float* A, int lda, const float* B,
-                               int ldb, float* C, int ldc) {
+// fp16 splits of one outer step's L21 (m x k) and U12 (k x n) blocks
+static void split_prepare(bemhip_ctx* ctx, int m, int n, int k, const float* A, int lda, const float* B, int ldb) {
     __half* acat = static_cast<__half*>(ctx->split_a.ensure(sizeof(__half) * 3 * (size_t)m * k));
     __half* bcat = static_cast<__half*>(ctx->split_b.ensure(sizeof(__half) * 3 * (size_t)k * n));
     unsigned int* mx = static_cast<unsigned int*>(ctx->split_mx.ensure(64));
     float* alpha = reinterpret_cast<float*>(mx + 4);
     hipStream_t st = ctx->stream;
     k_fill_u32<<<1, 64, 0, st>>>(mx, 2, 0u);
-    const int gb = 1024;
-    k_block_absmax<<<gb, 256, 0, st>>>(m, k, A, lda, mx);
-    k_block_absmax<<<gb, 256, 0, st>>>(k, n, B, ldb, mx + 1);
+    k_block_absmax<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx);
+    k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + 1);
     k_split_alpha<<<1, 1, 0, st>>>(mx, alpha);
-    k_split_a<<<gb, 256, 0, st>>>(m, k, A, lda, mx, acat);
-    k_split_b<<<gb, 256, 0, st>>>(k, n, B, ldb, mx, bcat);
+    k_split_a<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx, acat);
+    k_split_b<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx, bcat);
     BH_HIP(hipGetLastError());
-    static const float one = 1.0f;
+}
+
+// C[:, col0:col0+n] -= L21 U12[:, col0:col0+n] from the prepared splits
+static void split_gemm(bemhip_ctx* ctx, int m, int n, int k, int col0, float* C, int ldc) {
+    const __half* acat = ctx->split_a.as<__half>();
+    const __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
+    float* alpha = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4);
     BH_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_device));
-    float* d_one = alpha + 1;
-    k_fill_u32<<<1, 1, 0, st>>>(reinterpret_cast<unsigned int*>(d_one), 1, __builtin_bit_cast(unsigned int, one));
     rocblas_status s = rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, alpha,
-                                       acat, rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, d_one, C,
-                                       rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
+                                       acat, rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, alpha + 1,
+                                       C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
                                        rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
     BH_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
     BH_BLAS(s);
 }
 
-// timed trailing update
+// timed trailing update C -= A B; with split16 the fp16 splits of A and of
+// the full B block row were prepared by split_prepare and `col0` selects the
+// first column of B used here.
 template <class T>
 static void trailing_update(bemhip_ctx* ctx, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C,
-                            int ldc, bool split16) {
+                            int ldc, bool split16, int col0 = 0) {
     if (m <= 0 || n <= 0 || k <= 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -320,7 +403,7 @@ static void trailing_update(bemhip_ctx* ctx, int m, int n, int k, const T* A, in
         BH_HIP(hipEventRecord(e0, ctx->stream));
     }
     if constexpr (std::is_same<T, float>::value) {
-        if (split16) gemm_minus_split16(ctx, m, n, k, A, lda, B, ldb, C, ldc);
+        if (split16) split_gemm(ctx, m, n, k, col0, C, ldc);
         else gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
         gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
@@ -385,10 +468,74 @@ static void panel_rec(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int*
     }
 }
 
-// Blocked right-looking GEPP with one block of look-ahead: while the main
-// stream applies step k's update to the trailing matrix, the panel of step
-// k+1 (whose columns were updated first) is factored on the second stream.
+template <class T>
+static void panel_rec_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+    if (n <= LEAF) {
+        T* top = static_cast<T*>(ctx->lu.leaftop.ensure(sizeof(T) * LEAF * LEAF));
+        k_leaf_np<T><<<(m + 255) / 256, 256, 0, L.s>>>(m, n, A, lda, maxl, top);
+        k_copy_top<T><<<1, 256, 0, L.s>>>(n, top, A, lda);
+        BH_HIP(hipGetLastError());
+        return;
+    }
+    int n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
+    if (n1 >= n) n1 = n - LEAF;
+    const int n2 = n - n1;
+    panel_rec_np(ctx, L, m, n1, A, lda, maxl);
+    T* A12 = A + (size_t)n1 * lda;
+    trsm_llnu(L.h, n1, n2, A, lda, A12, lda);
+    if (m > n1) {
+        gemm_minus(L.h, m - n1, n2, n1, A + n1, lda, A12, lda, A12 + n1, lda);
+        panel_rec_np(ctx, L, m - n1, n2, A12 + n1, lda, maxl);
+    }
+}
+
+// Panel of one outer step.  With speculation on, the panel is saved, factored
+// without interchanges and the largest multiplier is copied to the host; the
+// caller enqueues independent work and then calls panel_finish, which keeps the
+// result when max |l| <= 1 (identical to GEPP's choices) and otherwise
+// restores the panel and factors it with partial pivoting.
+template <class T>
+static void panel_start(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
+                        unsigned int& base) {
+    if (!ctx->speculate) { panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
+    T* bk = static_cast<T*>(ctx->lu.backup.ensure(sizeof(T) * (size_t)m * n));
+    BH_HIP(hipMemcpy2DAsync(bk, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
+    unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
+    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 2, 0u);
+    panel_rec_np(ctx, L, m, n, A, lda, maxl);
+    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, sizeof(double), hipMemcpyDeviceToHost, L.s));
+    BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
+}
+
+// returns true when the panel was factored with row interchanges
+template <class T>
+static bool panel_finish(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
+                         unsigned int& base) {
+    ctx->panels += 1;
+    if (!ctx->speculate) { ctx->panels_pivoted += 1; return true; }
+    BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
+    const double ml = ctx->host_scal[32];
+    if (ml <= 1.0) {
+        k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
+        BH_HIP(hipGetLastError());
+        return false;
+    }
+    const T* bk = ctx->lu.backup.as<T>();
+    BH_HIP(hipMemcpy2DAsync(A, sizeof(T) * lda, bk, sizeof(T) * m, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
+    panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base);
+    ctx->panels_pivoted += 1;
+    return true;
+}
+
+// Blocked right-looking LU with partial pivoting and one block of look-ahead:
+// while the main stream applies step k's update to the trailing matrix, the
+// panel of step k+1 (whose columns were updated first) is factored on the
+// second stream.
 template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16) {
+    // outer block: the split-fp16 update runs at fp16 MFMA speed, so its K must
+    // be larger for the C read/write of each update to stay below the compute
+    const int LU_NB = split16 ? ctx->nb_split : (sizeof(T) == 8 ? ctx->nb64 : ctx->nb32);
+    BH_CHECK(LU_NB >= LEAF && LU_NB % LEAF == 0, BEMHIP_ERR_ARG, "LU block must be a multiple of 32");
     ctx->lu.panel_ws.ensure(PanelWs::BYTES);
     char* ws = ctx->lu.panel_ws.as<char>();
     const Lane main{ctx->stream, ctx->blas};
@@ -401,28 +548,33 @@ template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool s
     // panel 0
     BH_HIP(hipEventRecord(ev_ready, main.s));
     BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-    panel_rec(ctx, side, N, std::min(LU_NB, N), A, lda, ipiv, 0, base);
+    const int jb0 = std::min(LU_NB, N);
+    panel_start(ctx, side, N, jb0, A, lda, ipiv, 0, base);
+    bool pivoted = panel_finish(ctx, side, N, jb0, A, lda, ipiv, 0, base);
     BH_HIP(hipEventRecord(ev_panel, side.s));
     for (int k = 0; k < N; k += LU_NB) {
         const int jb = std::min(LU_NB, N - k);
         T* Akk = A + (size_t)k * lda + k;
         BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));   // panel k factored
-        laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
+        if (pivoted) laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
         const int rest = N - k - jb;
         if (rest <= 0) break;
         T* A12 = A + (size_t)(k + jb) * lda + k;
-        laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
+        if (pivoted) laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
         trsm_llnu(main.h, jb, rest, Akk, lda, A12, lda);
         const int jb2 = std::min(LU_NB, rest);
+        if constexpr (std::is_same<T, float>::value)
+            if (split16) split_prepare(ctx, rest, rest, jb, Akk + jb, lda, A12, lda);
         // next panel's columns first, then hand them to the side stream
-        trailing_update<T>(ctx, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16);
+        trailing_update<T>(ctx, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16, 0);
         BH_HIP(hipEventRecord(ev_ready, main.s));
         BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-        panel_rec(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
-        BH_HIP(hipEventRecord(ev_panel, side.s));
+        panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
         if (rest > jb2)
             trailing_update<T>(ctx, rest, rest - jb2, jb, Akk + jb, lda, A12 + (size_t)jb2 * lda, lda,
-                               A12 + (size_t)jb2 * lda + jb, lda, split16);
+                               A12 + (size_t)jb2 * lda + jb, lda, split16, jb2);
+        pivoted = panel_finish(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        BH_HIP(hipEventRecord(ev_panel, side.s));
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     unsigned int flags[3] = {0, 0, 0};
@@ -493,11 +645,188 @@ void pivots_to_perm(const std::vector<int>& ipiv, std::vector<int>& perm) {
     for (int k = 0; k < n; ++k) std::swap(perm[k], perm[ipiv[k]]);
 }
 
+// ---- persistent triangular solves -------------------------------------
+// FP64 inverses of the TS x TS diagonal blocks (column-major per block), one
+// work-group per block; thread c builds column c of the inverse.
+template <class T, bool LOWER>
+__global__ __launch_bounds__(TS) void k_diag_inv(int N, const T* __restrict__ A, double* __restrict__ inv) {
+    extern __shared__ double ys[];   // [TS][TS]: ys[r * TS + c]
+    const int b = blockIdx.x, c = threadIdx.x;
+    const int j0 = b * TS, bs = min(TS, N - j0);
+    const T* Ab = A + (size_t)j0 * N + j0;
+    if (LOWER) {
+        for (int r = 0; r < TS; ++r) {
+            double v = (r == c) ? 1.0 : 0.0;
+            if (r < bs && r > c)
+                for (int k = c; k < r; ++k) v -= (double)Ab[(size_t)k * N + r] * ys[k * TS + c];
+            if (r >= bs || c >= bs) v = (r == c) ? 1.0 : 0.0;
+            ys[r * TS + c] = v;
+        }
+    } else {
+        for (int r = TS - 1; r >= 0; --r) {
+            double v = (r == c) ? 1.0 : 0.0;
+            if (r < bs && c < bs) {
+                if (r <= c) {
+                    for (int k = r + 1; k <= c; ++k) v -= (double)Ab[(size_t)k * N + r] * ys[k * TS + c];
+                    v /= (double)Ab[(size_t)r * N + r];
+                } else {
+                    v = 0.0;
+                }
+            }
+            ys[r * TS + c] = v;
+        }
+    }
+    double* o = inv + (size_t)b * TS * TS;
+    for (int r = 0; r < TS; ++r) o[(size_t)c * TS + r] = ys[r * TS + c];
+}
+
+// One work-group per TS-row block, taken in dependency order through a ticket
+// (so every block a work-group waits on belongs to an already running one).
+// x holds the right-hand side on entry and the solution on exit.  Block i
+// accumulates A_ij x_j over the blocks j it depends on: one wave polls up to 64
+// dependency flags at once, and the work-group then consumes every ready block
+// (up to TRSV_BATCH per round) before polling again.  Finally
+// x_i = inv_i (b_i - sum) is published and the block's flag raised.
+constexpr int TRSV_BATCH = 16;
+
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict__ A, const double* __restrict__ inv,
+                                                      double* __restrict__ x, unsigned int* __restrict__ sync,
+                                                      unsigned int epoch) {
+    __shared__ double xs[TRSV_BATCH][TS];
+    __shared__ double part[2][TS];
+    __shared__ int s_t, s_ready;
+    unsigned int* ticket = sync;
+    unsigned int* err = sync + 1;
+    unsigned int* flags = sync + 16;
+    const int nb = (N + TS - 1) / TS;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_t = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int t = s_t;
+    const int ib = LOWER ? t : nb - 1 - t;
+    const int i0 = ib * TS, bs = min(TS, N - i0);
+    const int r = tid & (TS - 1), h = tid >> 7;
+    double acc = 0.0;
+    unsigned int spins = 0;
+    for (int q = 0; q < t;) {
+        if (tid < 64) {
+            const int qq = q + tid;
+            bool ok = true;
+            if (qq < t) {
+                const int jb = LOWER ? qq : nb - 1 - qq;
+                ok = __hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+            }
+            const unsigned long long bal = __ballot(ok);
+            int nready = (~bal == 0ull) ? 64 : (__ffsll((long long)~bal) - 1);
+            nready = min(nready, min(t - q, TRSV_BATCH));
+            if (tid == 0) {
+                if (nready == 0) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > (1u << 24) ||
+                        __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        nready = t - q;   // give up waiting: finish (results invalid, error reported)
+                    }
+                }
+                s_ready = nready;
+            }
+        }
+        __syncthreads();
+        const int nr = s_ready;
+        __syncthreads();   // every wave has read s_ready before wave 0 may overwrite it
+        if (nr == 0) continue;
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        for (int e = tid; e < nr * TS; e += 256) {
+            const int b = e / TS, c = e % TS;
+            const int jb = LOWER ? q + b : nb - 1 - (q + b);
+            const int col = jb * TS + c;
+            xs[b][c] = col < N ? __hip_atomic_load(&x[col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        }
+        __syncthreads();
+        if (r < bs) {
+            for (int b = 0; b < nr; ++b) {
+                const int jb = LOWER ? q + b : nb - 1 - (q + b);
+                const int j0 = jb * TS, js = min(TS, N - j0);
+                const int c0 = h * (TS / 2), c1 = min(js, c0 + TS / 2);
+                const T* col = A + (size_t)(j0 + c0) * N + i0 + r;
+                if (c1 - c0 == TS / 2) {
+#pragma unroll
+                    for (int cc = 0; cc < TS / 2; cc += 16) {
+                        T v[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) v[u] = col[(size_t)(cc + u) * N];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) acc += (double)v[u] * xs[b][c0 + cc + u];
+                    }
+                } else {
+                    for (int c = c0; c < c1; ++c, col += N) acc += (double)col[0] * xs[b][c];
+                }
+            }
+        }
+        q += nr;
+        __syncthreads();
+    }
+    part[h][r] = acc;
+    __syncthreads();
+    if (tid < bs) xs[0][tid] = x[i0 + tid] - (part[0][tid] + part[1][tid]);
+    __syncthreads();
+    if (tid < bs) {
+        const double* iv = inv + (size_t)ib * TS * TS + tid;
+        double v0 = 0.0, v1 = 0.0;
+        int c = 0;
+        for (; c + 2 <= bs; c += 2) { v0 += iv[(size_t)c * TS] * xs[0][c]; v1 += iv[(size_t)(c + 1) * TS] * xs[0][c + 1]; }
+        if (c < bs) v0 += iv[(size_t)c * TS] * xs[0][c];
+        __hip_atomic_store(&x[i0 + tid], v0 + v1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU) {
+    const int nb = (N + TS - 1) / TS;
+    double* il = static_cast<double*>(ctx->lu.inv_lo.ensure(sizeof(double) * (size_t)nb * TS * TS));
+    double* iu = static_cast<double*>(ctx->lu.inv_up.ensure(sizeof(double) * (size_t)nb * TS * TS));
+    const size_t sh = sizeof(double) * TS * TS;
+    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    k_diag_inv<T, true><<<nb, TS, sh, ctx->stream>>>(N, LU, il);
+    k_diag_inv<T, false><<<nb, TS, sh, ctx->stream>>>(N, LU, iu);
+    BH_HIP(hipGetLastError());
+    const size_t fb = sizeof(unsigned int) * (16 + (size_t)nb);
+    if (ctx->lu.flag.bytes < fb) {
+        ctx->lu.flag.ensure(fb);
+        BH_HIP(hipMemsetAsync(ctx->lu.flag.p, 0, fb, ctx->stream));
+        ctx->trsv_epoch = 0;
+    }
+    ctx->lu.inv_n = N;
+}
+template void lu_prepare_solve<double>(bemhip_ctx*, int, const double*);
+template void lu_prepare_solve<float>(bemhip_ctx*, int, const float*);
+
+template <class T> static void trsv_persist(bemhip_ctx* ctx, int N, const T* LU, double* x, bool lower) {
+    const int nb = (N + TS - 1) / TS;
+    unsigned int* sync = ctx->lu.flag.as<unsigned int>();
+    const unsigned int epoch = ++ctx->trsv_epoch;
+    k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error flag
+    if (lower) k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch);
+    else k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), x, sync, epoch);
+    BH_HIP(hipGetLastError());
+}
+
 template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int* perm, double* x) {
     hipStream_t st = ctx->stream;
     double* tmp = static_cast<double*>(ctx->vec[7].ensure(sizeof(double) * N));
     BH_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     k_gather<<<(N + 255) / 256, 256, 0, st>>>(N, perm, tmp, x);
+    if (ctx->trsv_persistent && ctx->lu.inv_n == N) {
+        trsv_persist(ctx, N, LU, x, true);
+        trsv_persist(ctx, N, LU, x, false);
+        return;
+    }
     for (int j0 = 0; j0 < N; j0 += TS) {
         const int bs = std::min(TS, N - j0);
         k_trsv_diag<T, true><<<1, TS, 0, st>>>(N, LU, x, j0, bs);
@@ -526,11 +855,14 @@ __global__ __launch_bounds__(256) void k_gemv_part(int N, const double* __restri
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int c = c0;
     const double* col = A + (size_t)c0 * N + r;
-    for (; c + 4 <= c1; c += 4, col += 4 * (size_t)N) {
-        a0 += col[0] * x[c];
-        a1 += col[(size_t)N] * x[c + 1];
-        a2 += col[2 * (size_t)N] * x[c + 2];
-        a3 += col[3 * (size_t)N] * x[c + 3];
+    for (; c + 8 <= c1; c += 8, col += 8 * (size_t)N) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)u * N];
+        a0 += v[0] * x[c];     a1 += v[1] * x[c + 1];
+        a2 += v[2] * x[c + 2]; a3 += v[3] * x[c + 3];
+        a0 += v[4] * x[c + 4]; a1 += v[5] * x[c + 5];
+        a2 += v[6] * x[c + 6]; a3 += v[7] * x[c + 7];
     }
     for (; c < c1; ++c, col += N) a0 += col[0] * x[c];
     part[(size_t)blockIdx.y * N + r] = (a0 + a1) + (a2 + a3);
@@ -703,6 +1035,7 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
     hipStream_t sm = ctx->stream;
     int* ipiv = static_cast<int*>(ctx->lu.ipiv.ensure(sizeof(int) * N));
     ctx->gemm_ms = 0.0; ctx->gemm_flops = 0.0; ctx->gemm_launches = 0;
+    ctx->panels = 0; ctx->panels_pivoted = 0;
     hipEvent_t e0, e1, e2;
     BH_HIP(hipEventCreate(&e0)); BH_HIP(hipEventCreate(&e1)); BH_HIP(hipEventCreate(&e2));
     BH_HIP(hipEventRecord(e0, sm));
@@ -714,6 +1047,7 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
         const int info = lu_factor<double>(ctx, N, G64, ipiv, false);
         BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
         int* perm = make_perm(ctx, N, ipiv);
+        if (ctx->trsv_persistent) lu_prepare_solve<double>(ctx, N, G64);
         BH_HIP(hipEventRecord(e1, sm));
         BH_HIP(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, sm));
         lu_solve<double>(ctx, N, G64, perm, x);
@@ -722,6 +1056,7 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
         const int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
         BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
         int* perm = make_perm(ctx, N, ipiv);
+        if (ctx->trsv_persistent) lu_prepare_solve<float>(ctx, N, G32);
         BH_HIP(hipEventRecord(e1, sm));
         if (tol <= 0.0) tol = 1e-12;
         if (maxit <= 0) maxit = 30;
@@ -754,6 +1089,8 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
     st->gemm_ms = ctx->gemm_ms;
     st->gemm_flops = ctx->gemm_flops;
     st->gemm_launches = ctx->gemm_launches;
+    st->panels = ctx->panels;
+    st->panels_pivoted = ctx->panels_pivoted;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     // final FP64 residual |b - G x| / |b| (not for LU64, whose G was overwritten)
     if (solver != BEMHIP_SOLVE_LU64) {

@@ -94,14 +94,15 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
 // Device copies of the topology tables
 struct DevTopology {
     int64_t N = 0, M = 0;
-    int npatch = 0, nparts = 0;
+    int npatch = 0, nparts = 0, nentries = 0;
     DevBuf tri, patch_col, patch_ptr, patch_tri, patch_slots, vptr, vtri, vpart, part_tptr, part_tri;
     void upload(const Topology& t, hipStream_t s);
 };
 
 // ------------------------------------------------------------- context --
 struct LuWork {
-    DevBuf ipiv, panel_ws, flag;
+    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, leaftop;
+    int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
 };
 
 }  // namespace bemhip
@@ -112,8 +113,13 @@ struct bemhip_ctx {
     rocblas_handle blas = nullptr;
     hipStream_t stream2 = nullptr;      // LU look-ahead: panels of the next block column
     rocblas_handle blas2 = nullptr;
-    hipEvent_t ev_la[2] = {};
+    hipEvent_t ev_la[3] = {};
     bool lookahead = true;
+    bool speculate = true;              // panels without interchanges, verified against GEPP's choice
+    int nb64 = 256, nb32 = 256, nb_split = 512;   // LU outer block per precision
+    int panels = 0, panels_pivoted = 0;
+    bool trsv_persistent = true;
+    unsigned int trsv_epoch = 0;
     std::string err;
     int cus = 0;
     // workspace
@@ -149,6 +155,8 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
 template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
 template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int* ipiv, double* x);
+// after a factorisation: diagonal-block inverses for the persistent solves
+template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU);
 // r = b - A x (A FP64 column-major), or y = A x when b == nullptr
 void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const double* b, double* r);
 // solver front end on a device-resident FP64 G (destroyed by LU64) and rhs b.

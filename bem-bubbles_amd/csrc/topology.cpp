@@ -178,6 +178,7 @@ void DevTopology::upload(const Topology& t, hipStream_t s) {
     M = t.M;
     npatch = t.npatch;
     nparts = t.nparts;
+    nentries = (int)t.patch_tri.size();
     up(tri, t.tri, s);
     up(patch_col, t.patch_col, s);
     up(patch_ptr, t.patch_ptr, s);

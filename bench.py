@@ -117,6 +117,7 @@ def main():
                    "N": N, "M": M, "solver": args.solver, "stages_timed": stages},
         "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
         "refinement_sweeps_per_stage": acc["iterations"] / stages,
+        "lu_panels": acc["panels"], "lu_panels_pivoted": acc["panels_pivoted"],
     }
     gf, gms = acc["gemm_flops"], acc["gemm_ms"]
     if gms > 0:

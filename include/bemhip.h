@@ -64,6 +64,8 @@ typedef struct {
     double gemm_flops;       /* algorithmic flops of those GEMMs (2 m n k each)    */
     int32_t gemm_launches;   /* number of trailing-update GEMM launches            */
     int32_t pad_;
+    int32_t panels;          /* LU panels factored                                 */
+    int32_t panels_pivoted;  /* panels that needed row interchanges                */
 } bemhip_solve_stats;
 
 /* physics and flags of a simulation (Simulation.hpp:29-45, LinLinSim.hpp:22-27) */

@@ -50,6 +50,31 @@ def test_lu64_with_pivoting(ctx):
     b = rng.standard_normal(1000)
     x, st = ctx.solve(A, b, "lu64")
     assert rel(x, np.linalg.solve(A, b)) < 1e-10
+    # the speculative (interchange-free) panel is rejected and GEPP takes over
+    assert st["panels"] >= 4 and st["panels_pivoted"] == st["panels"]
+    for solver in ("lu32_ir64", "lu16x3_ir64"):
+        xm, stm = ctx.solve(A, b, solver)
+        assert rel(xm, np.linalg.solve(A, b)) < 1e-10 and stm["panels_pivoted"] >= 1
+
+
+def test_speculative_panels_equal_gepp(oracle_lib, monkeypatch):
+    """On BEM matrices partial pivoting keeps the diagonal: the interchange-free
+    panels are accepted and give GEPP's factorisation (checked against a
+    context that always runs the GEPP panels)."""
+    say("speculation vs gepp")
+    v, t = meshgen.icosphere(12)
+    G, H = oracle_lib.assemble(v, t)
+    b = H @ (1.0 + 0.05 * meshgen.random_sh_field(v, 1, 4, 5))
+    c1 = bemhip.Context(0)
+    x1, st1 = c1.solve(G, b, "lu64")
+    c1.close()
+    monkeypatch.setenv("BEMHIP_LU_SPECULATE", "0")
+    c2 = bemhip.Context(0)
+    x2, st2 = c2.solve(G, b, "lu64")
+    c2.close()
+    assert st1["panels_pivoted"] == 0 and st2["panels_pivoted"] == st2["panels"] > 0
+    assert rel(x1, x2) < 1e-12
+    assert rel(x1, oracle_lib.solve(G, b)) < 1e-10
 
 
 @pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu32_ir64", 1e-8), ("lu16x3_ir64", 1e-8),
@@ -198,7 +223,8 @@ def test_large_sphere_properties(ctx):
     say("large nu=32")
     v, t = meshgen.icosphere(32)
     psi64, st64 = ctx.compute_psi(v, t, np.ones(len(v)), "lu64")
-    assert abs(psi64.mean() + 1.0) < 1e-4          # psi = -phi0 / R, O(h^2)
+    # psi = -phi0 / R up to the O(h^2) discretisation error (4.2e-3 at nu = 8, /16 at nu = 32)
+    assert abs(psi64.mean() + 1.0) < 4e-4 and np.std(psi64) < 1e-4
     Y = meshgen.real_sph_harm(2, v)[(2, 1)]
     phi = -1.0 + 0.02 * Y
     p64, _ = ctx.compute_psi(v, t, phi, "lu64")
@@ -209,3 +235,23 @@ def test_large_sphere_properties(ctx):
     # linearity: psi(phi) = psi(-1) + 0.02 psi(Y)
     pY, _ = ctx.compute_psi(v, t, Y, "lu64")
     assert rel(p64, -psi64 + 0.02 * pY) < 1e-11
+
+
+def test_large_speculative_lu_equals_gepp(monkeypatch):
+    """nu = 32: the look-ahead LU with interchange-free panels (many
+    work-groups, concurrent streams) reproduces the always-GEPP factorisation."""
+    say("large speculation vs gepp")
+    v, t = meshgen.icosphere(32)
+    phi = -1.0 + 0.02 * meshgen.random_sh_field(v, 1, 8, 0)
+    c1 = bemhip.Context(0)
+    p1, st1 = c1.compute_psi(v, t, phi, "lu64")
+    pm, stm = c1.compute_psi(v, t, phi, "lu16x3_ir64")
+    c1.close()
+    monkeypatch.setenv("BEMHIP_LU_SPECULATE", "0")
+    c2 = bemhip.Context(0)
+    p2, st2 = c2.compute_psi(v, t, phi, "lu64")
+    c2.close()
+    say(st1, st2)
+    assert st1["panels_pivoted"] == 0 and st2["panels_pivoted"] == st2["panels"]
+    assert rel(p1, p2) < 1e-12
+    assert rel(pm, p2) < 1e-8
