@@ -19,6 +19,7 @@ import ctypes as C
 import math
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -28,6 +29,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "bemhip.h")
 
 OK, ERR_ARG, ERR_BAD_MESH, ERR_NEG_X, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_NAN, ERR_OOM, ERR_HIP, ERR_NO_DEVICE = range(10)
 MIRROR_X0 = 0x1
+INDEPENDENT_PARTS = 0x2
 SOLVERS = {"lu64": 0, "lu32_ir64": 1, "bicgstab64": 2, "lu16x3_ir64": 3}
 
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -109,6 +111,11 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_accum_stats.argtypes = [vp, C.POINTER(SolveStats), C.c_int]
     L.bemhip_sim_set_params.argtypes = [vp, C.POINTER(SimParams)]
     L.bemhip_sim_set_mesh.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p]
+    L.bemhip_sim_get_parts.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.bemhip_sim_get_part_volumes.argtypes = [vp, f64p, C.c_int32]
+    L.bemhip_sim_set_part_physics.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+    L.bemhip_sim_adjoint_radius.argtypes = [vp, C.c_int32, C.c_double, f64p, C.c_int32, C.POINTER(C.c_double),
+                                            vp, vp, vp]
     _lib = L
     return L
 
@@ -137,9 +144,14 @@ class Context:
         if rc != OK:
             raise BemHipError(rc, "bemhip_create failed (no gfx950 device?)")
         self.h = h
+        self._sims = weakref.WeakSet()   # simulations living on this context
 
     def close(self):
+        """Destroys the simulations still open on this context, then the context
+        (a simulation must never outlive its context)."""
         if getattr(self, "h", None):
+            for sim in list(getattr(self, "_sims", ())):
+                sim.close()
             self.L.bemhip_destroy(self.h)
             self.h = None
 
@@ -176,15 +188,15 @@ class Context:
         return x, st.as_dict()
 
     # Simulation::compute_psi (Simulation.hpp:73-77)
-    def compute_psi(self, verts, tris, phi, solver="lu64", mirror=False, tol=0.0, maxit=0):
+    def compute_psi(self, verts, tris, phi, solver="lu64", mirror=False, tol=0.0, maxit=0, independent_parts=False):
         v = np.ascontiguousarray(verts, dtype=np.float64)
         t = np.ascontiguousarray(tris, dtype=np.uint32)
         psi = np.zeros(v.shape[0])
         st = SolveStats()
+        flags = (MIRROR_X0 if mirror else 0) | (INDEPENDENT_PARTS if independent_parts else 0)
         self.check(self.L.bemhip_compute_psi(self.h, v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1),
                                              np.ascontiguousarray(phi, dtype=np.float64),
-                                             MIRROR_X0 if mirror else 0, _solver_id(solver), tol, maxit, psi,
-                                             C.byref(st)))
+                                             flags, _solver_id(solver), tol, maxit, psi, C.byref(st)))
         return psi, st.as_dict()
 
 
@@ -193,14 +205,14 @@ class ColocSim:
 
     def __init__(self, ctx: Context, verts, tris, p_inf=1.0, epsilon=1.0, sigma=0.0, gamma=1.0,
                  field_amp=0.0, field_freq=0.0, field_phase=0.0, mirror=False, wall_guards=None,
-                 frozen_stage_time=True, solver="bicgstab64", tol=0.0, maxit=0, V_0=None):
+                 frozen_stage_time=True, solver="bicgstab64", tol=0.0, maxit=0, V_0=None, independent_parts=False):
         self.ctx = ctx
         L = ctx.L
         p = SimParams()
         L.bemhip_sim_default_params(C.byref(p))
         p.p_inf, p.epsilon, p.sigma, p.gamma = p_inf, epsilon, sigma, gamma
         p.field_amp, p.field_freq, p.field_phase = field_amp, field_freq, field_phase
-        p.flags = MIRROR_X0 if mirror else 0
+        p.flags = (MIRROR_X0 if mirror else 0) | (INDEPENDENT_PARTS if independent_parts else 0)
         p.wall_guards = int(mirror if wall_guards is None else wall_guards)
         p.frozen_stage_time = int(frozen_stage_time)
         p.solver, p.tol, p.maxit = _solver_id(solver), tol, maxit
@@ -212,10 +224,12 @@ class ColocSim:
         h = C.c_void_p()
         ctx.check(L.bemhip_sim_create(ctx.h, C.byref(p), self.N, v.reshape(-1), self.M, t.reshape(-1), C.byref(h)))
         self.h = h
+        ctx._sims.add(self)
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.L.bemhip_sim_destroy(self.h)
+            if getattr(self.ctx, "h", None):   # the context destroys its simulations first when it closes
+                self.ctx.L.bemhip_sim_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -249,6 +263,14 @@ class ColocSim:
     def set_bcgstab(self, value: bool):   # Simulation.hpp:153-155
         self.set_solver("bicgstab64" if value else "lu64")
 
+    def set_mesh(self, verts, tris):
+        """New vertex positions / connectivity (remeshing); phi and psi are reset
+        to zero, time and V_0 are kept."""
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        self._check(self.ctx.L.bemhip_sim_set_mesh(self.h, v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1)))
+        self.N, self.M = v.shape[0], t.shape[0]
+
     def state(self):
         x = np.zeros((self.N, 3))
         phi = np.zeros(self.N)
@@ -273,6 +295,40 @@ class ColocSim:
         v = C.c_double()
         self._check(self.ctx.L.bemhip_sim_get_volume(self.h, C.byref(v)))
         return v.value
+
+    # loose parts (Mesh.cpp:382-440) and per-part physics of a batch
+    def nparts(self):
+        n = C.c_int32()
+        self._check(self.ctx.L.bemhip_sim_get_parts(self.h, C.byref(n)))
+        return n.value
+
+    def part_volumes(self):
+        n = self.nparts()
+        out = np.zeros(n)
+        self._check(self.ctx.L.bemhip_sim_get_part_volumes(self.h, out, n))
+        return out
+
+    def adjoint_radius(self, nsteps, dt, target, every=10, want_grad=True):
+        """Loss sum_n (R_n - target_n)^2 over `nsteps` fixed RK4 steps from the
+        current state, the radii R_n, and (when want_grad) dL/dx0 (N x 3) and
+        dL/dphi0 (N); the simulation's state is left unchanged."""
+        tgt = np.ascontiguousarray(target, dtype=np.float64)
+        assert tgt.shape == (nsteps,)
+        loss = C.c_double()
+        R = np.zeros(nsteps)
+        gx = np.zeros((self.N, 3)) if want_grad else None
+        gp = np.zeros(self.N) if want_grad else None
+        self._check(self.ctx.L.bemhip_sim_adjoint_radius(self.h, nsteps, dt, tgt, every, C.byref(loss), _ptr(R),
+                                                         _ptr(gx), _ptr(gp)))
+        return loss.value, R, gx, gp
+
+    def set_part_physics(self, V_0=None, field_amp=None, field_freq=None, field_phase=None):
+        """Per-part V_0 and forcing p_field = amp sin(2 pi freq t + phase); a
+        parameter given as None is left as it is."""
+        n = self.nparts()
+        arrs = [None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,)))
+                for a in (V_0, field_amp, field_freq, field_phase)]
+        self._check(self.ctx.L.bemhip_sim_set_part_physics(self.h, n, *[_ptr(a) for a in arrs]))
 
     # hot path
     def position_t(self):

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <vector>
 
 #include "internal.hpp"
 #include "quad_rules.hpp"
@@ -28,7 +29,12 @@ struct bemhip_sim {
     DevTopology topo;
     int64_t N = 0, M = 0;
     double time = 0.0;
-    DevBuf x, phi, psi, x1, p1, xs, ps, kx, kp, avg, vn, tnorm, tgrad, tvol, pvol, tk, hphi, psil;
+    DevBuf x, phi, psi, x1, p1, xs, ps, kx, kp, avg, vn, tnorm, tgrad, tvol, pvol, tk, hphi, psil, pphys;
+    // per-part V_0, field amplitude, frequency, phase; NaN = the scalar in P
+    std::vector<double> part_v0, part_amp, part_freq, part_phase;
+    // adjoint work space: start state, checkpoints, segment states, cotangents
+    DevBuf a_x0, a_p0, a_ckx, a_ckp, a_sgx, a_sgp, a_kxb, a_kpb, a_xb, a_pb, a_xsb, a_psb, a_psib, a_lam, a_vc, a_kxt,
+        a_kpt;
     bemhip_solve_stats last{};
     bemhip_solve_stats accum{};
 };
@@ -79,10 +85,11 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
 
 // assembly + H phi + solve on device-resident positions / potential
 void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi, uint32_t flags,
-                int solver, double tol, int maxit, double* d_hphi, double* d_psi, bemhip_solve_stats* st) {
-    const int64_t N = topo.N;
-    BH_CHECK(N <= 65536, BEMHIP_ERR_ARG, "dense solve supports N <= 65536");
-    const size_t nn = (size_t)N * (size_t)N;
+                int solver, double tol, int maxit, double* d_hphi, double* d_psi, bemhip_solve_stats* st,
+                bool report_residual = true) {
+    for (const AsmProb& pr : topo.probs)
+        BH_CHECK(pr.n <= 65536, BEMHIP_ERR_ARG, "dense solve supports at most 65536 unknowns per problem");
+    const size_t nn = (size_t)topo.gsize;
     double* G64 = dev<double>(ctx->G64, nn);
     float* G32 = mixed(solver) ? dev<float>(ctx->G32, nn) : nullptr;
     hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
@@ -90,7 +97,31 @@ void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, c
     assemble_device(ctx, topo, d_xyz, d_phi, flags, AsmOut{G64, G32, nullptr, d_hphi});
     BH_HIP(hipEventRecord(e1, ctx->stream));
     const float ams = event_ms(e0, e1);
-    solve_device(ctx, (int)N, G64, G32, d_hphi, d_psi, solver, tol, maxit, st);
+    if (topo.probs.size() == 1) {
+        solve_device(ctx, (int)topo.N, G64, G32, d_hphi, d_psi, solver, tol, maxit, st, report_residual);
+    } else {
+        // independent problems one after the other on the context's streams;
+        // every solve reuses the same work buffers in stream order
+        bemhip_solve_stats sum{};
+        sum.converged = 1;
+        for (const AsmProb& pr : topo.probs) {
+            bemhip_solve_stats one{};
+            solve_device(ctx, pr.n, G64 + pr.gofs, G32 ? G32 + pr.gofs : nullptr, d_hphi + pr.v0, d_psi + pr.v0,
+                         solver, tol, maxit, &one, report_residual);
+            sum.iterations = std::max(sum.iterations, one.iterations);
+            sum.converged = sum.converged && one.converged;
+            sum.residual = std::max(sum.residual, one.residual);
+            sum.correction = std::max(sum.correction, one.correction);
+            sum.ms_factor += one.ms_factor;
+            sum.ms_solve += one.ms_solve;
+            sum.gemm_ms += one.gemm_ms;
+            sum.gemm_flops += one.gemm_flops;
+            sum.gemm_launches += one.gemm_launches;
+            sum.panels += one.panels;
+            sum.panels_pivoted += one.panels_pivoted;
+        }
+        *st = sum;
+    }
     st->ms_assemble = ams;
 }
 
@@ -111,11 +142,38 @@ void accumulate(bemhip_solve_stats& a, const bemhip_solve_stats& s) {
     a.pad_ = 1;
 }
 
+// resolved per-part Bernoulli parameters [V_0, amp, freq, phase] x nparts on the device
+void upload_part_physics(bemhip_sim* s) {
+    const int np = s->topo.nparts;
+    for (auto* v : {&s->part_v0, &s->part_amp, &s->part_freq, &s->part_phase})
+        if ((int)v->size() != np) v->assign(np, std::numeric_limits<double>::quiet_NaN());
+    std::vector<double> h(4 * (size_t)np);
+    auto pick = [](double a, double d) { return std::isnan(a) ? d : a; };
+    for (int c = 0; c < np; ++c) {
+        h[4 * c + 0] = pick(s->part_v0[c], s->P.V_0);
+        h[4 * c + 1] = pick(s->part_amp[c], s->P.field_amp);
+        h[4 * c + 2] = pick(s->part_freq[c], s->P.field_freq);
+        h[4 * c + 3] = pick(s->part_phase[c], s->P.field_phase);
+    }
+    double* d = dev<double>(s->pphys, h.size());
+    h2d(d, h.data(), sizeof(double) * h.size(), s->ctx->stream);
+    BH_HIP(hipStreamSynchronize(s->ctx->stream));
+}
+
+// signed volume of every loose part at positions x (host copy)
+std::vector<double> part_volumes_host(bemhip_sim* s, const double* x) {
+    double* tvol = dev<double>(s->tvol, s->M);
+    double* pvol = dev<double>(s->pvol, s->topo.nparts);
+    tri_geometry(s->ctx, s->topo, x, nullptr, nullptr, nullptr, tvol);
+    part_volumes(s->ctx, s->topo, tvol, pvol);
+    std::vector<double> pv(s->topo.nparts);
+    d2h(pv.data(), pvol, sizeof(double) * pv.size(), s->ctx->stream);
+    return pv;
+}
+
 StageParams stage_params(const bemhip_sim* s) {
     StageParams sp;
     sp.p_inf = s->P.p_inf; sp.epsilon = s->P.epsilon; sp.sigma = s->P.sigma; sp.gamma = s->P.gamma;
-    sp.V_0 = s->P.V_0;
-    sp.field_amp = s->P.field_amp; sp.field_freq = s->P.field_freq; sp.field_phase = s->P.field_phase;
     return sp;
 }
 
@@ -127,7 +185,7 @@ void stage(bemhip_sim* s, const double* xs, const double* ps, double t, double* 
     const int64_t N = s->N, M = s->M;
     double* hphi = dev<double>(s->hphi, N);
     double* psil = dev<double>(s->psil, N);
-    psi_device(ctx, s->topo, xs, ps, s->P.flags, s->P.solver, s->P.tol, s->P.maxit, hphi, psil, &s->last);
+    psi_device(ctx, s->topo, xs, ps, s->P.flags, s->P.solver, s->P.tol, s->P.maxit, hphi, psil, &s->last, false);
     accumulate(s->accum, s->last);
     double* tnorm = dev<double>(s->tnorm, 3 * M);
     double* tgrad = dev<double>(s->tgrad, 3 * M);
@@ -144,7 +202,7 @@ void stage(bemhip_sim* s, const double* xs, const double* ps, double t, double* 
         vertex_normals(ctx, s->topo, xs, vn);
         tri_curvature(ctx, s->topo, xs, vn, tk);
     }
-    vertex_dpot(ctx, s->topo, stage_params(s), xs, kx, tk, pvol, t, kp);
+    vertex_dpot(ctx, s->topo, stage_params(s), xs, kx, tk, pvol, s->pphys.as<double>(), t, kp);
 }
 
 double get_dt(bemhip_sim* s, double dp, const double* kx, const double* kp) {  // Simulation.cpp:77-92
@@ -184,6 +242,7 @@ int bemhip_create(int device, bemhip_ctx** out) {
         BH_HIP(hipHostMalloc((void**)&ctx->host_scal, 64 * sizeof(double), hipHostMallocDefault));
         for (auto& e : ctx->ev) BH_HIP(hipEventCreate(&e));
         for (auto& e : ctx->ev_la) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (const char* v = std::getenv("BEMHIP_LA_MIN_REST")) ctx->la_min_rest = std::atoi(v);
         const char* la = std::getenv("BEMHIP_LU_LOOKAHEAD");
         if (la && la[0] == '0') ctx->lookahead = false;
         const char* sp = std::getenv("BEMHIP_LU_SPECULATE");
@@ -294,8 +353,9 @@ int bemhip_compute_psi(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M,
     if (!ctx) return BEMHIP_ERR_ARG;
     return guarded(ctx, [&] {
         BH_CHECK(xyz && tri && phi && psi, BEMHIP_ERR_ARG, "null argument");
+        BH_CHECK(!(flags & ~(BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)), BEMHIP_ERR_ARG, "unknown flag");
         check_finite(phi, N, "phi");
-        Topology th = build_topology(N, xyz, M, tri);
+        Topology th = build_topology(N, xyz, M, tri, (flags & BEMHIP_INDEPENDENT_PARTS) != 0);
         DevTopology topo;
         topo.upload(th, ctx->stream);
         double* dx = dev<double>(ctx->xyz, 3 * N);
@@ -336,30 +396,27 @@ int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, co
     auto s = std::make_unique<bemhip_sim>();
     int rc = guarded(ctx, [&] {
         BH_CHECK(p && xyz && tri, BEMHIP_ERR_ARG, "null argument");
-        BH_CHECK(!(p->flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown flag");
+        BH_CHECK(!(p->flags & ~(BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)), BEMHIP_ERR_ARG, "unknown flag");
         BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
         s->ctx = ctx;
         s->P = *p;
         s->N = N;
         s->M = M;
-        s->th = build_topology(N, xyz, M, tri);
+        s->th = build_topology(N, xyz, M, tri, (p->flags & BEMHIP_INDEPENDENT_PARTS) != 0);
         s->topo.upload(s->th, ctx->stream);
         double* x = dev<double>(s->x, 3 * N);
         h2d(x, xyz, sizeof(double) * 3 * N, ctx->stream);
         BH_HIP(hipMemsetAsync(dev<double>(s->phi, N), 0, sizeof(double) * N, ctx->stream));   // LinLinSim.hpp:28-29
         BH_HIP(hipMemsetAsync(dev<double>(s->psi, N), 0, sizeof(double) * N, ctx->stream));
         if (std::isnan(s->P.V_0)) {   // Simulation.hpp:42: total volume of the initial mesh
-            double* tvol = dev<double>(s->tvol, M);
-            double* pvol = dev<double>(s->pvol, s->topo.nparts);
-            tri_geometry(ctx, s->topo, x, nullptr, nullptr, nullptr, tvol);
-            part_volumes(ctx, s->topo, tvol, pvol);
-            std::vector<double> pv(s->topo.nparts);
-            d2h(pv.data(), pvol, sizeof(double) * pv.size(), ctx->stream);
+            const std::vector<double> pv = part_volumes_host(s.get(), x);
             double v = 0.0;
             for (double a : pv) v += a;
             s->P.V_0 = v;
+            // a batch of independent bubbles: each part is its own simulation
+            if (s->th.independent) s->part_v0 = pv;
         }
-        BH_HIP(hipStreamSynchronize(ctx->stream));
+        upload_part_physics(s.get());
     });
     if (rc != BEMHIP_OK) return rc;
     *out = s.release();
@@ -394,8 +451,46 @@ int bemhip_sim_set_psi(bemhip_sim* s, const double* psi) {
 
 int bemhip_sim_set_V_0(bemhip_sim* s, double v) {
     if (!s) return BEMHIP_ERR_ARG;
-    s->P.V_0 = v;
+    return guarded(s->ctx, [&] {
+        s->P.V_0 = v;
+        s->part_v0.assign(s->topo.nparts, std::numeric_limits<double>::quiet_NaN());
+        upload_part_physics(s);
+    });
+}
+
+int bemhip_sim_set_part_physics(bemhip_sim* s, int32_t nparts, const double* V_0, const double* amp,
+                                const double* freq, const double* phase) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(nparts == s->topo.nparts, BEMHIP_ERR_ARG,
+                 "mesh has " + std::to_string(s->topo.nparts) + " parts, got " + std::to_string(nparts));
+        auto set = [&](std::vector<double>& dst, const double* src) {
+            if (!src) return;   // unchanged
+            check_finite(src, nparts, "per-part parameter");
+            dst.assign(src, src + nparts);
+        };
+        set(s->part_v0, V_0);
+        set(s->part_amp, amp);
+        set(s->part_freq, freq);
+        set(s->part_phase, phase);
+        upload_part_physics(s);
+    });
+}
+
+int bemhip_sim_get_parts(bemhip_sim* s, int32_t* nparts) {
+    if (!s || !nparts) return BEMHIP_ERR_ARG;
+    *nparts = s->topo.nparts;
     return BEMHIP_OK;
+}
+
+int bemhip_sim_get_part_volumes(bemhip_sim* s, double* vol, int32_t nparts) {
+    if (!s || !vol) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(nparts == s->topo.nparts, BEMHIP_ERR_ARG,
+                 "mesh has " + std::to_string(s->topo.nparts) + " parts, got " + std::to_string(nparts));
+        const std::vector<double> pv = part_volumes_host(s, s->x.as<double>());
+        std::copy(pv.begin(), pv.end(), vol);
+    });
 }
 
 int bemhip_sim_set_time(bemhip_sim* s, double t) {
@@ -416,11 +511,15 @@ int bemhip_sim_set_solver(bemhip_sim* s, int solver, double tol, int32_t maxit) 
 int bemhip_sim_set_params(bemhip_sim* s, const bemhip_sim_params* p) {
     if (!s || !p) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {
-        BH_CHECK(!(p->flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown flag");
+        BH_CHECK(!(p->flags & ~(BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)), BEMHIP_ERR_ARG, "unknown flag");
+        BH_CHECK(!((p->flags ^ s->P.flags) & BEMHIP_INDEPENDENT_PARTS), BEMHIP_ERR_ARG,
+                 "BEMHIP_INDEPENDENT_PARTS is fixed when the simulation is created");
         BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
         const double v0 = s->P.V_0;
         s->P = *p;
         if (std::isnan(p->V_0)) s->P.V_0 = v0;
+        else s->part_v0.assign(s->topo.nparts, std::numeric_limits<double>::quiet_NaN());
+        upload_part_physics(s);
     });
 }
 
@@ -429,7 +528,8 @@ int bemhip_sim_set_mesh(bemhip_sim* s, int64_t N, const double* xyz, int64_t M, 
     return guarded(s->ctx, [&] {
         BH_CHECK(xyz && tri, BEMHIP_ERR_ARG, "null argument");
         bemhip_ctx* ctx = s->ctx;
-        Topology th = build_topology(N, xyz, M, tri);
+        Topology th = build_topology(N, xyz, M, tri, (s->P.flags & BEMHIP_INDEPENDENT_PARTS) != 0);
+        const int old_parts = s->topo.nparts;
         s->th = std::move(th);
         s->topo.upload(s->th, ctx->stream);
         s->N = N;
@@ -438,7 +538,13 @@ int bemhip_sim_set_mesh(bemhip_sim* s, int64_t N, const double* xyz, int64_t M, 
         h2d(x, xyz, sizeof(double) * 3 * N, ctx->stream);
         BH_HIP(hipMemsetAsync(dev<double>(s->phi, N), 0, sizeof(double) * N, ctx->stream));
         BH_HIP(hipMemsetAsync(dev<double>(s->psi, N), 0, sizeof(double) * N, ctx->stream));
-        BH_HIP(hipStreamSynchronize(ctx->stream));
+        if (s->topo.nparts != old_parts) {
+            // per-part parameters no longer match: back to the scalars, and in a
+            // batch every part starts from its own volume
+            for (auto* v : {&s->part_v0, &s->part_amp, &s->part_freq, &s->part_phase}) v->clear();
+            if (s->th.independent) s->part_v0 = part_volumes_host(s, x);
+        }
+        upload_part_physics(s);
     });
 }
 
@@ -455,12 +561,7 @@ int bemhip_sim_get_state(bemhip_sim* s, double* xyz, double* phi, double* psi, d
 int bemhip_sim_get_volume(bemhip_sim* s, double* vol) {
     if (!s || !vol) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {
-        double* tvol = dev<double>(s->tvol, s->M);
-        double* pvol = dev<double>(s->pvol, s->topo.nparts);
-        tri_geometry(s->ctx, s->topo, s->x.as<double>(), nullptr, nullptr, nullptr, tvol);
-        part_volumes(s->ctx, s->topo, tvol, pvol);
-        std::vector<double> pv(s->topo.nparts);
-        d2h(pv.data(), pvol, sizeof(double) * pv.size(), s->ctx->stream);
+        const std::vector<double> pv = part_volumes_host(s, s->x.as<double>());
         double v = 0.0;
         for (double a : pv) v += a;
         *vol = v;
@@ -497,7 +598,8 @@ int bemhip_sim_pot_t_multi(bemhip_sim* s, const double* vel, double t, double* d
             vertex_normals(s->ctx, s->topo, s->x.as<double>(), vn);
             tri_curvature(s->ctx, s->topo, s->x.as<double>(), vn, tk);
         }
-        vertex_dpot(s->ctx, s->topo, stage_params(s), s->x.as<double>(), kx, tk, pvol, t, kp);
+        vertex_dpot(s->ctx, s->topo, stage_params(s), s->x.as<double>(), kx, tk, pvol, s->pphys.as<double>(), t,
+                    kp);
         d2h(dpot, kp, sizeof(double) * N, s->ctx->stream);
     });
 }
@@ -587,6 +689,182 @@ int bemhip_sim_accum_stats(bemhip_sim* s, bemhip_solve_stats* st, int reset) {
     if (st) *st = s->accum;
     if (reset) s->accum = bemhip_solve_stats{};
     return BEMHIP_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------- adjoint --
+namespace {
+
+// one RK4 step without wall guards or the psi projection (LinLinSim.cpp:194-241):
+// (x, phi) at t0 -> (x, phi) at t0 + dt; the stage slopes stay in s->kx, s->kp
+// and the start state in s->x1, s->p1
+void rk4_plain(bemhip_sim* s, double* x, double* phi, double t0, double dt) {
+    bemhip_ctx* ctx = s->ctx;
+    const int64_t N = s->N;
+    const size_t n3 = 3 * (size_t)N;
+    double* x1 = dev<double>(s->x1, n3);
+    double* p1 = dev<double>(s->p1, N);
+    double* xs = dev<double>(s->xs, n3);
+    double* ps = dev<double>(s->ps, N);
+    double* kx = dev<double>(s->kx, 4 * n3);
+    double* kp = dev<double>(s->kp, 4 * (size_t)N);
+    BH_HIP(hipMemcpyAsync(x1, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, ctx->stream));
+    BH_HIP(hipMemcpyAsync(p1, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool frozen = s->P.frozen_stage_time != 0;
+    const double cs[3] = {0.5 * dt, 0.5 * dt, dt};
+    const double ts[4] = {t0, frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + dt};
+    stage(s, x1, p1, ts[0], kx, kp);
+    BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "a stage solve stopped above tolerance");
+    for (int k = 1; k < 4; ++k) {
+        axpy_state(ctx, N, cs[k - 1], kx + (k - 1) * n3, x1, xs, kp + (size_t)(k - 1) * N, p1, ps);
+        stage(s, xs, ps, ts[k], kx + k * n3, kp + (size_t)k * N);
+        BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "a stage solve stopped above tolerance");
+    }
+    double* avg = dev<double>(s->avg, n3);
+    rk4_final(ctx, N, dt, kx, kp, x1, x, phi, avg, 0, 0.0);
+}
+
+double total_volume(bemhip_sim* s, const double* x) {
+    double v = 0.0;
+    for (double a : part_volumes_host(s, x)) v += a;
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bemhip_sim_adjoint_radius(bemhip_sim* s, int32_t nsteps, double dt, const double* target, int32_t every,
+                              double* loss, double* radius, double* grad_x, double* grad_phi) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(target && loss && nsteps > 0 && dt > 0.0 && every > 0, BEMHIP_ERR_ARG, "bad arguments");
+        BH_CHECK(s->P.sigma == 0.0, BEMHIP_ERR_ARG, "the adjoint supports sigma = 0 only");
+        BH_CHECK(!(s->P.flags & (BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)) && !s->P.wall_guards, BEMHIP_ERR_ARG,
+                 "the adjoint supports the free-space, coupled problem only");
+        BH_CHECK(s->P.solver != BEMHIP_SOLVE_BICGSTAB64, BEMHIP_ERR_ARG, "the adjoint needs an LU-based solver");
+        bemhip_ctx* ctx = s->ctx;
+        hipStream_t st = ctx->stream;
+        const int64_t N = s->N;
+        const size_t n3 = 3 * (size_t)N;
+        const double t0 = s->time;
+        double* x = s->x.as<double>();
+        double* phi = s->phi.as<double>();
+        double* x0 = dev<double>(s->a_x0, n3);
+        double* p0 = dev<double>(s->a_p0, N);
+        BH_HIP(hipMemcpyAsync(x0, x, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+        BH_HIP(hipMemcpyAsync(p0, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        const int nck = (nsteps + every - 1) / every;
+        double* ckx = dev<double>(s->a_ckx, n3 * nck);
+        double* ckp = dev<double>(s->a_ckp, (size_t)N * nck);
+        std::vector<double> R(nsteps), V(nsteps);
+
+        // forward sweep with a checkpoint every `every` steps
+        double L = 0.0;
+        for (int n = 0; n < nsteps; ++n) {
+            if (n % every == 0) {
+                BH_HIP(hipMemcpyAsync(ckx + n3 * (n / every), x, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                BH_HIP(hipMemcpyAsync(ckp + (size_t)N * (n / every), phi, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                      st));
+            }
+            rk4_plain(s, x, phi, t0 + n * dt, dt);
+            V[n] = total_volume(s, x);
+            R[n] = std::cbrt(3.0 * V[n] / (4.0 * PI));   // volume-equivalent radius
+            L += (R[n] - target[n]) * (R[n] - target[n]);
+        }
+
+        // reverse sweep, segment by segment
+        double* xb = dev<double>(s->a_xb, n3);
+        double* pb = dev<double>(s->a_pb, N);
+        BH_HIP(hipMemsetAsync(xb, 0, sizeof(double) * n3, st));
+        BH_HIP(hipMemsetAsync(pb, 0, sizeof(double) * N, st));
+        double* sgx = dev<double>(s->a_sgx, n3 * every);
+        double* sgp = dev<double>(s->a_sgp, (size_t)N * every);
+        double* kxb = dev<double>(s->a_kxb, 4 * n3);
+        double* kpb = dev<double>(s->a_kpb, 4 * (size_t)N);
+        double* xsb = dev<double>(s->a_xsb, n3);
+        double* psb = dev<double>(s->a_psb, N);
+        double* psib = dev<double>(s->a_psib, N);
+        double* lam = dev<double>(s->a_lam, N);
+        double* kxt = dev<double>(s->a_kxt, n3);
+        double* kpt = dev<double>(s->a_kpt, N);
+        const int np = s->topo.nparts;
+        double* vc = dev<double>(s->a_vc, np);
+        const bool frozen = s->P.frozen_stage_time != 0;
+        const bool mixed_lu = mixed(s->P.solver);
+        StageParams sp = stage_params(s);
+        for (int c = nck - 1; c >= 0; --c) {
+            const int n0 = c * every, n1 = std::min(nsteps, n0 + every);
+            BH_HIP(hipMemcpyAsync(x, ckx + n3 * c, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+            BH_HIP(hipMemcpyAsync(phi, ckp + (size_t)N * c, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+            for (int n = n0; n < n1; ++n) {
+                BH_HIP(hipMemcpyAsync(sgx + n3 * (n - n0), x, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                BH_HIP(hipMemcpyAsync(sgp + (size_t)N * (n - n0), phi, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                      st));
+                if (n + 1 < n1) rk4_plain(s, x, phi, t0 + n * dt, dt);
+            }
+            for (int n = n1 - 1; n >= n0; --n) {
+                // step n again from its start state: slopes in s->kx / s->kp, x_{n+1} in x
+                BH_HIP(hipMemcpyAsync(x, sgx + n3 * (n - n0), sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                BH_HIP(hipMemcpyAsync(phi, sgp + (size_t)N * (n - n0), sizeof(double) * N,
+                                      hipMemcpyDeviceToDevice, st));
+                const double tn = t0 + n * dt;
+                rk4_plain(s, x, phi, tn, dt);
+                // loss term of x_{n+1}: d(R - target)^2/dV = 2 (R - target) R / (3 V)
+                const double cf = 2.0 * (R[n] - target[n]) * R[n] / (3.0 * V[n]);
+                std::vector<double> h(np, cf);
+                h2d(vc, h.data(), sizeof(double) * np, st);
+                adj_volume(ctx, s->topo, x, vc, xb);
+                // slope cotangents (dt/6) (1, 2, 2, 1) of the RK4 combination
+                const double wgt[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+                for (int k = 0; k < 4; ++k) axpy_pair(ctx, N, wgt[k], xb, kxb + k * n3, pb, kpb + (size_t)k * N, false);
+                const double* x1 = s->x1.as<double>();
+                const double* p1 = s->p1.as<double>();
+                const double* kx = s->kx.as<double>();
+                const double* kp = s->kp.as<double>();
+                double* xs = s->xs.as<double>();
+                double* ps = s->ps.as<double>();
+                const double cs[3] = {0.5 * dt, 0.5 * dt, dt};
+                const double ts[4] = {tn, frozen ? tn : tn + 0.5 * dt, frozen ? tn : tn + 0.5 * dt,
+                                      frozen ? tn : tn + dt};
+                for (int k = 3; k >= 0; --k) {
+                    const double* xk = x1;
+                    const double* pk = p1;
+                    if (k > 0) {
+                        axpy_state(ctx, N, cs[k - 1], kx + (k - 1) * n3, x1, xs, kp + (size_t)(k - 1) * N, p1, ps);
+                        xk = xs;
+                        pk = ps;
+                    }
+                    // the stage again: its LU factors, psi and part volumes
+                    stage(s, xk, pk, ts[k], kxt, kpt);
+                    AdjStageIn in{xk, pk, s->psil.as<double>(), kxt, s->pvol.as<double>(), s->pphys.as<double>(),
+                                  sp.epsilon, sp.gamma};
+                    adj_stage_pre(ctx, s->topo, in, kxb + k * n3, kpb + (size_t)k * N, psib);
+                    bemhip_solve_stats tst{};
+                    solve_transposed_device(ctx, (int)N, ctx->G64.as<double>(),
+                                            mixed_lu ? ctx->G32.as<float>() : nullptr, s->P.solver, psib, lam,
+                                            s->P.tol, s->P.maxit, &tst);
+                    BH_CHECK(tst.converged, BEMHIP_ERR_NOT_CONVERGED, "transposed solve stopped above tolerance");
+                    BH_HIP(hipMemsetAsync(xsb, 0, sizeof(double) * n3, st));
+                    BH_HIP(hipMemsetAsync(psb, 0, sizeof(double) * N, st));
+                    adj_stage_post(ctx, s->topo, in, lam, xsb, psb);
+                    // the stage input is x1 + c kx_{k-1}: both receive its cotangent
+                    axpy_pair(ctx, N, 1.0, xsb, xb, psb, pb, true);
+                    if (k > 0) axpy_pair(ctx, N, cs[k - 1], xsb, kxb + (k - 1) * n3, psb, kpb + (size_t)(k - 1) * N, true);
+                }
+            }
+        }
+        if (grad_x) d2h(grad_x, xb, sizeof(double) * n3, st);
+        if (grad_phi) d2h(grad_phi, pb, sizeof(double) * N, st);
+        if (radius) std::copy(R.begin(), R.end(), radius);
+        *loss = L;
+        // leave the simulation where it started
+        BH_HIP(hipMemcpyAsync(x, x0, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+        BH_HIP(hipMemcpyAsync(phi, p0, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        BH_HIP(hipStreamSynchronize(st));
+        s->time = t0;
+    });
 }
 
 }  // extern "C"

@@ -13,8 +13,12 @@
 //                column patch): the 7 quadrature points, unit normal, Jacobian
 //                (|ab x bc|), the potential at its vertices, vertex ids and
 //                patch slots -- one contiguous 256-byte record.
-//   k_far        work-group = 128 collocation rows (one per lane) x a group of
-//                column patches.  A patch is 32 mesh-adjacent vertices; the
+//   k_far        work-group = 128 collocation rows (one per lane) x a range of
+//                column patches, one item of a host-built work table: one dense
+//                problem for a coupled mesh, one per loose part for a batch of
+//                independent bubbles (each with its own G block, about 8192
+//                work-groups in all, shared in proportion to n^2).
+//                A patch is 32 mesh-adjacent vertices; the
 //                work-group streams the triangles incident to the patch (wave-
 //                uniform, scalar loads) and accumulates each lane's 32 G (and
 //                optionally H) columns in LDS, private per lane, so no atomics
@@ -29,6 +33,7 @@
 
 #include "internal.hpp"
 #include "quad_rules.hpp"
+#include "vec3.hpp"
 
 namespace bemhip {
 
@@ -82,22 +87,24 @@ __global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __re
 
 template <bool MIRROR, bool STORE_H, bool WRITE32>
 __global__ __launch_bounds__(ASM_ROWS) void k_far(
-    int N, int npatch, int patches_per_group, const double* __restrict__ xyz, const double* __restrict__ phi,
-    const double* __restrict__ rec, const int32_t* __restrict__ pptr, const int32_t* __restrict__ pcol,
-    double* __restrict__ G, float* __restrict__ G32, double* __restrict__ H, double* __restrict__ part_hphi,
-    double* __restrict__ part_hrow) {
+    int Ntot, const AsmItem* __restrict__ items, const AsmProb* __restrict__ probs, const double* __restrict__ xyz,
+    const double* __restrict__ phi, const double* __restrict__ rec, const int32_t* __restrict__ pptr,
+    const int32_t* __restrict__ pcol, double* __restrict__ G, float* __restrict__ G32, double* __restrict__ H,
+    double* __restrict__ part_hphi, double* __restrict__ part_hrow) {
     __shared__ double accG[PATCH][ASM_ROWS];
     __shared__ double accH[STORE_H ? PATCH : 1][ASM_ROWS];
+    const AsmItem it = items[blockIdx.x];
+    const AsmProb pr = probs[it.prob];
     const int tid = threadIdx.x;
-    const int row = blockIdx.x * ASM_ROWS + tid;
-    const bool valid = row < N;
-    const int rr = valid ? row : N - 1;
-    const double xi = xyz[3 * rr], yi = xyz[3 * rr + 1], zi = xyz[3 * rr + 2];
-    const double phii = phi ? phi[rr] : 0.0;
+    const int rloc = it.row0 + tid;                 // row inside the problem
+    const bool valid = rloc < pr.n;
+    const int row = pr.v0 + (valid ? rloc : pr.n - 1);   // global vertex id
+    const size_t ld = (size_t)pr.n;
+    const double xi = xyz[3 * row], yi = xyz[3 * row + 1], zi = xyz[3 * row + 2];
+    const double phii = phi ? phi[row] : 0.0;
     double hphi = 0.0, hrow = 0.0;
 
-    const int p0 = blockIdx.y * patches_per_group;
-    const int p1 = min(npatch, p0 + patches_per_group);
+    const int p0 = it.p0, p1 = it.p1;
     for (int p = p0; p < p1; ++p) {
 #pragma unroll
         for (int s = 0; s < PATCH; ++s) {
@@ -119,48 +126,51 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
             for (int k = 0; k < 7; ++k) {
                 const double zx = q[3 * k] - xi, zy = q[3 * k + 1] - yi, zz = q[3 * k + 2] - zi;
                 const double yz2 = zy * zy + zz * zz;
-                const double inv = rsqrt(zx * zx + yz2);
+                const double inv = rsqrt_nr(zx * zx + yz2);
                 const double inv3 = inv * inv * inv;
                 g0 += basis_w(k, 0) * inv; g1 += basis_w(k, 1) * inv; g2 += basis_w(k, 2) * inv;
                 s0 += basis_w(k, 0) * inv3; s1 += basis_w(k, 1) * inv3; s2 += basis_w(k, 2) * inv3;
                 if constexpr (MIRROR) {
                     const double zxm = -q[3 * k] - xi;
-                    const double invm = rsqrt(zxm * zxm + yz2);
+                    const double invm = rsqrt_nr(zxm * zxm + yz2);
                     const double invm3 = invm * invm * invm;
                     gm0 += basis_w(k, 0) * invm; gm1 += basis_w(k, 1) * invm; gm2 += basis_w(k, 2) * invm;
                     sm0 += basis_w(k, 0) * invm3; sm1 += basis_w(k, 1) * invm3; sm2 += basis_w(k, 2) * invm3;
                 }
             }
-            // i in T: the direct part is the self panel (added by k_self), H = 0
+            // i in T: the direct part is the self panel (added by k_self) and H = 0.
+            // The quadrature points are strictly inside T (barycentric >= 0.0597),
+            // so g, s stay finite there and a zero Jacobian removes them.
             const bool far = ((uint32_t)row != v0) & ((uint32_t)row != v1) & ((uint32_t)row != v2);
-            double G0 = far ? g0 : 0.0, G1 = far ? g1 : 0.0, G2 = far ? g2 : 0.0;
-            double H0 = far ? -zn * s0 : 0.0, H1 = far ? -zn * s1 : 0.0, H2 = far ? -zn * s2 : 0.0;
+            const double Jf = far ? J : 0.0;
+            const double Hf = -zn * Jf;
+            double G0 = Jf * g0, G1 = Jf * g1, G2 = Jf * g2;
+            double H0 = Hf * s0, H1 = Hf * s1, H2 = Hf * s2;
             if constexpr (MIRROR) {
                 // image point R y (R: x -> -x) with normal R n: (Ry - x).(Rn) = (y - Rx).n
                 const double znm = (q[0] + xi) * nx + dy * ny + dz * nz;
-                G0 += gm0; G1 += gm1; G2 += gm2;
-                H0 += -znm * sm0; H1 += -znm * sm1; H2 += -znm * sm2;
+                const double Hm = -znm * J;
+                G0 += J * gm0; G1 += J * gm1; G2 += J * gm2;
+                H0 += Hm * sm0; H1 += Hm * sm1; H2 += Hm * sm2;
             }
-            G0 *= J; G1 *= J; G2 *= J;
-            H0 *= J; H1 *= J; H2 *= J;
             const uint32_t sa = sl & 0xFFu, sb = (sl >> 8) & 0xFFu, sc = (sl >> 16) & 0xFFu;
             if (sa != 0xFFu) {
                 accG[sa][tid] += G0;
                 if constexpr (STORE_H) accH[sa][tid] += H0;
                 hphi += H0 * (q[25] - phii);
-                hrow += H0;
+                if constexpr (STORE_H) hrow += H0;
             }
             if (sb != 0xFFu) {
                 accG[sb][tid] += G1;
                 if constexpr (STORE_H) accH[sb][tid] += H1;
                 hphi += H1 * (q[26] - phii);
-                hrow += H1;
+                if constexpr (STORE_H) hrow += H1;
             }
             if (sc != 0xFFu) {
                 accG[sc][tid] += G2;
                 if constexpr (STORE_H) accH[sc][tid] += H2;
                 hphi += H2 * (q[27] - phii);
-                hrow += H2;
+                if constexpr (STORE_H) hrow += H2;
             }
         }
         if (valid) {
@@ -169,7 +179,7 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
             for (int s = 0; s < PATCH; ++s) {
                 const int32_t c = cols[s];
                 if (c < 0) continue;
-                const size_t off = (size_t)c * (size_t)N + (size_t)row;
+                const size_t off = (size_t)pr.gofs + (size_t)(c - pr.v0) * ld + (size_t)rloc;
                 const double gv = accG[s][tid];
                 if (G) G[off] = gv;
                 if constexpr (WRITE32) G32[off] = (float)gv;
@@ -178,8 +188,8 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
         }
     }
     if (valid) {
-        part_hphi[(size_t)blockIdx.y * N + row] = hphi;
-        part_hrow[(size_t)blockIdx.y * N + row] = hrow;
+        part_hphi[(size_t)it.g * Ntot + row] = hphi;
+        if constexpr (STORE_H) part_hrow[(size_t)it.g * Ntot + row] = hrow;
     }
 }
 
@@ -203,7 +213,8 @@ __device__ inline void self_panel(double abx, double aby, double abz, double bcx
 }
 
 template <bool STORE_H, bool WRITE32>
-__global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* __restrict__ xyz,
+__global__ __launch_bounds__(256) void k_self(int N, const int32_t* __restrict__ vprob,
+                                              const AsmProb* __restrict__ probs, const double* __restrict__ xyz,
                                               const double* __restrict__ phi, const uint32_t* __restrict__ tri,
                                               const int32_t* __restrict__ vptr, const int32_t* __restrict__ vtri,
                                               double* __restrict__ G, float* __restrict__ G32,
@@ -211,6 +222,10 @@ __global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* 
                                               const double* __restrict__ part_hrow, double* __restrict__ hphi) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= N) return;
+    const AsmProb pr = probs[vprob ? vprob[v] : 0];
+    const size_t ld = (size_t)pr.n;
+    const size_t rloc = (size_t)(v - pr.v0);
+    const int ngroups = pr.groups;
     const double ax = xyz[3 * v], ay = xyz[3 * v + 1], az = xyz[3 * v + 2];
     for (int e = vptr[v]; e < vptr[v + 1]; ++e) {
         const uint32_t* t = tri + 3 * (size_t)vtri[e];
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* 
         const double gv[3] = {g0, g1, g2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const size_t off = (size_t)cols[k] * (size_t)N + (size_t)v;
+            const size_t off = (size_t)pr.gofs + (size_t)(cols[k] - (uint32_t)pr.v0) * ld + rloc;
             double val = gv[k];
             if (G) { val += G[off]; G[off] = val; }
             else val += (double)G32[off];
@@ -239,12 +254,12 @@ __global__ __launch_bounds__(256) void k_self(int N, int ngroups, const double* 
     double hp = 0.0, hr = 0.0;
     for (int g = 0; g < ngroups; ++g) {
         hp += part_hphi[(size_t)g * N + v];
-        hr += part_hrow[(size_t)g * N + v];
+        if constexpr (STORE_H) hr += part_hrow[(size_t)g * N + v];
     }
     const double four_pi = 4.0 * PI;
     if (hphi) hphi[v] = hp - four_pi * (phi ? phi[v] : 0.0);
     if constexpr (STORE_H) {
-        const size_t d = (size_t)v * (size_t)N + (size_t)v;
+        const size_t d = (size_t)pr.gofs + rloc * ld + rloc;
         H[d] = H[d] - four_pi - hr;  // ColocSim.cpp:96 with val_H = -sum_j H(i,j)
     }
 }
@@ -261,21 +276,19 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
                                                       rec);
     BH_HIP(hipGetLastError());
 
-    const int rowblocks = (N + ASM_ROWS - 1) / ASM_ROWS;
-    int groups = (4096 + rowblocks - 1) / rowblocks;
-    groups = std::max(1, std::min(groups, topo.npatch));
-    const int ppg = (topo.npatch + groups - 1) / groups;
-    groups = (topo.npatch + ppg - 1) / ppg;
+    const int groups = topo.max_groups;
     double* ph = static_cast<double*>(ctx->part_hphi.ensure(sizeof(double) * (size_t)groups * N));
     double* hr = static_cast<double*>(ctx->part_hrow.ensure(sizeof(double) * (size_t)groups * N));
     const bool mirror = flags & BEMHIP_MIRROR_X0;
     const bool storeH = out.H != nullptr;
     const bool w32 = out.G32 != nullptr;
-    dim3 grid(rowblocks, groups);
-#define BH_FAR(MI, SH, W3)                                                                              \
-    k_far<MI, SH, W3><<<grid, ASM_ROWS, 0, st>>>(                                                      \
-        N, topo.npatch, ppg, d_xyz, d_phi, rec, topo.patch_ptr.as<int32_t>(), topo.patch_col.as<int32_t>(), \
-        out.G64, out.G32, out.H, ph, hr)
+    const AsmItem* items = topo.d_items.as<AsmItem>();
+    const AsmProb* probs = topo.d_probs.as<AsmProb>();
+    dim3 grid(topo.nitems);
+#define BH_FAR(MI, SH, W3)                                                                                 \
+    k_far<MI, SH, W3><<<grid, ASM_ROWS, 0, st>>>(N, items, probs, d_xyz, d_phi, rec,                      \
+                                                 topo.patch_ptr.as<int32_t>(), topo.patch_col.as<int32_t>(), \
+                                                 out.G64, out.G32, out.H, ph, hr)
     if (mirror) {
         if (storeH) { if (w32) BH_FAR(true, true, true); else BH_FAR(true, true, false); }
         else { if (w32) BH_FAR(true, false, true); else BH_FAR(true, false, false); }
@@ -286,7 +299,8 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
 #undef BH_FAR
     BH_HIP(hipGetLastError());
 #define BH_SELF(SH, W3)                                                                             \
-    k_self<SH, W3><<<(N + 255) / 256, 256, 0, st>>>(N, groups, d_xyz, d_phi, topo.tri.as<uint32_t>(), \
+    k_self<SH, W3><<<(N + 255) / 256, 256, 0, st>>>(N, topo.independent ? topo.vpart.as<int32_t>() : nullptr, \
+                                                    probs, d_xyz, d_phi, topo.tri.as<uint32_t>(),                  \
                                                     topo.vptr.as<int32_t>(), topo.vtri.as<int32_t>(), \
                                                     out.G64, out.G32, out.H, ph, hr, out.hphi)
     if (storeH) { if (w32) BH_SELF(true, true); else BH_SELF(true, false); }

@@ -179,29 +179,34 @@ __global__ __launch_bounds__(256) void k_laswp(int ncols, T* __restrict__ A, int
     }
 }
 
-// Speculative leaf without row interchanges: every work-group factors the
-// leaf's top n x n block in LDS (redundantly) and eliminates its own rows
-// against it, with no inter-work-group synchronisation.  max |l| over all
-// multipliers goes to *maxl (bits of a non-negative double; NaN sorts above
-// every number).  If max |l| <= 1, partial pivoting with first-maximum ties
-// (Eigen::PartialPivLU, LAPACK) would have chosen the diagonal at every column,
-// i.e. this is exactly its elimination.
-// The factored top block goes to `topout` (n x n, column-major), never into
-// A: other work-groups may still be reading A's original top rows, so
-// k_copy_top writes it into place after this kernel.
+// Speculative leaf without row interchanges (up to NPL = 32 columns): every
+// work-group factors the leaf's top n x n block in LDS (redundantly) and
+// eliminates its own NPR rows against it, with no inter-work-group
+// synchronisation in the arithmetic.  max |l| over all multipliers goes to
+// *maxl (bits of a non-negative double; NaN sorts above every number).  If
+// max |l| <= 1, partial pivoting with first-maximum ties (Eigen::PartialPivLU,
+// LAPACK) would have chosen the diagonal at every column, i.e. this is exactly
+// its elimination.  Work-group 0 writes the factored top rows back only after
+// every work-group has taken its copy of the original ones (monotone counter).
+constexpr int NPL = 32;    // speculative leaf width (64 measured slower in FP32 and FP64)
+constexpr int NPR = 256;   // rows per work-group of the speculative leaf
+
 template <class T>
-__global__ __launch_bounds__(256) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
-                                                 unsigned long long* __restrict__ maxl, T* __restrict__ topout) {
-    __shared__ T top[LEAF][LEAF + 1];
-    __shared__ T as[LEAF][256];
-    __shared__ unsigned long long red[4];
+__global__ __launch_bounds__(NPR) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
+                                                 unsigned long long* __restrict__ maxl,
+                                                 unsigned int* __restrict__ readers, unsigned int reader_target,
+                                                 unsigned int* __restrict__ err) {
+    __shared__ T top[NPL][NPL + 1];
+    __shared__ T as[NPL][NPR];
+    __shared__ unsigned long long red[NPR / 64];
     const int tid = threadIdx.x;
-    const int r = blockIdx.x * 256 + tid;
-    for (int e = tid; e < n * n; e += 256) {
+    const int r = blockIdx.x * NPR + tid;
+    for (int e = tid; e < n * n; e += NPR) {
         const int i = e % n, j = e / n;
         top[i][j] = A[(size_t)j * lda + i];
     }
     __syncthreads();
+    if (tid == 0) __hip_atomic_fetch_add(readers, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     double lmax = 0.0;
     for (int j = 0; j < n; ++j) {
         if (tid > j && tid < n) {
@@ -225,8 +230,6 @@ __global__ __launch_bounds__(256) void k_leaf_np(int m, int n, T* __restrict__ A
         }
         for (int j = 0; j < n; ++j) A[(size_t)j * lda + r] = as[j][tid];
     }
-    if (blockIdx.x == 0 && tid < n)
-        for (int j = 0; j < n; ++j) topout[(size_t)j * n + tid] = top[tid][j];
     unsigned long long bits = (unsigned long long)__double_as_longlong(lmax);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -237,16 +240,20 @@ __global__ __launch_bounds__(256) void k_leaf_np(int m, int n, T* __restrict__ A
     __syncthreads();
     if (tid == 0) {
         unsigned long long b = red[0];
-        for (int w = 1; w < 4; ++w) b = red[w] > b ? red[w] : b;
+        for (int w = 1; w < NPR / 64; ++w) b = red[w] > b ? red[w] : b;
         atomicMax(maxl, b);
     }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_copy_top(int n, const T* __restrict__ top, T* __restrict__ A, int lda) {
-    for (int e = threadIdx.x; e < n * n; e += 256) {
-        const int i = e % n, j = e / n;
-        A[(size_t)j * lda + i] = top[(size_t)j * n + i];
+    if (blockIdx.x == 0) {
+        if (tid == 0) {
+            unsigned int spins = 0;
+            while (__hip_atomic_load(readers, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < reader_target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 26)) { __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
+            }
+        }
+        __syncthreads();
+        if (tid < n)
+            for (int j = 0; j < n; ++j) A[(size_t)j * lda + tid] = top[tid][j];
     }
 }
 
@@ -282,6 +289,40 @@ static void gemm_minus(rocblas_handle h, int m, int n, int k, const float* A, in
     const float alpha = -1.0f, beta = 1.0f;
     BH_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A, lda, B,
                           ldb, &beta, C, ldc));
+}
+
+// B := B U^-1 for the upper non-unit triangle U (the rows of L below a block)
+static void trsm_runn(rocblas_handle h, int m, int n, const double* U, int ldu, double* B, int ldb) {
+    const double one = 1.0;
+    BH_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, m, n, &one, U, ldu, B, ldb));
+}
+static void trsm_runn(rocblas_handle h, int m, int n, const float* U, int ldu, float* B, int ldb) {
+    const float one = 1.0f;
+    BH_BLAS(rocblas_strsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, m, n, &one, U, ldu, B, ldb));
+}
+
+// max |x| over a column-major block, merged into *out as the bits of a
+// non-negative double (NaN orders above every number, so it fails a <= test).
+// grid.x: 256-row tiles, grid.y: 64-column chunks.
+template <class T>
+__global__ __launch_bounds__(256) void k_absmax_bits(int rows, int cols, const T* __restrict__ X, int ld,
+                                                     unsigned long long* __restrict__ out) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * 64, c1 = min(cols, c0 + 64);
+    unsigned long long bits = 0;
+    if (r < rows)
+        for (int c = c0; c < c1; ++c) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs((double)X[(size_t)c * ld + r]));
+            bits = b > bits ? b : bits;
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, bits);
 }
 
 // ---------------------------------------------------- split-fp16 update --
@@ -350,12 +391,12 @@ __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* _
 }
 
 // fp16 splits of one outer step's L21 (m x k) and U12 (k x n) blocks
-static void split_prepare(bemhip_ctx* ctx, int m, int n, int k, const float* A, int lda, const float* B, int ldb) {
+static void split_prepare(bemhip_ctx* ctx, hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B,
+                          int ldb) {
     __half* acat = static_cast<__half*>(ctx->split_a.ensure(sizeof(__half) * 3 * (size_t)m * k));
     __half* bcat = static_cast<__half*>(ctx->split_b.ensure(sizeof(__half) * 3 * (size_t)k * n));
     unsigned int* mx = static_cast<unsigned int*>(ctx->split_mx.ensure(64));
     float* alpha = reinterpret_cast<float*>(mx + 4);
-    hipStream_t st = ctx->stream;
     k_fill_u32<<<1, 64, 0, st>>>(mx, 2, 0u);
     k_block_absmax<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx);
     k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + 1);
@@ -366,16 +407,16 @@ static void split_prepare(bemhip_ctx* ctx, int m, int n, int k, const float* A, 
 }
 
 // C[:, col0:col0+n] -= L21 U12[:, col0:col0+n] from the prepared splits
-static void split_gemm(bemhip_ctx* ctx, int m, int n, int k, int col0, float* C, int ldc) {
+static void split_gemm(bemhip_ctx* ctx, rocblas_handle h, int m, int n, int k, int col0, float* C, int ldc) {
     const __half* acat = ctx->split_a.as<__half>();
     const __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
     float* alpha = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4);
-    BH_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_device));
-    rocblas_status s = rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, alpha,
+    BH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
+    rocblas_status s = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, alpha,
                                        acat, rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, alpha + 1,
                                        C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
                                        rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
-    BH_BLAS(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host));
+    BH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
     BH_BLAS(s);
 }
 
@@ -383,8 +424,8 @@ static void split_gemm(bemhip_ctx* ctx, int m, int n, int k, int col0, float* C,
 // the full B block row were prepared by split_prepare and `col0` selects the
 // first column of B used here.
 template <class T>
-static void trailing_update(bemhip_ctx* ctx, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C,
-                            int ldc, bool split16, int col0 = 0) {
+static void trailing_update(bemhip_ctx* ctx, hipStream_t st, rocblas_handle h, int m, int n, int k, const T* A, int lda,
+                            const T* B, int ldb, T* C, int ldc, bool split16, int col0 = 0) {
     if (m <= 0 || n <= 0 || k <= 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -400,15 +441,15 @@ static void trailing_update(bemhip_ctx* ctx, int m, int n, int k, const T* A, in
         ctx->gemm_fl.resize(i + 1);
         ctx->gemm_fl[i] = 2.0 * (double)m * (double)n * (double)k;
         ctx->gemm_ev_used = i + 1;
-        BH_HIP(hipEventRecord(e0, ctx->stream));
+        BH_HIP(hipEventRecord(e0, st));
     }
     if constexpr (std::is_same<T, float>::value) {
-        if (split16) split_gemm(ctx, m, n, k, col0, C, ldc);
-        else gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
+        if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc);
+        else gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
-        gemm_minus(ctx->blas, m, n, k, A, lda, B, ldb, C, ldc);
+        gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
     }
-    if (ctx->timing) BH_HIP(hipEventRecord(e1, ctx->stream));
+    if (ctx->timing) BH_HIP(hipEventRecord(e1, st));
 }
 
 // sums the recorded trailing-update timings (after the stream has drained)
@@ -470,15 +511,17 @@ static void panel_rec(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int*
 
 template <class T>
 static void panel_rec_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
-    if (n <= LEAF) {
-        T* top = static_cast<T*>(ctx->lu.leaftop.ensure(sizeof(T) * LEAF * LEAF));
-        k_leaf_np<T><<<(m + 255) / 256, 256, 0, L.s>>>(m, n, A, lda, maxl, top);
-        k_copy_top<T><<<1, 256, 0, L.s>>>(n, top, A, lda);
+    if (n <= NPL) {
+        unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
+        unsigned int* err = readers + 1;
+        const unsigned int P = (unsigned int)((m + NPR - 1) / NPR);
+        ctx->leaf_readers += P;
+        k_leaf_np<T><<<P, NPR, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
         BH_HIP(hipGetLastError());
         return;
     }
-    int n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
-    if (n1 >= n) n1 = n - LEAF;
+    int n1 = ((n / 2 + NPL - 1) / NPL) * NPL;
+    if (n1 >= n) n1 = n - NPL;
     const int n2 = n - n1;
     panel_rec_np(ctx, L, m, n1, A, lda, maxl);
     T* A12 = A + (size_t)n1 * lda;
@@ -486,6 +529,21 @@ static void panel_rec_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, u
     if (m > n1) {
         gemm_minus(L.h, m - n1, n2, n1, A + n1, lda, A12, lda, A12 + n1, lda);
         panel_rec_np(ctx, L, m - n1, n2, A12 + n1, lda, maxl);
+    }
+}
+
+// Interchange-free factorisation of an m x n panel: the top n x n block by
+// the recursive leaves, then every row below it at once, L21 = A21 U11^-1 (one
+// large TRSM on the matrix cores), and max |l| over both parts into *maxl.
+// Without interchanges this is the same elimination as column-by-column
+// Gaussian elimination of the whole panel.
+template <class T>
+static void panel_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+    panel_rec_np(ctx, L, std::min(m, n), n, A, lda, maxl);
+    if (m > n) {
+        trsm_runn(L.h, m - n, n, A, lda, A + n, lda);
+        k_absmax_bits<T><<<dim3((m - n + 255) / 256, (n + 63) / 64), 256, 0, L.s>>>(m - n, n, A + n, lda, maxl);
+        BH_HIP(hipGetLastError());
     }
 }
 
@@ -500,10 +558,12 @@ static void panel_start(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, in
     if (!ctx->speculate) { panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
     T* bk = static_cast<T*>(ctx->lu.backup.ensure(sizeof(T) * (size_t)m * n));
     BH_HIP(hipMemcpy2DAsync(bk, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
+    // layout: [max |l| bits (8 B)][leaf reader counter][error flag]
     unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
-    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 2, 0u);
-    panel_rec_np(ctx, L, m, n, A, lda, maxl);
-    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, sizeof(double), hipMemcpyDeviceToHost, L.s));
+    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 4, 0u);
+    ctx->leaf_readers = 0;
+    panel_np(ctx, L, m, n, A, lda, maxl);
+    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, 2 * sizeof(double), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
 }
 
@@ -515,6 +575,8 @@ static bool panel_finish(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, i
     if (!ctx->speculate) { ctx->panels_pivoted += 1; return true; }
     BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
     const double ml = ctx->host_scal[32];
+    const unsigned int* words = reinterpret_cast<const unsigned int*>(ctx->host_scal + 33);
+    BH_CHECK(words[1] == 0u, BEMHIP_ERR_HIP, "speculative panel: top-block hand-off timed out");
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
@@ -545,13 +607,11 @@ template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool s
     BH_HIP(hipGetLastError());
     unsigned int base = 0;
     const int lda = N;
-    // panel 0
-    BH_HIP(hipEventRecord(ev_ready, main.s));
-    BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+    // panel 0: nothing to overlap it with
     const int jb0 = std::min(LU_NB, N);
-    panel_start(ctx, side, N, jb0, A, lda, ipiv, 0, base);
-    bool pivoted = panel_finish(ctx, side, N, jb0, A, lda, ipiv, 0, base);
-    BH_HIP(hipEventRecord(ev_panel, side.s));
+    panel_start(ctx, main, N, jb0, A, lda, ipiv, 0, base);
+    bool pivoted = panel_finish(ctx, main, N, jb0, A, lda, ipiv, 0, base);
+    BH_HIP(hipEventRecord(ev_panel, main.s));
     for (int k = 0; k < N; k += LU_NB) {
         const int jb = std::min(LU_NB, N - k);
         T* Akk = A + (size_t)k * lda + k;
@@ -564,17 +624,24 @@ template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool s
         trsm_llnu(main.h, jb, rest, Akk, lda, A12, lda);
         const int jb2 = std::min(LU_NB, rest);
         if constexpr (std::is_same<T, float>::value)
-            if (split16) split_prepare(ctx, rest, rest, jb, Akk + jb, lda, A12, lda);
+            if (split16) split_prepare(ctx, main.s, rest, rest, jb, Akk + jb, lda, A12, lda);
         // next panel's columns first, then hand them to the side stream
-        trailing_update<T>(ctx, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16, 0);
-        BH_HIP(hipEventRecord(ev_ready, main.s));
-        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-        panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16, 0);
+        // the panel's small kernels only hide well behind a long update; a
+        // short one is cheaper to run first and then factor the panel alone
+        const bool overlap = ctx->lookahead && rest - jb2 >= ctx->la_min_rest;
+        const Lane pl = overlap ? side : main;
+        if (overlap) {
+            BH_HIP(hipEventRecord(ev_ready, main.s));
+            BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+            panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        }
         if (rest > jb2)
-            trailing_update<T>(ctx, rest, rest - jb2, jb, Akk + jb, lda, A12 + (size_t)jb2 * lda, lda,
-                               A12 + (size_t)jb2 * lda + jb, lda, split16, jb2);
-        pivoted = panel_finish(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
-        BH_HIP(hipEventRecord(ev_panel, side.s));
+            trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, Akk + jb, lda, A12 + (size_t)jb2 * lda,
+                               lda, A12 + (size_t)jb2 * lda + jb, lda, split16, jb2);
+        if (!overlap) panel_start(ctx, main, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        pivoted = panel_finish(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        BH_HIP(hipEventRecord(ev_panel, pl.s));
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     unsigned int flags[3] = {0, 0, 0};
@@ -707,6 +774,14 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     const int ib = LOWER ? t : nb - 1 - t;
     const int i0 = ib * TS, bs = min(TS, N - i0);
     const int r = tid & (TS - 1), h = tid >> 7;
+    // the diagonal-block inverse does not depend on x: load this thread's half
+    // row of it now so the final product is off the dependency chain
+    double ivr[TS / 2];
+    {
+        const double* iv = inv + (size_t)ib * TS * TS + (size_t)(h * (TS / 2)) * TS + r;
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) ivr[c] = iv[(size_t)c * TS];
+    }
     double acc = 0.0;
     unsigned int spins = 0;
     for (int q = 0; q < t;) {
@@ -769,16 +844,19 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     }
     part[h][r] = acc;
     __syncthreads();
-    if (tid < bs) xs[0][tid] = x[i0 + tid] - (part[0][tid] + part[1][tid]);
+    if (tid < TS) xs[0][tid] = tid < bs ? x[i0 + tid] - (part[0][tid] + part[1][tid]) : 0.0;
     __syncthreads();
-    if (tid < bs) {
-        const double* iv = inv + (size_t)ib * TS * TS + tid;
-        double v0 = 0.0, v1 = 0.0;
-        int c = 0;
-        for (; c + 2 <= bs; c += 2) { v0 += iv[(size_t)c * TS] * xs[0][c]; v1 += iv[(size_t)(c + 1) * TS] * xs[0][c + 1]; }
-        if (c < bs) v0 += iv[(size_t)c * TS] * xs[0][c];
-        __hip_atomic_store(&x[i0 + tid], v0 + v1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    {
+        // x_i = inv_i s: thread (r, h) sums its half of row r (entries beyond bs are
+        // zero in s, and the inverse is the identity there)
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) v += ivr[c] * xs[0][h * (TS / 2) + c];
+        part[h][r] = v;
     }
+    __syncthreads();
+    if (tid < bs)
+        __hip_atomic_store(&x[i0 + tid], part[0][tid] + part[1][tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (tid == 0) {
         __atomic_thread_fence(__ATOMIC_RELEASE);
@@ -1031,7 +1109,7 @@ static float elapsed(bemhip_ctx* ctx, hipEvent_t a, hipEvent_t b) {
 }
 
 void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double* b, double* x, int solver,
-                  double tol, int maxit, bemhip_solve_stats* st) {
+                  double tol, int maxit, bemhip_solve_stats* st, bool report_residual) {
     hipStream_t sm = ctx->stream;
     int* ipiv = static_cast<int*>(ctx->lu.ipiv.ensure(sizeof(int) * N));
     ctx->gemm_ms = 0.0; ctx->gemm_flops = 0.0; ctx->gemm_launches = 0;
@@ -1058,7 +1136,9 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
         int* perm = make_perm(ctx, N, ipiv);
         if (ctx->trsv_persistent) lu_prepare_solve<float>(ctx, N, G32);
         BH_HIP(hipEventRecord(e1, sm));
-        if (tol <= 0.0) tol = 1e-12;
+        // stop once the correction is below tol |x|: the error left is about
+        // tol times the contraction factor of a sweep (1e-4 .. 1e-3 here)
+        if (tol <= 0.0) tol = 1e-10;
         if (maxit <= 0) maxit = 30;
         BH_HIP(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, sm));
         lu_solve<float>(ctx, N, G32, perm, x);
@@ -1093,12 +1173,88 @@ void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double*
     st->panels_pivoted = ctx->panels_pivoted;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     // final FP64 residual |b - G x| / |b| (not for LU64, whose G was overwritten)
-    if (solver != BEMHIP_SOLVE_LU64) {
+    if (report_residual && solver != BEMHIP_SOLVE_LU64) {
         double* r = static_cast<double*>(ctx->vec[5].ensure(sizeof(double) * N));
         gemv64(ctx, N, G64, x, b, r);
         const double rn = dev_dot(ctx, N, r, r), bn = dev_dot(ctx, N, b, b);
         st->residual = bn > 0.0 ? std::sqrt(rn / bn) : std::sqrt(rn);
     }
+}
+
+// ------------------------------------------------- transposed solves --
+// G = P^T L U, so G^T x = b is U^T L^T (P x) = b: x = P^T (L^-T U^-T b).
+__global__ void k_d2f_vec(int n, const double* __restrict__ a, float* __restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+template <class T>
+__global__ void k_scatter_perm(int n, const int* __restrict__ perm, const T* __restrict__ src,
+                               double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[perm[i]] = (double)src[i];
+}
+
+// x = G^-T b from FP64 factors
+static void lu_solve_t64(bemhip_ctx* ctx, int N, const double* LU, const int* perm, const double* b, double* x) {
+    hipStream_t st = ctx->stream;
+    double* t = static_cast<double*>(ctx->vec[8].ensure(sizeof(double) * N));
+    BH_HIP(hipMemcpyAsync(t, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    BH_BLAS(rocblas_dtrsv(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, rocblas_diagonal_non_unit, N,
+                          LU, N, t, 1));
+    BH_BLAS(rocblas_dtrsv(ctx->blas, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_unit, N, LU,
+                          N, t, 1));
+    k_scatter_perm<double><<<(N + 255) / 256, 256, 0, st>>>(N, perm, t, x);
+    BH_HIP(hipGetLastError());
+}
+
+// x = G^-T b from FP32 factors (b rounded to FP32; refinement recovers it)
+static void lu_solve_t32(bemhip_ctx* ctx, int N, const float* LU, const int* perm, const double* b, double* x) {
+    hipStream_t st = ctx->stream;
+    float* t = static_cast<float*>(ctx->vec[9].ensure(sizeof(float) * N));
+    k_d2f_vec<<<(N + 255) / 256, 256, 0, st>>>(N, b, t);
+    BH_BLAS(rocblas_strsv(ctx->blas, rocblas_fill_upper, rocblas_operation_transpose, rocblas_diagonal_non_unit, N,
+                          LU, N, t, 1));
+    BH_BLAS(rocblas_strsv(ctx->blas, rocblas_fill_lower, rocblas_operation_transpose, rocblas_diagonal_unit, N, LU,
+                          N, t, 1));
+    k_scatter_perm<float><<<(N + 255) / 256, 256, 0, st>>>(N, perm, t, x);
+    BH_HIP(hipGetLastError());
+}
+
+void solve_transposed_device(bemhip_ctx* ctx, int N, const double* G64, const float* G32, int solver,
+                             const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st) {
+    const int* perm = ctx->perm.as<int>();
+    BH_CHECK(perm != nullptr, BEMHIP_ERR_ARG, "transposed solve needs the factors of a previous LU solve");
+    st->iterations = 0; st->converged = 1; st->correction = 0.0; st->residual = 0.0;
+    if (solver == BEMHIP_SOLVE_LU64) {
+        lu_solve_t64(ctx, N, G64, perm, b, x);
+        return;
+    }
+    BH_CHECK(solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64, BEMHIP_ERR_ARG,
+             "transposed solve needs an LU-based solver");
+    if (tol <= 0.0) tol = 1e-10;
+    if (maxit <= 0) maxit = 30;
+    lu_solve_t32(ctx, N, G32, perm, b, x);
+    double* r = static_cast<double*>(ctx->vec[6].ensure(sizeof(double) * N));
+    double* d = static_cast<double*>(ctx->vec[5].ensure(sizeof(double) * N));
+    const double one = 1.0, mone = -1.0;
+    int it = 0;
+    bool conv = false;
+    double cn = 0.0;
+    while (it < maxit) {
+        // r = b - G^T x in FP64
+        BH_HIP(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+        BH_BLAS(rocblas_dgemv(ctx->blas, rocblas_operation_transpose, N, N, &mone, G64, N, x, 1, &one, r, 1));
+        lu_solve_t32(ctx, N, G32, perm, r, d);
+        axpby(ctx, N, 1.0, d, 1.0, x);
+        ++it;
+        const double dn = dev_absmax(ctx, N, d), xn = dev_absmax(ctx, N, x);
+        cn = xn > 0.0 ? dn / xn : dn;
+        if (!std::isfinite(cn)) break;
+        if (cn <= tol) { conv = true; break; }
+    }
+    st->iterations = it;
+    st->converged = conv ? 1 : 0;
+    st->correction = cn;
 }
 
 }  // namespace bemhip

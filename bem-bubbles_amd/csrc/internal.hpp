@@ -71,9 +71,29 @@ struct DevBuf {
 constexpr int PATCH = 32;          // columns per assembly patch
 constexpr int ASM_ROWS = 128;      // collocation rows per assembly work-group
 
+// One dense BEM problem inside a mesh: all vertices of the mesh (coupled
+// bubbles, the reference's semantics) or one loose part (independent bubbles).
+// Its G block is n x n, column-major with leading dimension n, at element
+// offset gofs of the G buffer; its vertices are [v0, v0 + n).
+struct AsmProb {
+    int32_t v0, n;
+    int64_t gofs;
+    int32_t groups;   // patch groups = H phi partial slices of its rows
+    int32_t p0, p1;   // its patches
+    int32_t pad;
+};
+// One assembly work-group: 128 rows of a problem x a range of its patches
+struct AsmItem {
+    int32_t prob, row0, p0, p1, g, pad0, pad1, pad2;
+};
+
 struct Topology {
     int64_t N = 0, M = 0;
     int npatch = 0;
+    bool independent = false;
+    std::vector<AsmProb> probs;
+    std::vector<AsmItem> items;
+    int max_groups = 0;
     std::vector<uint32_t> tri;             // M x 3
     // assembly patches: vertex -> (patch, slot); per patch, the incident triangles
     std::vector<int32_t> patch_col;        // npatch * PATCH vertex ids (-1 = empty slot)
@@ -85,23 +105,32 @@ struct Topology {
     // loose parts (Mesh.cpp:382-440): per vertex part id; per part triangle list
     int nparts = 0;
     std::vector<int32_t> vpart;
-    std::vector<int32_t> part_tptr, part_tri;
+    std::vector<int32_t> part_tptr, part_tri;   // triangles of each loose part (ascending)
+    std::vector<int32_t> part_vptr, part_vtx;   // vertices of each loose part (ascending)
+    std::vector<int32_t> part_pptr;        // per part, its patch range
     double redundancy = 0.0;               // patch entries / M
 };
 
-Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri);
+// independent = every loose part is its own problem (parts must then be
+// contiguous vertex ranges in increasing order)
+Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri, bool independent = false);
 
 // Device copies of the topology tables
 struct DevTopology {
     int64_t N = 0, M = 0;
-    int npatch = 0, nparts = 0, nentries = 0;
-    DevBuf tri, patch_col, patch_ptr, patch_tri, patch_slots, vptr, vtri, vpart, part_tptr, part_tri;
+    int npatch = 0, nparts = 0, nentries = 0, nitems = 0, max_groups = 0;
+    bool independent = false;
+    std::vector<AsmProb> probs;        // host copy (solve loop)
+    int64_t gsize = 0;                 // elements of all G blocks
+    DevBuf tri, patch_col, patch_ptr, patch_tri, patch_slots, vptr, vtri, vpart, part_tptr, part_tri, part_vptr,
+        part_vtx;
+    DevBuf d_probs, d_items;
     void upload(const Topology& t, hipStream_t s);
 };
 
 // ------------------------------------------------------------- context --
 struct LuWork {
-    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, leaftop;
+    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl;
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
 };
 
@@ -113,10 +142,15 @@ struct bemhip_ctx {
     rocblas_handle blas = nullptr;
     hipStream_t stream2 = nullptr;      // LU look-ahead: panels of the next block column
     rocblas_handle blas2 = nullptr;
+    // look-ahead only while the trailing update of a step is long enough to
+    // hide the next panel: with fewer than this many columns left after it,
+    // the panel runs after the update on the main stream, uncontended
+    int la_min_rest = 16384;
     hipEvent_t ev_la[3] = {};
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
-    int nb64 = 256, nb32 = 256, nb_split = 512;   // LU outer block per precision
+    int nb64 = 512, nb32 = 512, nb_split = 1024;   // LU outer block per precision
+    unsigned int leaf_readers = 0;                  // running target of the speculative-leaf hand-off
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
     unsigned int trsv_epoch = 0;
@@ -126,6 +160,7 @@ struct bemhip_ctx {
     bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H, vec[12], scal, gemv_part;
     bemhip::LuWork lu;
     bemhip::DevBuf split_a, split_b, split_mx, perm;
+    bemhip::DevBuf adj_coef, adj_tc, adj_tpart, adj_rpart, adj_sc, adj_zero;   // adjoint partials
     std::vector<int> h_ipiv, h_perm;
     double* host_scal = nullptr;  // pinned
     // instrumentation of trailing updates
@@ -161,15 +196,21 @@ template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU);
 void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const double* b, double* r);
 // solver front end on a device-resident FP64 G (destroyed by LU64) and rhs b.
 void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double* b, double* x,
-                  int solver, double tol, int maxit, bemhip_solve_stats* st);
+                  int solver, double tol, int maxit, bemhip_solve_stats* st, bool report_residual = true);
+// G^T x = b with the factors left by the last LU-based solve_device call on
+// this context (same N and solver; G64 is the unfactored FP64 G for the mixed
+// solvers, whose transposed solve is refined against it)
+void solve_transposed_device(bemhip_ctx* ctx, int N, const double* G64, const float* G32, int solver,
+                             const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st);
 // device reductions
 double dev_absmax(bemhip_ctx* ctx, int n, const double* x);
 double dev_dot(bemhip_ctx* ctx, int n, const double* x, const double* y);
 
 // ------------------------------------------------------------- stage --
+// scalars of the Bernoulli right-hand side; V_0 and the forcing are per part
+// (an array of [V_0, amplitude, frequency, phase] per loose part)
 struct StageParams {
-    double p_inf, epsilon, sigma, gamma, V_0;
-    double field_amp, field_freq, field_phase;
+    double p_inf, epsilon, sigma, gamma;
 };
 void tri_geometry(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, const double* pot,
                   double* tnorm, double* tgrad, double* tvol);
@@ -180,9 +221,13 @@ void part_volumes(bemhip_ctx* ctx, const DevTopology& topo, const double* tvol, 
 void vertex_velocity(bemhip_ctx* ctx, const DevTopology& topo, const double* tnorm, const double* tgrad,
                      const double* psi, double* vel);
 void vertex_dpot(bemhip_ctx* ctx, const DevTopology& topo, const StageParams& sp, const double* xyz,
-                 const double* vel, const double* tkappa, const double* pvol, double t, double* dpot);
+                 const double* vel, const double* tkappa, const double* pvol, const double* pphys, double t,
+                 double* dpot);
 void axpy_state(bemhip_ctx* ctx, int64_t N, double c, const double* kx, const double* x1, double* xs,
                 const double* kp, const double* p1, double* ps);
+// x = c dx (+ x), p = c dp (+ p) over N vertices (x: N x 3)
+void axpy_pair(bemhip_ctx* ctx, int64_t N, double c, const double* dx, double* x, const double* dp, double* p,
+               bool accumulate);
 void rk4_final(bemhip_ctx* ctx, int64_t N, double dt, const double* kx, const double* kp,
                const double* x1, double* x, double* phi, double* avg, int wall_guards, double wall_eps);
 void psi_project(bemhip_ctx* ctx, int64_t N, const double* avg, const double* vn, double* psi);
@@ -190,5 +235,26 @@ void euler_update(bemhip_ctx* ctx, int64_t N, double dt, const double* vel, cons
                   double* x, double* phi);
 void dt_reduce(bemhip_ctx* ctx, int64_t N, const double* vel, const double* dpot, double* out2);
 int any_negative_x(bemhip_ctx* ctx, int64_t N, const double* xyz);
+
+// ----------------------------------------------------------- adjoint --
+// forward quantities of one stage that its reverse sweep reads
+struct AdjStageIn {
+    const double* xyz;     // stage positions
+    const double* pot;     // stage potential
+    const double* psi;     // stage dphi/dn
+    const double* vel;     // stage vertex velocities
+    const double* pvol;    // part volumes at xyz
+    const double* pphys;   // per-part [V_0, amplitude, frequency, phase]
+    double epsilon, gamma;
+};
+// kxbar += d(kp)/d(kx)^T kpbar; psibar = cotangent of psi; per-triangle
+// partials of the velocity / volume terms kept in the context
+void adj_stage_pre(bemhip_ctx* ctx, const DevTopology& topo, const AdjStageIn& in, double* kxbar, const double* kpbar,
+                   double* psibar);
+// with lam = G^-T psibar: xbar, pbar += the stage's input cotangents
+void adj_stage_post(bemhip_ctx* ctx, const DevTopology& topo, const AdjStageIn& in, const double* lam, double* xbar,
+                    double* pbar);
+// xbar += vcoef[part] * d(part volume)/dx
+void adj_volume(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, const double* vcoef, double* xbar);
 
 }  // namespace bemhip

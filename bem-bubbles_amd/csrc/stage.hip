@@ -6,17 +6,9 @@
 
 #include "internal.hpp"
 #include "quad_rules.hpp"
+#include "vec3.hpp"
 
 namespace bemhip {
-
-struct d3 { double x, y, z; };
-__device__ inline d3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
-__device__ inline d3 sub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ inline d3 add(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ inline d3 scl(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline d3 crs(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-__device__ inline void st3(double* p, d3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 
 // generate_triangle_normals (Mesh.cpp:130-138), generate_tangent_gradients
 // (LinLinSim.cpp:288-305), signed tetrahedron volume (Mesh.cpp:206-218)
@@ -172,7 +164,8 @@ __global__ __launch_bounds__(256) void k_vertex_dpot(int N, StageParams sp, doub
                                                      const double* __restrict__ vel, const int32_t* __restrict__ vptr,
                                                      const int32_t* __restrict__ vtri, const double* __restrict__ tk,
                                                      const int32_t* __restrict__ vpart,
-                                                     const double* __restrict__ pvol, double* __restrict__ dpot) {
+                                                     const double* __restrict__ pvol,
+                                                     const double* __restrict__ pphys, double* __restrict__ dpot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double kap = 0.0;
@@ -182,9 +175,11 @@ __global__ __launch_bounds__(256) void k_vertex_dpot(int N, StageParams sp, doub
         kap /= w;
     }
     const d3 v = ld3(vel + 3 * (size_t)i);
-    const double field = sp.field_amp * sin(2.0 * PI * sp.field_freq * t + sp.field_phase);
-    dpot[i] = 2.0 * sp.sigma * kap + 0.5 * dot(v, v) + sp.p_inf -
-              sp.epsilon * pow(sp.V_0 / pvol[vpart[i]], sp.gamma) + field;
+    const int c = vpart[i];
+    const double* pp = pphys + 4 * (size_t)c;   // part c: V_0, amplitude, frequency, phase
+    const double field = pp[1] * sin(2.0 * PI * pp[2] * t + pp[3]);
+    dpot[i] = 2.0 * sp.sigma * kap + 0.5 * dot(v, v) + sp.p_inf - sp.epsilon * pow(pp[0] / pvol[c], sp.gamma) +
+              field;
 }
 
 // stage state: xs = c kx + x1, ps = c kp + p1 (LinLinSim.cpp:218-232)
@@ -197,6 +192,19 @@ __global__ __launch_bounds__(256) void k_axpy_state(int N, double c, const doubl
 #pragma unroll
     for (int d = 0; d < 3; ++d) xs[3 * (size_t)i + d] = c * kx[3 * (size_t)i + d] + x1[3 * (size_t)i + d];
     ps[i] = c * kp[i] + p1[i];
+}
+
+// x (N x 3) = c dx (+ x when acc), p (N) = c dp (+ p when acc)
+__global__ __launch_bounds__(256) void k_axpy_pair(int N, double c, const double* dx, double* x, const double* dp,
+                                                   double* p, int acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const size_t e = 3 * (size_t)i + d;
+        x[e] = c * dx[e] + (acc ? x[e] : 0.0);
+    }
+    p[i] = c * dp[i] + (acc ? p[i] : 0.0);
 }
 
 // RK4 combination (LinLinSim.cpp:236-240) incl. the wall guard nopenetration
@@ -303,15 +311,21 @@ void vertex_velocity(bemhip_ctx* ctx, const DevTopology& topo, const double* tno
     BH_HIP(hipGetLastError());
 }
 void vertex_dpot(bemhip_ctx* ctx, const DevTopology& topo, const StageParams& sp, const double* xyz,
-                 const double* vel, const double* tk, const double* pvol, double t, double* dpot) {
+                 const double* vel, const double* tk, const double* pvol, const double* pphys, double t,
+                 double* dpot) {
     k_vertex_dpot<<<nb(topo.N), 256, 0, ctx->stream>>>((int)topo.N, sp, t, xyz, vel, topo.vptr.as<int32_t>(),
                                                        topo.vtri.as<int32_t>(), tk, topo.vpart.as<int32_t>(), pvol,
-                                                       dpot);
+                                                       pphys, dpot);
     BH_HIP(hipGetLastError());
 }
 void axpy_state(bemhip_ctx* ctx, int64_t N, double c, const double* kx, const double* x1, double* xs,
                 const double* kp, const double* p1, double* ps) {
     k_axpy_state<<<nb(N), 256, 0, ctx->stream>>>((int)N, c, kx, x1, xs, kp, p1, ps);
+    BH_HIP(hipGetLastError());
+}
+void axpy_pair(bemhip_ctx* ctx, int64_t N, double c, const double* dx, double* x, const double* dp, double* p,
+               bool accumulate) {
+    k_axpy_pair<<<nb(N), 256, 0, ctx->stream>>>((int)N, c, dx, x, dp, p, accumulate ? 1 : 0);
     BH_HIP(hipGetLastError());
 }
 void rk4_final(bemhip_ctx* ctx, int64_t N, double dt, const double* kx, const double* kp, const double* x1,

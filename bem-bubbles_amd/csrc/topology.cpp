@@ -18,7 +18,7 @@ static uint32_t spread10(uint32_t v) {
     return v;
 }
 
-Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri) {
+Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t* tri, bool independent) {
     BH_CHECK(N > 0 && M > 0, BEMHIP_ERR_ARG, "empty mesh");
     BH_CHECK(N < (int64_t)1 << 31 && 3 * M < (int64_t)1 << 31, BEMHIP_ERR_ARG, "mesh too large");
     Topology T;
@@ -77,6 +77,12 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
         T.part_tri.resize(M);
         std::vector<int32_t> fill(T.part_tptr.begin(), T.part_tptr.end() - 1);
         for (int64_t j = 0; j < M; ++j) T.part_tri[fill[T.vpart[tri[3 * j]]]++] = (int32_t)j;
+        T.part_vptr.assign(n + 1, 0);
+        for (int64_t i = 0; i < N; ++i) T.part_vptr[T.vpart[i] + 1]++;
+        for (int c = 0; c < n; ++c) T.part_vptr[c + 1] += T.part_vptr[c];
+        T.part_vtx.resize(N);
+        std::vector<int32_t> vfill(T.part_vptr.begin(), T.part_vptr.end() - 1);
+        for (int64_t i = 0; i < N; ++i) T.part_vtx[vfill[T.vpart[i]]++] = (int32_t)i;
     }
 
     // assembly patches: compact groups of PATCH vertices grown breadth-first
@@ -132,6 +138,23 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
             ++np;
         }
         T.npatch = np;
+        // patches never span loose parts (they grow through triangles); order
+        // them by part so that every part owns a contiguous patch range
+        {
+            std::vector<int32_t> ppart(np), order(np), rank(np);
+            for (int p = 0; p < np; ++p) ppart[p] = T.vpart[T.patch_col[(size_t)p * PATCH]];
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ppart[a] < ppart[b]; });
+            for (int p = 0; p < np; ++p) rank[order[p]] = p;
+            std::vector<int32_t> cols(T.patch_col.size());
+            for (int p = 0; p < np; ++p)
+                for (int s2 = 0; s2 < PATCH; ++s2) cols[(size_t)rank[p] * PATCH + s2] = T.patch_col[(size_t)p * PATCH + s2];
+            T.patch_col.swap(cols);
+            for (int64_t i = 0; i < N; ++i) pid[i] = rank[pid[i]];
+            T.part_pptr.assign(T.nparts + 1, 0);
+            for (int p = 0; p < np; ++p) T.part_pptr[ppart[order[p]] + 1]++;
+            for (int c = 0; c < T.nparts; ++c) T.part_pptr[c + 1] += T.part_pptr[c];
+        }
         T.patch_ptr.assign(np + 1, 0);
         for (int64_t j = 0; j < M; ++j) {
             int32_t ps[3] = {pid[tri[3 * j]], pid[tri[3 * j + 1]], pid[tri[3 * j + 2]]};
@@ -165,6 +188,71 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
         }
         T.redundancy = (double)ne / (double)M;
     }
+
+    // dense problems and the assembly work table
+    T.independent = independent;
+    if (independent) {
+        // part c must be the vertex range [v0_c, v0_c + n_c) with v0_c increasing
+        std::vector<int64_t> first(T.nparts, -1), count(T.nparts, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            const int c = T.vpart[i];
+            if (first[c] < 0) first[c] = i;
+            count[c]++;
+        }
+        int64_t next = 0;
+        for (int c = 0; c < T.nparts; ++c) {
+            BH_CHECK(first[c] == next, BEMHIP_ERR_BAD_MESH,
+                     "independent parts must be contiguous vertex ranges in increasing order");
+            next += count[c];
+        }
+        for (int64_t i = 1; i < N; ++i)
+            BH_CHECK(T.vpart[i] >= T.vpart[i - 1], BEMHIP_ERR_BAD_MESH,
+                     "independent parts must be contiguous vertex ranges in increasing order");
+        int64_t gofs = 0;
+        for (int c = 0; c < T.nparts; ++c) {
+            AsmProb pr{};
+            pr.v0 = (int32_t)first[c];
+            pr.n = (int32_t)count[c];
+            pr.gofs = gofs;
+            pr.p0 = T.part_pptr[c];
+            pr.p1 = T.part_pptr[c + 1];
+            gofs += ((int64_t)pr.n * pr.n + 63) / 64 * 64;   // 256-B aligned blocks in FP32 too
+            T.probs.push_back(pr);
+        }
+    } else {
+        AsmProb pr{};
+        pr.v0 = 0;
+        pr.n = (int32_t)N;
+        pr.gofs = 0;
+        pr.p0 = 0;
+        pr.p1 = T.npatch;
+        T.probs.push_back(pr);
+    }
+    // about 8192 work-groups in total, shared in proportion to n^2
+    double work = 0.0;
+    for (const AsmProb& pr : T.probs) work += (double)pr.n * pr.n;
+    for (size_t q = 0; q < T.probs.size(); ++q) {
+        AsmProb& pr = T.probs[q];
+        const int rb = (pr.n + ASM_ROWS - 1) / ASM_ROWS;
+        const int npp = pr.p1 - pr.p0;
+        const double want = std::max<double>(rb, 8192.0 * (double)pr.n * pr.n / work);
+        int groups = (int)std::ceil(want / rb);
+        groups = std::max(1, std::min(groups, npp));
+        const int ppg = (npp + groups - 1) / groups;
+        groups = (npp + ppg - 1) / ppg;
+        pr.groups = groups;
+        T.max_groups = std::max(T.max_groups, groups);
+        for (int g = 0; g < groups; ++g)
+            for (int b = 0; b < rb; ++b) {
+                AsmItem it{};
+                it.prob = (int32_t)q;
+                it.row0 = b * ASM_ROWS;
+                it.p0 = pr.p0 + g * ppg;
+                it.p1 = std::min(pr.p1, it.p0 + ppg);
+                it.g = g;
+                T.items.push_back(it);
+            }
+    }
     return T;
 }
 
@@ -179,6 +267,14 @@ void DevTopology::upload(const Topology& t, hipStream_t s) {
     npatch = t.npatch;
     nparts = t.nparts;
     nentries = (int)t.patch_tri.size();
+    nitems = (int)t.items.size();
+    max_groups = t.max_groups;
+    independent = t.independent;
+    probs = t.probs;
+    gsize = 0;
+    for (const AsmProb& pr : probs) gsize = std::max<int64_t>(gsize, pr.gofs + (int64_t)pr.n * pr.n);
+    up(d_probs, t.probs, s);
+    up(d_items, t.items, s);
     up(tri, t.tri, s);
     up(patch_col, t.patch_col, s);
     up(patch_ptr, t.patch_ptr, s);
@@ -189,6 +285,8 @@ void DevTopology::upload(const Topology& t, hipStream_t s) {
     up(vpart, t.vpart, s);
     up(part_tptr, t.part_tptr, s);
     up(part_tri, t.part_tri, s);
+    up(part_vptr, t.part_vptr, s);
+    up(part_vtx, t.part_vtx, s);
     BH_HIP(hipStreamSynchronize(s));
 }
 

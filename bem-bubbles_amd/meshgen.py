@@ -208,13 +208,20 @@ def config_large(seed: int = 0, nu: int = 64) -> Case:
                 parts=[(0, v.shape[0])])
 
 
-def config_adjoint(seed: int = 0) -> Case:
-    """One sphere, nu=16 (2562 vertices) in p_inf(t) = 1 + 0.3 sin(pi t)."""
-    v, t = icosphere(16)
+def config_adjoint(seed: int = 0, nu: int = 16) -> Case:
+    """One sphere, nu=16 (2562 vertices) at rest (phi = 0) in
+    p_inf(t) = 1 + 0.3 sin(pi t), 50 RK4 steps of 1e-2."""
+    v, t = icosphere(nu)
     phi = np.zeros(v.shape[0])
-    return Case("sphere_nu16_forced", v, t, phi,
-                Physics(1.0, 1.0, 0.0, 1.0, field_amp=0.3, field_freq=0.5), dt=1e-2, steps=50,
-                parts=[(0, v.shape[0])])
+    return Case(f"sphere_nu{nu}_forced", v, t, phi,
+                Physics(1.0, 1.0, 0.0, 1.0, field_amp=0.3, field_freq=0.5),
+                dt=1e-2, steps=50, parts=[(0, v.shape[0])])
+
+
+def adjoint_target_phi(case: Case, seed: int = 1, rel: float = 0.01):
+    """Initial potential of the target run: the case's phi plus a seeded
+    perturbation of 1% (unit-variance real SH, degrees 0..4, times rel)."""
+    return case.phi + rel * random_sh_field(case.verts, 0, 4, seed)
 
 
 def config_batch(seed: int = 0, count: int = 256):
@@ -231,3 +238,23 @@ def config_batch(seed: int = 0, count: int = 256):
                         Physics(1.0, 1.0, 0.0, 1.0, field_amp=A, field_freq=0.5), dt=1e-2, steps=20,
                         parts=[(0, v.shape[0])]))
     return out
+
+
+def join_batch(cases, spacing: float = 5.0):
+    """Join independent single-bubble cases into one mesh whose loose parts are
+    contiguous vertex ranges in case order, bubbles placed on a cubic grid
+    `spacing` apart (their interaction is ignored by the batched path).
+    Returns (verts, tris, phi, parts, field_amp, field_freq) with per-part
+    forcing arrays."""
+    side = max(1, int(np.ceil(len(cases) ** (1.0 / 3.0))))
+    meshes, phis, parts, off = [], [], [], 0
+    for b, c in enumerate(cases):
+        g = np.array([b % side, (b // side) % side, b // (side * side)], dtype=np.float64)
+        meshes.append((c.verts + spacing * g, c.tris))
+        phis.append(c.phi)
+        parts.append((off, c.verts.shape[0]))
+        off += c.verts.shape[0]
+    v, t = join_meshes(meshes)
+    amp = np.array([c.physics.field_amp for c in cases])
+    freq = np.array([c.physics.field_freq for c in cases])
+    return v, t, np.concatenate(phis), parts, amp, freq

@@ -66,6 +66,68 @@ def cpu_baseline(threads):
                       f"LU by (N/n)^3"}
 
 
+def batch_bench(args):
+    """256 independent spheres (nu in {4, 8, 16, 32}, R ~ U[0.5, 2], forcing
+    amplitude ~ U[0.1, 0.5] at frequency 0.5, seed 0) advanced together, one
+    dense problem per bubble, RK4 dt = 1e-2."""
+    cases = meshgen.config_batch(seed=0, count=args.batch)
+    v, t, phi, parts, amp, freq = meshgen.join_batch(cases)
+    sizes = np.array([n for _, n in parts], dtype=np.float64)
+    N, M = len(v), len(t)
+    log(f"[bench] batch of {len(cases)} bubbles: N={N} M={M} solver={args.solver}")
+    ctx = bemhip.Context(0)
+    sim = bemhip.ColocSim(ctx, v, t, 1.0, 1.0, 0.0, 1.0, solver=args.solver, independent_parts=True)
+    sim.set_phi(phi)
+    sim.set_part_physics(field_amp=amp, field_freq=freq)
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        sim.evolve_system_RK4(1e-2, True)
+        log(f"[bench] warmup step {w} {time.perf_counter() - t0:.3f}s")
+    sim.accum_stats(reset=True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.evolve_system_RK4(1e-2, True)
+        log(f"[bench] step {k} done {time.perf_counter() - t0:.3f}s")
+    wall = time.perf_counter() - t0
+    acc = sim.accum_stats(reset=True)
+    stages = 4 * args.steps
+    ms_stage = wall * 1e3 / stages
+    log(f"[bench] batch {ms_stage:.2f} ms/stage {acc}")
+    out = {
+        "metric": f"ms/RK4 stage (assembly+solve), batch of {len(cases)} independent bubbles",
+        "value": round(ms_stage, 3), "unit": "ms/stage", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-lu(split-f16 mfma)+f64-ir" if args.solver == "lu16x3_ir64" else
+                 ("f32-lu+f64-ir" if args.solver == "lu32_ir64" else "f64"),
+        "data": "synthetic",
+        "config": {"workload": f"{len(cases)} independent spheres, nu in {{4,8,16,32}}, R~U[0.5,2], "
+                               f"forcing amplitude~U[0.1,0.5] at 0.5, seed 0, RK4 dt=1e-2",
+                   "N_total": N, "M_total": M, "solver": args.solver, "stages_timed": stages,
+                   "sum_n2": float((sizes ** 2).sum()), "lu_flops_per_stage": float((2.0 / 3.0 * sizes ** 3).sum())},
+        "unknowns_per_s": round(N * stages / wall, 1),
+        "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
+        "refinement_sweeps_max": acc["iterations"],
+    }
+    if not args.no_fp64:
+        sim.compute_psi()
+        psi_mixed = sim.get_psi()
+        sim.set_solver("lu64")
+        sim.accum_stats(reset=True)
+        t0 = time.perf_counter()
+        sim.compute_psi()
+        fp64_ms = (time.perf_counter() - t0) * 1e3
+        psi64 = sim.get_psi()
+        errs = [float(np.max(np.abs(psi_mixed[o:o + n] - psi64[o:o + n])) / np.max(np.abs(psi64[o:o + n])))
+                for o, n in parts]
+        out["psi_rel_err_vs_fp64_max_over_bubbles"] = max(errs)
+        out["fp64_compute_psi_ms"] = round(fp64_ms, 3)
+        log(f"[bench] fp64 compute_psi {fp64_ms:.1f} ms, max err {max(errs):.3e}")
+    sim.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,11 +137,28 @@ def main():
     ap.add_argument("--solver", default="lu16x3_ir64", choices=sorted(bemhip.SOLVERS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead of the single-sphere workload, time a batch of this many independent bubbles")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
               f"design-only in this build", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        batch_bench(args)
+        return
 
     case = meshgen.config_large(seed=0, nu=args.nu)
     N, M = len(case.verts), len(case.tris)

@@ -39,6 +39,13 @@ typedef enum {
 
 /* assembly flags */
 #define BEMHIP_MIRROR_X0 0x1u   /* image across the wall x = 0 (MIRROR_MESH, LinLinSim.hpp:13) */
+/* Batch of independent bubbles: every loose part of the mesh is its own BEM
+ * problem (its own dense G, solve, V_0 and forcing), as if each part were a
+ * separate reference Simulation advanced with a shared dt.  Parts must be
+ * contiguous vertex ranges in increasing order.  Accepted by
+ * bemhip_compute_psi and in bemhip_sim_params.flags; not by bemhip_assemble
+ * (whose output is one N x N matrix). */
+#define BEMHIP_INDEPENDENT_PARTS 0x2u
 
 /* dense solvers for G psi = H phi */
 typedef enum {
@@ -79,7 +86,7 @@ typedef struct {
     double tol;                 /* solver tolerance (<= 0: method default) */
     int32_t maxit;              /* solver iteration cap (<= 0: method default) */
     int32_t solver;             /* bemhip_solver; the reference default is BiCGSTAB */
-    uint32_t flags;             /* BEMHIP_MIRROR_X0 */
+    uint32_t flags;             /* BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS */
     int32_t wall_guards;        /* test_negative + nopenetration (reference RK4 in wall mode) */
     int32_t frozen_stage_time;  /* 1: every stage uses the step-start time (LinLinSim.cpp:208-238) */
     int32_t pad_;
@@ -116,7 +123,7 @@ int bemhip_compute_psi(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M,
 void bemhip_sim_default_params(bemhip_sim_params* p);
 int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, const double* xyz,
                       int64_t M, const uint32_t* tri, bemhip_sim** out);
-int bemhip_sim_destroy(bemhip_sim* sim);
+int bemhip_sim_destroy(bemhip_sim* sim);   /* before bemhip_destroy of its context */
 int bemhip_sim_set_phi(bemhip_sim* sim, const double* phi);            /* Simulation.hpp:92-95   */
 int bemhip_sim_set_psi(bemhip_sim* sim, const double* psi);            /* Simulation.hpp:97-100  */
 int bemhip_sim_set_V_0(bemhip_sim* sim, double V_0);                   /* Simulation.hpp:106-108 */
@@ -131,6 +138,21 @@ int bemhip_sim_set_params(bemhip_sim* sim, const bemhip_sim_params* p);
 int bemhip_sim_set_mesh(bemhip_sim* sim, int64_t N, const double* xyz, int64_t M, const uint32_t* tri);
 int bemhip_sim_get_state(bemhip_sim* sim, double* xyz, double* phi, double* psi, double* time);
 int bemhip_sim_get_volume(bemhip_sim* sim, double* vol);               /* Simulation.hpp:128-130 */
+/* number of loose parts (split_by_loose_parts, Mesh.cpp:382-440) and their
+ * signed volumes, part p = the part holding the smallest vertex id not in
+ * parts 0..p-1 */
+int bemhip_sim_get_parts(bemhip_sim* sim, int32_t* nparts);
+int bemhip_sim_get_part_volumes(bemhip_sim* sim, double* vol, int32_t nparts);
+/* Per-part Bernoulli parameters for pot_t_multi (LinLinSim.cpp:98-120): part p
+ * uses V_0[p] and p_field = field_amp[p] sin(2 pi field_freq[p] t +
+ * field_phase[p]).  A null array leaves that parameter as it is.  No
+ * counterpart in the reference, whose V_0 and forcing are one scalar per
+ * Simulation; it serves batches of independent bubbles
+ * (BEMHIP_INDEPENDENT_PARTS), where each part's V_0 defaults to its own initial
+ * volume.  bemhip_sim_set_V_0, and bemhip_sim_set_params with a non-NaN V_0,
+ * reset every part to the scalar V_0; set_params resets nothing else. */
+int bemhip_sim_set_part_physics(bemhip_sim* sim, int32_t nparts, const double* V_0, const double* field_amp,
+                                const double* field_freq, const double* field_phase);
 /* LinLinSim::position_t (LinLinSim.cpp:26-83) on the current state: vel N x 3
  * and psi N (either may be null) */
 int bemhip_sim_position_t(bemhip_sim* sim, double* vel, double* psi);
@@ -147,6 +169,21 @@ int bemhip_sim_last_stats(bemhip_sim* sim, bemhip_solve_stats* stats);
 /* sums of the timing fields (and iteration counts) over all stages since the
  * last reset; residual / correction hold the maxima */
 int bemhip_sim_accum_stats(bemhip_sim* sim, bemhip_solve_stats* stats, int reset);
+
+/* ---- adjoint of a radius trajectory (no counterpart in the reference) ----
+ * From the simulation's current state (x0, phi0, time), runs `nsteps` RK4
+ * steps of fixed size dt (LinLinSim::evolve_system_RK4 with fixdt, no wall)
+ * and evaluates L = sum_{n=1..nsteps} (R_n - target[n-1])^2 with R_n the
+ * volume-equivalent radius (3 V_n / 4 pi)^(1/3) of the whole mesh after step
+ * n (radius.cpp:45).  The reverse sweep recomputes each segment of `every`
+ * steps from a checkpoint, re-assembles and re-factors every stage and solves
+ * the transposed systems with those LU factors; no dG/dx is formed.  Outputs
+ * L, the radii (nsteps, may be null) and dL/dx0 (N x 3) and dL/dphi0 (N)
+ * (either may be null), with V_0, the forcing and the mesh connectivity held
+ * fixed.  Needs sigma = 0, no wall mode, the coupled problem and an LU-based
+ * solver.  The simulation's state and time are left unchanged. */
+int bemhip_sim_adjoint_radius(bemhip_sim* sim, int32_t nsteps, double dt, const double* target, int32_t every,
+                              double* loss, double* radius, double* grad_x, double* grad_phi);
 
 #ifdef __cplusplus
 }

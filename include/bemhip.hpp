@@ -55,12 +55,12 @@ class ColocSim {
 public:
     ColocSim(Context& ctx, const std::vector<vec3d>& verts, const std::vector<tri3>& trigs, double p_inf = 1.0,
              double epsilon = 1.0, double sigma = 0.0, double gamma = 1.0, PressureField field = {},
-             bool mirror = false)
+             bool mirror = false, bool independent_parts = false)
         : ctx_(ctx) {
         bemhip_sim_default_params(&p_);
         p_.p_inf = p_inf; p_.epsilon = epsilon; p_.sigma = sigma; p_.gamma = gamma;
         p_.field_amp = field.amp; p_.field_freq = field.freq; p_.field_phase = field.phase;
-        p_.flags = mirror ? BEMHIP_MIRROR_X0 : 0u;
+        p_.flags = (mirror ? BEMHIP_MIRROR_X0 : 0u) | (independent_parts ? BEMHIP_INDEPENDENT_PARTS : 0u);
         p_.wall_guards = mirror ? 1 : 0;
         std::vector<uint32_t> t = narrow(trigs, verts.size());
         ctx_.check(bemhip_sim_create(ctx_.get(), &p_, (int64_t)verts.size(), verts.empty() ? nullptr : verts[0].data(),
@@ -114,6 +114,18 @@ public:
     }
     double get_time() const { double t; state(nullptr, nullptr, nullptr, &t); return t; }
     double get_volume() const { double v; ctx_.check(bemhip_sim_get_volume(h_, &v)); return v; }
+    // loose parts and, for a batch of independent bubbles, their own V_0 and forcing
+    int nparts() const { int32_t n = 0; ctx_.check(bemhip_sim_get_parts(h_, &n)); return n; }
+    std::vector<double> part_volumes() const {
+        std::vector<double> v(nparts());
+        ctx_.check(bemhip_sim_get_part_volumes(h_, v.data(), (int32_t)v.size()));
+        return v;
+    }
+    void set_part_physics(const std::vector<double>& V_0, const std::vector<double>& amp = {},
+                          const std::vector<double>& freq = {}, const std::vector<double>& phase = {}) {
+        auto ptr = [](const std::vector<double>& a) { return a.empty() ? nullptr : a.data(); };
+        ctx_.check(bemhip_sim_set_part_physics(h_, nparts(), ptr(V_0), ptr(amp), ptr(freq), ptr(phase)));
+    }
     std::vector<vec3d> get_vertices() const { std::vector<vec3d> v(N_); state(v[0].data(), nullptr, nullptr, nullptr); return v; }
     std::vector<double> get_phi() const { std::vector<double> v(N_); state(nullptr, v.data(), nullptr, nullptr); return v; }
     std::vector<double> get_psi() const { std::vector<double> v(N_); state(nullptr, nullptr, v.data(), nullptr); return v; }

@@ -51,7 +51,7 @@ def test_lu64_with_pivoting(ctx):
     x, st = ctx.solve(A, b, "lu64")
     assert rel(x, np.linalg.solve(A, b)) < 1e-10
     # the speculative (interchange-free) panel is rejected and GEPP takes over
-    assert st["panels"] >= 4 and st["panels_pivoted"] == st["panels"]
+    assert st["panels"] >= 2 and st["panels_pivoted"] == st["panels"]
     for solver in ("lu32_ir64", "lu16x3_ir64"):
         xm, stm = ctx.solve(A, b, solver)
         assert rel(xm, np.linalg.solve(A, b)) < 1e-10 and stm["panels_pivoted"] >= 1
