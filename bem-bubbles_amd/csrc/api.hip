@@ -3,6 +3,10 @@
 // Simulation.hpp:29-167).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <numeric>
+#include <thread>
+#include <exception>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -83,6 +87,85 @@ float event_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+void merge_stats(bemhip_solve_stats& sum, const bemhip_solve_stats& one) {
+    sum.iterations = std::max(sum.iterations, one.iterations);
+    sum.converged = sum.converged && one.converged;
+    sum.residual = std::max(sum.residual, one.residual);
+    sum.correction = std::max(sum.correction, one.correction);
+    sum.ms_factor += one.ms_factor;
+    sum.ms_solve += one.ms_solve;
+    sum.gemm_ms += one.gemm_ms;
+    sum.gemm_flops += one.gemm_flops;
+    sum.gemm_launches += one.gemm_launches;
+    sum.panels += one.panels;
+    sum.panels_pivoted += one.panels_pivoted;
+}
+
+// Independent problems of a batch: the largest first, dealt to up to
+// max_lanes solvers by accumulated n^3 (the LU cost); every solver has its
+// own streams and work buffers and is driven by its own host thread, so the
+// many short, latency-bound kernels of different problems overlap on the
+// device.  The solvers start after the assembly (an event on the context's
+// stream) and the context's stream waits for all of them.
+void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G32, const double* d_hphi,
+                 double* d_psi, int solver, double tol, int maxit, bemhip_solve_stats* st, bool report_residual) {
+    const int np = (int)topo.probs.size();
+    const int L = std::max(1, std::min(ctx->max_lanes, np));
+    while ((int)ctx->lanes.size() < L - 1) {
+        ctx->lanes.emplace_back(new Solver());
+        ctx->lanes.back()->init(ctx);
+    }
+    std::vector<Solver*> lane(L);
+    lane[0] = ctx;
+    for (int l = 1; l < L; ++l) lane[l] = ctx->lanes[l - 1].get();
+    std::vector<int> order(np);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return topo.probs[a].n > topo.probs[b].n; });
+    std::vector<std::vector<int>> work(L);
+    std::vector<double> load(L, 0.0);
+    for (int q : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        work[l].push_back(q);
+        load[l] += std::pow((double)topo.probs[q].n, 3.0);
+    }
+    hipEvent_t ready = ctx->ev[0];
+    BH_HIP(hipEventRecord(ready, ctx->stream));
+    for (int l = 1; l < L; ++l) BH_HIP(hipStreamWaitEvent(lane[l]->stream, ready, 0));
+    std::vector<bemhip_solve_stats> part(L);
+    std::vector<std::exception_ptr> fail(L);
+    auto run = [&](int l) {
+        try {
+            BH_HIP(hipSetDevice(ctx->device));
+            bemhip_solve_stats& sum = part[l];
+            sum = bemhip_solve_stats{};
+            sum.converged = 1;
+            for (int q : work[l]) {
+                const AsmProb& pr = topo.probs[q];
+                bemhip_solve_stats one{};
+                solve_device(lane[l], pr.n, G64 + pr.gofs, G32 ? G32 + pr.gofs : nullptr, d_hphi + pr.v0,
+                             d_psi + pr.v0, solver, tol, maxit, &one, report_residual);
+                merge_stats(sum, one);
+            }
+        } catch (...) {
+            fail[l] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int l = 1; l < L; ++l) th.emplace_back(run, l);
+    run(0);
+    for (auto& t : th) t.join();
+    for (int l = 1; l < L; ++l) {
+        hipEvent_t done = ctx->ev[1];
+        BH_HIP(hipEventRecord(done, lane[l]->stream));
+        BH_HIP(hipStreamWaitEvent(ctx->stream, done, 0));
+    }
+    for (auto& f : fail) if (f) std::rethrow_exception(f);
+    bemhip_solve_stats sum{};
+    sum.converged = 1;
+    for (const auto& p : part) merge_stats(sum, p);
+    *st = sum;
+}
+
 // assembly + H phi + solve on device-resident positions / potential
 void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi, uint32_t flags,
                 int solver, double tol, int maxit, double* d_hphi, double* d_psi, bemhip_solve_stats* st,
@@ -100,27 +183,7 @@ void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, c
     if (topo.probs.size() == 1) {
         solve_device(ctx, (int)topo.N, G64, G32, d_hphi, d_psi, solver, tol, maxit, st, report_residual);
     } else {
-        // independent problems one after the other on the context's streams;
-        // every solve reuses the same work buffers in stream order
-        bemhip_solve_stats sum{};
-        sum.converged = 1;
-        for (const AsmProb& pr : topo.probs) {
-            bemhip_solve_stats one{};
-            solve_device(ctx, pr.n, G64 + pr.gofs, G32 ? G32 + pr.gofs : nullptr, d_hphi + pr.v0, d_psi + pr.v0,
-                         solver, tol, maxit, &one, report_residual);
-            sum.iterations = std::max(sum.iterations, one.iterations);
-            sum.converged = sum.converged && one.converged;
-            sum.residual = std::max(sum.residual, one.residual);
-            sum.correction = std::max(sum.correction, one.correction);
-            sum.ms_factor += one.ms_factor;
-            sum.ms_solve += one.ms_solve;
-            sum.gemm_ms += one.gemm_ms;
-            sum.gemm_flops += one.gemm_flops;
-            sum.gemm_launches += one.gemm_launches;
-            sum.panels += one.panels;
-            sum.panels_pivoted += one.panels_pivoted;
-        }
-        *st = sum;
+        solve_batch(ctx, topo, G64, G32, d_hphi, d_psi, solver, tol, maxit, st, report_residual);
     }
     st->ms_assemble = ams;
 }
@@ -233,28 +296,30 @@ int bemhip_create(int device, bemhip_ctx** out) {
                  std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
         ctx->device = device;
         ctx->cus = prop.multiProcessorCount;
-        BH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        BH_BLAS(rocblas_create_handle(&ctx->blas));
-        BH_BLAS(rocblas_set_stream(ctx->blas, ctx->stream));
-        BH_HIP(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-        BH_BLAS(rocblas_create_handle(&ctx->blas2));
-        BH_BLAS(rocblas_set_stream(ctx->blas2, ctx->stream2));
-        BH_HIP(hipHostMalloc((void**)&ctx->host_scal, 64 * sizeof(double), hipHostMallocDefault));
         for (auto& e : ctx->ev) BH_HIP(hipEventCreate(&e));
-        for (auto& e : ctx->ev_la) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (const char* v = std::getenv("BEMHIP_LA_MIN_REST")) ctx->la_min_rest = std::atoi(v);
         const char* la = std::getenv("BEMHIP_LU_LOOKAHEAD");
         if (la && la[0] == '0') ctx->lookahead = false;
         const char* sp = std::getenv("BEMHIP_LU_SPECULATE");
         if (sp && sp[0] == '0') ctx->speculate = false;
+        auto leaf = [](const char* v, int d) {
+            const int w = v ? std::atoi(v) : d;
+            return (w == 32 || w == 64 || w == 128) ? w : d;
+        };
+        ctx->leaf32 = leaf(std::getenv("BEMHIP_LEAF32"), ctx->leaf32);
+        ctx->leaf64 = leaf(std::getenv("BEMHIP_LEAF64"), ctx->leaf64);
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
+        if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
+        ctx->init(nullptr);
     });
     if (rc != BEMHIP_OK) {
-        // keep the context alive just long enough to report; callers get no handle
+        // callers get no handle: release whatever was created
+        for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+        ctx->release();
         return rc;
     }
     *out = ctx.release();
@@ -264,16 +329,11 @@ int bemhip_create(int device, bemhip_ctx** out) {
 int bemhip_destroy(bemhip_ctx* ctx) {
     if (!ctx) return BEMHIP_OK;
     (void)hipSetDevice(ctx->device);
+    for (auto& l : ctx->lanes) l->release();
+    ctx->lanes.clear();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
-    for (auto& e : ctx->ev_la) if (e) (void)hipEventDestroy(e);
-    if (ctx->blas2) (void)rocblas_destroy_handle(ctx->blas2);
-    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
-    for (auto& e : ctx->gemm_ev) if (e) (void)hipEventDestroy(e);
-    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
-    if (ctx->host_scal) (void)hipHostFree(ctx->host_scal);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->release();
     delete ctx;
     return BEMHIP_OK;
 }

@@ -179,26 +179,25 @@ __global__ __launch_bounds__(256) void k_laswp(int ncols, T* __restrict__ A, int
     }
 }
 
-// Speculative leaf without row interchanges (up to NPL = 32 columns): every
+// Speculative leaf without row interchanges (up to W columns): every
 // work-group factors the leaf's top n x n block in LDS (redundantly) and
-// eliminates its own NPR rows against it, with no inter-work-group
+// eliminates its own R rows against it, with no inter-work-group
 // synchronisation in the arithmetic.  max |l| over all multipliers goes to
 // *maxl (bits of a non-negative double; NaN sorts above every number).  If
 // max |l| <= 1, partial pivoting with first-maximum ties (Eigen::PartialPivLU,
 // LAPACK) would have chosen the diagonal at every column, i.e. this is exactly
 // its elimination.  Work-group 0 writes the factored top rows back only after
 // every work-group has taken its copy of the original ones (monotone counter).
-constexpr int NPL = 32;    // speculative leaf width (64 measured slower in FP32 and FP64)
-constexpr int NPR = 256;   // rows per work-group of the speculative leaf
-
-template <class T>
-__global__ __launch_bounds__(NPR) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
-                                                 unsigned long long* __restrict__ maxl,
-                                                 unsigned int* __restrict__ readers, unsigned int reader_target,
-                                                 unsigned int* __restrict__ err) {
-    __shared__ T top[NPL][NPL + 1];
-    __shared__ T as[NPL][NPR];
-    __shared__ unsigned long long red[NPR / 64];
+template <class T, int W, int R>
+__global__ __launch_bounds__(R) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
+                                               unsigned long long* __restrict__ maxl,
+                                               unsigned int* __restrict__ readers, unsigned int reader_target,
+                                               unsigned int* __restrict__ err) {
+    static_assert(W <= R, "the top block is factored by one thread per row");
+    constexpr int NPR = R;
+    __shared__ T top[W][W + 1];
+    __shared__ T as[W][R];
+    __shared__ unsigned long long red[R / 64];
     const int tid = threadIdx.x;
     const int r = blockIdx.x * NPR + tid;
     for (int e = tid; e < n * n; e += NPR) {
@@ -349,6 +348,13 @@ __device__ inline int scale_exp(unsigned int bits) {
     return 12 - e;
 }
 
+__global__ void k_split_init(unsigned int* __restrict__ mx) {
+    if (threadIdx.x == 0) {
+        mx[0] = 0x3F800000u;   // 1.0f: the bound on |L21|
+        mx[1] = 0u;            // max |U12|, reduced next
+    }
+}
+
 __global__ void k_split_alpha(const unsigned int* __restrict__ mx, float* __restrict__ alpha) {
     const int ea = scale_exp(mx[0]), eb = scale_exp(mx[1]);
     alpha[0] = -ldexpf(1.0f, -(ea + eb));
@@ -391,14 +397,16 @@ __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* _
 }
 
 // fp16 splits of one outer step's L21 (m x k) and U12 (k x n) blocks
-static void split_prepare(bemhip_ctx* ctx, hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B,
+static void split_prepare(Solver* ctx, hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B,
                           int ldb) {
     __half* acat = static_cast<__half*>(ctx->split_a.ensure(sizeof(__half) * 3 * (size_t)m * k));
     __half* bcat = static_cast<__half*>(ctx->split_b.ensure(sizeof(__half) * 3 * (size_t)k * n));
     unsigned int* mx = static_cast<unsigned int*>(ctx->split_mx.ensure(64));
     float* alpha = reinterpret_cast<float*>(mx + 4);
-    k_fill_u32<<<1, 64, 0, st>>>(mx, 2, 0u);
-    k_block_absmax<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx);
+    // |L21| <= 1 for every panel that reaches a trailing update (partial
+    // pivoting, or a speculative panel accepted because max |l| <= 1), so its
+    // scale is fixed by the bound 1.0f instead of a reduction over the block
+    k_split_init<<<1, 64, 0, st>>>(mx);
     k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + 1);
     k_split_alpha<<<1, 1, 0, st>>>(mx, alpha);
     k_split_a<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx, acat);
@@ -407,7 +415,7 @@ static void split_prepare(bemhip_ctx* ctx, hipStream_t st, int m, int n, int k, 
 }
 
 // C[:, col0:col0+n] -= L21 U12[:, col0:col0+n] from the prepared splits
-static void split_gemm(bemhip_ctx* ctx, rocblas_handle h, int m, int n, int k, int col0, float* C, int ldc) {
+static void split_gemm(Solver* ctx, rocblas_handle h, int m, int n, int k, int col0, float* C, int ldc) {
     const __half* acat = ctx->split_a.as<__half>();
     const __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
     float* alpha = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4);
@@ -424,7 +432,7 @@ static void split_gemm(bemhip_ctx* ctx, rocblas_handle h, int m, int n, int k, i
 // the full B block row were prepared by split_prepare and `col0` selects the
 // first column of B used here.
 template <class T>
-static void trailing_update(bemhip_ctx* ctx, hipStream_t st, rocblas_handle h, int m, int n, int k, const T* A, int lda,
+static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m, int n, int k, const T* A, int lda,
                             const T* B, int ldb, T* C, int ldc, bool split16, int col0 = 0) {
     if (m <= 0 || n <= 0 || k <= 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -453,7 +461,7 @@ static void trailing_update(bemhip_ctx* ctx, hipStream_t st, rocblas_handle h, i
 }
 
 // sums the recorded trailing-update timings (after the stream has drained)
-static void collect_gemm_timing(bemhip_ctx* ctx) {
+static void collect_gemm_timing(Solver* ctx) {
     for (size_t i = 0; i < ctx->gemm_ev_used; ++i) {
         float ms = 0.0f;
         BH_HIP(hipEventElapsedTime(&ms, ctx->gemm_ev[2 * i], ctx->gemm_ev[2 * i + 1]));
@@ -472,7 +480,7 @@ struct Lane {
 };
 
 template <class T>
-static void leaf(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
+static void leaf(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
     const int P = (m + PROWS - 1) / PROWS;
     BH_CHECK(P <= MAXP, BEMHIP_ERR_ARG, "matrix too large for the panel kernel (N > 65536)");
     char* ws = ctx->lu.panel_ws.as<char>();
@@ -493,7 +501,7 @@ static void laswp(Lane L, int ncols, T* A, int lda, const int* ipiv, int k1, int
 }
 
 template <class T>
-static void panel_rec(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
+static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
     if (n <= LEAF) { leaf(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
     int n1 = ((n / 2 + LEAF - 1) / LEAF) * LEAF;
     if (n1 >= n) n1 = n - LEAF;
@@ -509,15 +517,34 @@ static void panel_rec(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int*
     }
 }
 
+// width of the speculative leaves: fewer, wider leaves make the panel's
+// chain of dependent launches shorter, and every launch of that chain waits
+// for compute units held by the concurrent trailing-update GEMM
+template <class T> static int leaf_width(const Solver* ctx) {
+    const int w = sizeof(T) == 8 ? ctx->leaf64 : ctx->leaf32;
+    return (sizeof(T) == 8 && w > 64) ? 64 : w;
+}
+
 template <class T>
-static void panel_rec_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+static void leaf_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+    unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
+    unsigned int* err = readers + 1;
+    const int W = leaf_width<T>(ctx);
+    const int R = W == 32 ? 256 : 128;
+    const unsigned int P = (unsigned int)((m + R - 1) / R);
+    ctx->leaf_readers += P;
+    if (W == 32) k_leaf_np<T, 32, 256><<<P, 256, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
+    else if (W == 64) k_leaf_np<T, 64, 128><<<P, 128, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
+    else k_leaf_np<T, (sizeof(T) == 8 ? 64 : 128), 128><<<P, 128, 0, L.s>>>(m, n, A, lda, maxl, readers,
+                                                                             ctx->leaf_readers, err);
+    BH_HIP(hipGetLastError());
+}
+
+template <class T>
+static void panel_rec_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+    const int NPL = leaf_width<T>(ctx);
     if (n <= NPL) {
-        unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
-        unsigned int* err = readers + 1;
-        const unsigned int P = (unsigned int)((m + NPR - 1) / NPR);
-        ctx->leaf_readers += P;
-        k_leaf_np<T><<<P, NPR, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
-        BH_HIP(hipGetLastError());
+        leaf_np(ctx, L, m, n, A, lda, maxl);
         return;
     }
     int n1 = ((n / 2 + NPL - 1) / NPL) * NPL;
@@ -536,9 +563,11 @@ static void panel_rec_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, u
 // the recursive leaves, then every row below it at once, L21 = A21 U11^-1 (one
 // large TRSM on the matrix cores), and max |l| over both parts into *maxl.
 // Without interchanges this is the same elimination as column-by-column
-// Gaussian elimination of the whole panel.
+// Gaussian elimination of the whole panel.  (A single grid-barrier launch for
+// the top block was measured slower: its spinning work-groups take compute
+// units from the concurrent trailing-update GEMM.)
 template <class T>
-static void panel_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
     panel_rec_np(ctx, L, std::min(m, n), n, A, lda, maxl);
     if (m > n) {
         trsm_runn(L.h, m - n, n, A, lda, A + n, lda);
@@ -553,12 +582,13 @@ static void panel_np(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, unsig
 // result when max |l| <= 1 (identical to GEPP's choices) and otherwise
 // restores the panel and factors it with partial pivoting.
 template <class T>
-static void panel_start(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
+static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
                         unsigned int& base) {
     if (!ctx->speculate) { panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
     T* bk = static_cast<T*>(ctx->lu.backup.ensure(sizeof(T) * (size_t)m * n));
     BH_HIP(hipMemcpy2DAsync(bk, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
-    // layout: [max |l| bits (8 B)][leaf reader counter][error flag]
+    // layout (32-bit words): [0,1] max |l| bits, [2] leaf reader counter,
+    // [3] leaf hand-off error
     unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
     k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 4, 0u);
     ctx->leaf_readers = 0;
@@ -569,14 +599,14 @@ static void panel_start(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, in
 
 // returns true when the panel was factored with row interchanges
 template <class T>
-static bool panel_finish(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
+static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
                          unsigned int& base) {
     ctx->panels += 1;
     if (!ctx->speculate) { ctx->panels_pivoted += 1; return true; }
     BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
     const double ml = ctx->host_scal[32];
-    const unsigned int* words = reinterpret_cast<const unsigned int*>(ctx->host_scal + 33);
-    BH_CHECK(words[1] == 0u, BEMHIP_ERR_HIP, "speculative panel: top-block hand-off timed out");
+    const unsigned int* words = reinterpret_cast<const unsigned int*>(ctx->host_scal + 32);
+    BH_CHECK(words[3] == 0u, BEMHIP_ERR_HIP, "speculative panel: top-block hand-off timed out");
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
@@ -593,7 +623,7 @@ static bool panel_finish(bemhip_ctx* ctx, Lane L, int m, int n, T* A, int lda, i
 // while the main stream applies step k's update to the trailing matrix, the
 // panel of step k+1 (whose columns were updated first) is factored on the
 // second stream.
-template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16) {
+template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16) {
     // outer block: the split-fp16 update runs at fp16 MFMA speed, so its K must
     // be larger for the C read/write of each update to stay below the compute
     const int LU_NB = split16 ? ctx->nb_split : (sizeof(T) == 8 ? ctx->nb64 : ctx->nb32);
@@ -653,8 +683,8 @@ template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool s
     return 0;
 }
 
-template int lu_factor<double>(bemhip_ctx*, int, double*, int*, bool);
-template int lu_factor<float>(bemhip_ctx*, int, float*, int*, bool);
+template int lu_factor<double>(Solver*, int, double*, int*, bool);
+template int lu_factor<float>(Solver*, int, float*, int*, bool);
 
 // --------------------------------------------------------------- solves --
 constexpr int TS = 128;  // triangular solve block
@@ -864,7 +894,7 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     }
 }
 
-template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU) {
+template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
     const int nb = (N + TS - 1) / TS;
     double* il = static_cast<double*>(ctx->lu.inv_lo.ensure(sizeof(double) * (size_t)nb * TS * TS));
     double* iu = static_cast<double*>(ctx->lu.inv_up.ensure(sizeof(double) * (size_t)nb * TS * TS));
@@ -882,10 +912,10 @@ template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU) {
     }
     ctx->lu.inv_n = N;
 }
-template void lu_prepare_solve<double>(bemhip_ctx*, int, const double*);
-template void lu_prepare_solve<float>(bemhip_ctx*, int, const float*);
+template void lu_prepare_solve<double>(Solver*, int, const double*);
+template void lu_prepare_solve<float>(Solver*, int, const float*);
 
-template <class T> static void trsv_persist(bemhip_ctx* ctx, int N, const T* LU, double* x, bool lower) {
+template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, double* x, bool lower) {
     const int nb = (N + TS - 1) / TS;
     unsigned int* sync = ctx->lu.flag.as<unsigned int>();
     const unsigned int epoch = ++ctx->trsv_epoch;
@@ -895,7 +925,7 @@ template <class T> static void trsv_persist(bemhip_ctx* ctx, int N, const T* LU,
     BH_HIP(hipGetLastError());
 }
 
-template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int* perm, double* x) {
+template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* perm, double* x) {
     hipStream_t st = ctx->stream;
     double* tmp = static_cast<double*>(ctx->vec[7].ensure(sizeof(double) * N));
     BH_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -919,8 +949,8 @@ template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int*
     }
     BH_HIP(hipGetLastError());
 }
-template void lu_solve<double>(bemhip_ctx*, int, const double*, const int*, double*);
-template void lu_solve<float>(bemhip_ctx*, int, const float*, const int*, double*);
+template void lu_solve<double>(Solver*, int, const double*, const int*, double*);
+template void lu_solve<float>(Solver*, int, const float*, const int*, double*);
 
 // ------------------------------------------------------ gemv, reductions --
 constexpr int GV_CHUNKS = 32;
@@ -955,7 +985,7 @@ __global__ __launch_bounds__(256) void k_gemv_finish(int N, int chunks, const do
     r[i] = b ? b[i] - s : s;
 }
 
-void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const double* b, double* r) {
+void gemv64(Solver* ctx, int N, const double* A, const double* x, const double* b, double* r) {
     const int chunks = std::min(GV_CHUNKS, N);
     const int chunk = (N + chunks - 1) / chunks;
     const int nch = (N + chunk - 1) / chunk;
@@ -986,7 +1016,7 @@ __global__ __launch_bounds__(256) void k_reduce_part(int n, const double* __rest
 }
 
 template <int OP>
-static double reduce(bemhip_ctx* ctx, int n, const double* x, const double* y) {
+static double reduce(Solver* ctx, int n, const double* x, const double* y) {
     const int nb = 256;
     double* part = static_cast<double*>(ctx->scal.ensure(sizeof(double) * 1024));
     k_reduce_part<OP><<<nb, 256, 0, ctx->stream>>>(n, x, y, part);
@@ -995,8 +1025,8 @@ static double reduce(bemhip_ctx* ctx, int n, const double* x, const double* y) {
     BH_HIP(hipStreamSynchronize(ctx->stream));
     return ctx->host_scal[0];
 }
-double dev_absmax(bemhip_ctx* ctx, int n, const double* x) { return reduce<0>(ctx, n, x, nullptr); }
-double dev_dot(bemhip_ctx* ctx, int n, const double* x, const double* y) { return reduce<1>(ctx, n, x, y); }
+double dev_absmax(Solver* ctx, int n, const double* x) { return reduce<0>(ctx, n, x, nullptr); }
+double dev_dot(Solver* ctx, int n, const double* x, const double* y) { return reduce<1>(ctx, n, x, y); }
 
 // y = a*x + b*y (+ c*z when z)
 __global__ __launch_bounds__(256) void k_axpby(int n, double a, const double* __restrict__ x, double b,
@@ -1007,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_axpby(int n, double a, const double* __
     if (z) v += c * z[i];
     y[i] = v;
 }
-static void axpby(bemhip_ctx* ctx, int n, double a, const double* x, double b, double* y, double c = 0.0,
+static void axpby(Solver* ctx, int n, double a, const double* x, double b, double* y, double c = 0.0,
                   const double* z = nullptr) {
     k_axpby<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, a, x, b, y, c, z);
 }
@@ -1025,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_mul(int n, const double* __restrict__ a
 }
 
 // Eigen 3.4 bicgstab() with DiagonalPreconditioner, zero initial guess
-static void bicgstab(bemhip_ctx* ctx, int N, const double* A, const double* rhs, double* x, double tol, int maxit,
+static void bicgstab(Solver* ctx, int N, const double* A, const double* rhs, double* x, double tol, int maxit,
                      bemhip_solve_stats* st) {
     if (tol <= 0.0) tol = 2.220446049250313e-16;
     if (maxit <= 0) maxit = 2 * N;
@@ -1090,7 +1120,7 @@ static void bicgstab(bemhip_ctx* ctx, int N, const double* A, const double* rhs,
 }
 
 // ------------------------------------------------------------ front end --
-static int* make_perm(bemhip_ctx* ctx, int N, const int* d_ipiv) {
+static int* make_perm(Solver* ctx, int N, const int* d_ipiv) {
     ctx->h_ipiv.resize(N);
     BH_HIP(hipMemcpyAsync(ctx->h_ipiv.data(), d_ipiv, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
     BH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1101,14 +1131,14 @@ static int* make_perm(bemhip_ctx* ctx, int N, const int* d_ipiv) {
     return d;
 }
 
-static float elapsed(bemhip_ctx* ctx, hipEvent_t a, hipEvent_t b) {
+static float elapsed(Solver* ctx, hipEvent_t a, hipEvent_t b) {
     BH_HIP(hipEventSynchronize(b));
     float ms = 0.0f;
     BH_HIP(hipEventElapsedTime(&ms, a, b));
     return ms;
 }
 
-void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double* b, double* x, int solver,
+void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, double* x, int solver,
                   double tol, int maxit, bemhip_solve_stats* st, bool report_residual) {
     hipStream_t sm = ctx->stream;
     int* ipiv = static_cast<int*>(ctx->lu.ipiv.ensure(sizeof(int) * N));
@@ -1195,7 +1225,7 @@ __global__ void k_scatter_perm(int n, const int* __restrict__ perm, const T* __r
 }
 
 // x = G^-T b from FP64 factors
-static void lu_solve_t64(bemhip_ctx* ctx, int N, const double* LU, const int* perm, const double* b, double* x) {
+static void lu_solve_t64(Solver* ctx, int N, const double* LU, const int* perm, const double* b, double* x) {
     hipStream_t st = ctx->stream;
     double* t = static_cast<double*>(ctx->vec[8].ensure(sizeof(double) * N));
     BH_HIP(hipMemcpyAsync(t, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -1208,7 +1238,7 @@ static void lu_solve_t64(bemhip_ctx* ctx, int N, const double* LU, const int* pe
 }
 
 // x = G^-T b from FP32 factors (b rounded to FP32; refinement recovers it)
-static void lu_solve_t32(bemhip_ctx* ctx, int N, const float* LU, const int* perm, const double* b, double* x) {
+static void lu_solve_t32(Solver* ctx, int N, const float* LU, const int* perm, const double* b, double* x) {
     hipStream_t st = ctx->stream;
     float* t = static_cast<float*>(ctx->vec[9].ensure(sizeof(float) * N));
     k_d2f_vec<<<(N + 255) / 256, 256, 0, st>>>(N, b, t);
@@ -1220,7 +1250,7 @@ static void lu_solve_t32(bemhip_ctx* ctx, int N, const float* LU, const int* per
     BH_HIP(hipGetLastError());
 }
 
-void solve_transposed_device(bemhip_ctx* ctx, int N, const double* G64, const float* G32, int solver,
+void solve_transposed_device(Solver* ctx, int N, const double* G64, const float* G32, int solver,
                              const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st) {
     const int* perm = ctx->perm.as<int>();
     BH_CHECK(perm != nullptr, BEMHIP_ERR_ARG, "transposed solve needs the factors of a previous LU solve");
@@ -1255,6 +1285,44 @@ void solve_transposed_device(bemhip_ctx* ctx, int N, const double* G64, const fl
     st->iterations = it;
     st->converged = conv ? 1 : 0;
     st->correction = cn;
+}
+
+}  // namespace bemhip
+
+namespace bemhip {
+
+void Solver::init(const Solver* cfg) {
+    if (cfg) {
+        la_min_rest = cfg->la_min_rest;
+        lookahead = cfg->lookahead;
+        speculate = cfg->speculate;
+        leaf32 = cfg->leaf32;
+        leaf64 = cfg->leaf64;
+        nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
+        trsv_persistent = cfg->trsv_persistent;
+        timing = cfg->timing;
+    }
+    BH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    BH_BLAS(rocblas_create_handle(&blas));
+    BH_BLAS(rocblas_set_stream(blas, stream));
+    BH_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    BH_BLAS(rocblas_create_handle(&blas2));
+    BH_BLAS(rocblas_set_stream(blas2, stream2));
+    BH_HIP(hipHostMalloc((void**)&host_scal, 64 * sizeof(double), hipHostMallocDefault));
+    for (auto& e : ev_la) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+void Solver::release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    for (auto& e : ev_la) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    for (auto& e : gemm_ev) if (e) (void)hipEventDestroy(e);
+    gemm_ev.clear();
+    if (blas2) { (void)rocblas_destroy_handle(blas2); blas2 = nullptr; }
+    if (stream2) { (void)hipStreamDestroy(stream2); stream2 = nullptr; }
+    if (blas) { (void)rocblas_destroy_handle(blas); blas = nullptr; }
+    if (host_scal) { (void)hipHostFree(host_scal); host_scal = nullptr; }
+    if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
 
 }  // namespace bemhip

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -134,43 +135,61 @@ struct LuWork {
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
 };
 
-}  // namespace bemhip
-
-struct bemhip_ctx {
-    int device = 0;
+// Everything one dense solve needs: its streams, BLAS handles, work buffers,
+// pinned host words and instrumentation.  The context is one solver (lane 0);
+// a batch of independent problems adds more, each driven by its own host
+// thread so the problems' latency-bound kernels overlap on the device.
+struct Solver {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     hipStream_t stream2 = nullptr;      // LU look-ahead: panels of the next block column
     rocblas_handle blas2 = nullptr;
-    // look-ahead only while the trailing update of a step is long enough to
-    // hide the next panel: with fewer than this many columns left after it,
-    // the panel runs after the update on the main stream, uncontended
-    int la_min_rest = 16384;
+    // look-ahead only while the trailing update of a step has at least this
+    // many columns besides the next panel's (0: always; measured best at
+    // N = 40962 in FP64 and mixed precision)
+    int la_min_rest = 0;
     hipEvent_t ev_la[3] = {};
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
+    int leaf32 = 32, leaf64 = 32;       // speculative leaf width (32, 64 or 128; FP64 at most 64)
     int nb64 = 512, nb32 = 512, nb_split = 1024;   // LU outer block per precision
     unsigned int leaf_readers = 0;                  // running target of the speculative-leaf hand-off
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
     unsigned int trsv_epoch = 0;
-    std::string err;
-    int cus = 0;
     // workspace
-    bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H, vec[12], scal, gemv_part;
-    bemhip::LuWork lu;
-    bemhip::DevBuf split_a, split_b, split_mx, perm;
-    bemhip::DevBuf adj_coef, adj_tc, adj_tpart, adj_rpart, adj_sc, adj_zero;   // adjoint partials
+    DevBuf vec[12], scal, gemv_part;
+    LuWork lu;
+    DevBuf split_a, split_b, split_mx, perm;
     std::vector<int> h_ipiv, h_perm;
     double* host_scal = nullptr;  // pinned
     // instrumentation of trailing updates
     bool timing = true;
-    hipEvent_t ev[4] = {};
     std::vector<hipEvent_t> gemm_ev;   // start/stop pairs of the trailing updates of one factorisation
     std::vector<double> gemm_fl;
     size_t gemm_ev_used = 0;
     double gemm_ms = 0.0, gemm_flops = 0.0;
     int gemm_launches = 0;
+
+    // streams, handles, pinned words and events on the current device; the
+    // tuning knobs are copied from `cfg` when given
+    void init(const Solver* cfg);
+    void release();
+};
+
+}  // namespace bemhip
+
+struct bemhip_ctx : bemhip::Solver {
+    int device = 0;
+    std::string err;
+    int cus = 0;
+    // assembly and per-problem buffers
+    bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H;
+    bemhip::DevBuf adj_coef, adj_tc, adj_tpart, adj_rpart, adj_sc, adj_zero;   // adjoint partials
+    hipEvent_t ev[4] = {};
+    // extra solvers for batches of independent problems (created on first use)
+    std::vector<std::unique_ptr<bemhip::Solver>> lanes;
+    int max_lanes = 4;
 };
 
 namespace bemhip {
@@ -187,24 +206,24 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
 
 // ---------------------------------------------------------- dense LA --
 // LU with partial pivoting (column-major, lda = N), pivots as absolute rows.
-template <class T> int lu_factor(bemhip_ctx* ctx, int N, T* A, int* ipiv, bool split16 = false);
+template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
-template <class T> void lu_solve(bemhip_ctx* ctx, int N, const T* LU, const int* ipiv, double* x);
+template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* ipiv, double* x);
 // after a factorisation: diagonal-block inverses for the persistent solves
-template <class T> void lu_prepare_solve(bemhip_ctx* ctx, int N, const T* LU);
+template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU);
 // r = b - A x (A FP64 column-major), or y = A x when b == nullptr
-void gemv64(bemhip_ctx* ctx, int N, const double* A, const double* x, const double* b, double* r);
+void gemv64(Solver* ctx, int N, const double* A, const double* x, const double* b, double* r);
 // solver front end on a device-resident FP64 G (destroyed by LU64) and rhs b.
-void solve_device(bemhip_ctx* ctx, int N, double* G64, float* G32, const double* b, double* x,
+void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, double* x,
                   int solver, double tol, int maxit, bemhip_solve_stats* st, bool report_residual = true);
 // G^T x = b with the factors left by the last LU-based solve_device call on
 // this context (same N and solver; G64 is the unfactored FP64 G for the mixed
 // solvers, whose transposed solve is refined against it)
-void solve_transposed_device(bemhip_ctx* ctx, int N, const double* G64, const float* G32, int solver,
+void solve_transposed_device(Solver* ctx, int N, const double* G64, const float* G32, int solver,
                              const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st);
 // device reductions
-double dev_absmax(bemhip_ctx* ctx, int n, const double* x);
-double dev_dot(bemhip_ctx* ctx, int n, const double* x, const double* y);
+double dev_absmax(Solver* ctx, int n, const double* x);
+double dev_dot(Solver* ctx, int n, const double* x, const double* y);
 
 // ------------------------------------------------------------- stage --
 // scalars of the Bernoulli right-hand side; V_0 and the forcing are per part
