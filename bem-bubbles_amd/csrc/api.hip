@@ -302,15 +302,10 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (la && la[0] == '0') ctx->lookahead = false;
         const char* sp = std::getenv("BEMHIP_LU_SPECULATE");
         if (sp && sp[0] == '0') ctx->speculate = false;
-        auto leaf = [](const char* v, int d) {
-            const int w = v ? std::atoi(v) : d;
-            return (w == 32 || w == 64 || w == 128) ? w : d;
-        };
-        ctx->leaf32 = leaf(std::getenv("BEMHIP_LEAF32"), ctx->leaf32);
-        ctx->leaf64 = leaf(std::getenv("BEMHIP_LEAF64"), ctx->leaf64);
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_NB_RATIO")) ctx->nb_ratio = std::max(0, std::atoi(v));
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));

@@ -179,55 +179,106 @@ __global__ __launch_bounds__(256) void k_laswp(int ncols, T* __restrict__ A, int
     }
 }
 
-// Speculative leaf without row interchanges (up to W columns): every
-// work-group factors the leaf's top n x n block in LDS (redundantly) and
-// eliminates its own R rows against it, with no inter-work-group
-// synchronisation in the arithmetic.  max |l| over all multipliers goes to
-// *maxl (bits of a non-negative double; NaN sorts above every number).  If
-// max |l| <= 1, partial pivoting with first-maximum ties (Eigen::PartialPivLU,
-// LAPACK) would have chosen the diagonal at every column, i.e. this is exactly
-// its elimination.  Work-group 0 writes the factored top rows back only after
-// every work-group has taken its copy of the original ones (monotone counter).
-template <class T, int W, int R>
-__global__ __launch_bounds__(R) void k_leaf_np(int m, int n, T* __restrict__ A, int lda,
-                                               unsigned long long* __restrict__ maxl,
-                                               unsigned int* __restrict__ readers, unsigned int reader_target,
-                                               unsigned int* __restrict__ err) {
-    static_assert(W <= R, "the top block is factored by one thread per row");
-    constexpr int NPR = R;
-    __shared__ T top[W][W + 1];
-    __shared__ T as[W][R];
-    __shared__ unsigned long long red[R / 64];
-    const int tid = threadIdx.x;
-    const int r = blockIdx.x * NPR + tid;
-    for (int e = tid; e < n * n; e += NPR) {
-        const int i = e % n, j = e / n;
+// One 32-column step of the interchange-free factorisation of a panel's top
+// block.  The block column starts at A (its diagonal element) and has m rows
+// and bs <= 32 columns; U12 is the ncols columns to its right (rows 0..bs-1).
+// Every work-group
+//  * factors the bs x bs diagonal block in wave 0 (a row per lane, in
+//    registers, the pivot row broadcast by shuffles) and keeps it in LDS,
+//  * eliminates its 256 rows below the diagonal block (a row per thread,
+//    in registers),
+//  * solves its 256 columns of U12 = L11^-1 A12 (a column per thread),
+// with no inter-work-group synchronisation in the arithmetic.  max |l| over
+// all multipliers goes to *maxl (bits of a non-negative double; NaN sorts
+// above every number).  If max |l| <= 1, partial pivoting with first-maximum
+// ties (Eigen::PartialPivLU, LAPACK) would have chosen the diagonal at every
+// column, i.e. this is exactly its elimination.  Work-group 0 writes the
+// factored diagonal block back only after every work-group has read the
+// original one (monotone counter).
+constexpr int NPB = 32;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __restrict__ A, int lda,
+                                                 unsigned long long* __restrict__ maxl,
+                                                 unsigned int* __restrict__ readers, unsigned int reader_target,
+                                                 unsigned int* __restrict__ err) {
+    __shared__ T top[NPB][NPB + 1];
+    __shared__ unsigned long long red[4];
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (int e = tid; e < bs * bs; e += 256) {
+        const int i = e % bs, j = e / bs;
         top[i][j] = A[(size_t)j * lda + i];
     }
     __syncthreads();
     if (tid == 0) __hip_atomic_fetch_add(readers, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     double lmax = 0.0;
-    for (int j = 0; j < n; ++j) {
-        if (tid > j && tid < n) {
-            const T l = top[tid][j] / top[j][j];
-            top[tid][j] = l;
-            const double a = fabs((double)l);
-            if (!(a <= lmax)) lmax = a;   // NaN sticks
-            for (int k = j + 1; k < n; ++k) top[tid][k] -= l * top[j][k];
+    if (tid < 64) {
+        const bool row = lane < bs;
+        T r[NPB];
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) r[k] = (row && k < bs) ? top[lane][k] : T(0);
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            if (j < bs) {
+                const T piv = __shfl(r[j], j);
+                const bool below = row && lane > j;
+                const T l = r[j] / piv;
+                if (below) {
+                    r[j] = l;
+                    const double a = fabs((double)l);
+                    if (blockIdx.x == 0 && !(a <= lmax)) lmax = a;   // reported once; NaN sticks
+                }
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) {
+                    const T pk = __shfl(r[k], j);
+                    if (below) r[k] -= l * pk;
+                }
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPB; ++k)
+            if (row && k < bs) top[lane][k] = r[k];
     }
-    if (blockIdx.x != 0) lmax = 0.0;   // the top block is reported once
-    if (r >= n && r < m) {
-        for (int j = 0; j < n; ++j) as[j][tid] = A[(size_t)j * lda + r];
-        for (int j = 0; j < n; ++j) {
-            const T l = as[j][tid] / top[j][j];
-            as[j][tid] = l;
-            const double a = fabs((double)l);
-            if (!(a <= lmax)) lmax = a;
-            for (int k = j + 1; k < n; ++k) as[k][tid] -= l * top[j][k];
+    __syncthreads();
+    // rows below the diagonal block
+    const int r = bs + blockIdx.x * 256 + tid;
+    if (r < m) {
+        T a[NPB];
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) a[j] = j < bs ? A[(size_t)j * lda + r] : T(0);
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            if (j < bs) {
+                const T l = a[j] / top[j][j];
+                a[j] = l;
+                const double al = fabs((double)l);
+                if (!(al <= lmax)) lmax = al;
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) a[k] -= l * top[j][k];   // k >= bs: never stored
+            }
         }
-        for (int j = 0; j < n; ++j) A[(size_t)j * lda + r] = as[j][tid];
+#pragma unroll
+        for (int j = 0; j < NPB; ++j)
+            if (j < bs) A[(size_t)j * lda + r] = a[j];
+    }
+    __syncthreads();   // keeps the compiler from overlapping the two register-heavy phases
+    // U12 = L11^-1 A12
+    const int c = blockIdx.x * 256 + tid;
+    if (c < ncols) {
+        T* col = A + (size_t)(bs + c) * lda;
+        T u[NPB];
+#pragma unroll
+        for (int i = 0; i < NPB; ++i) u[i] = i < bs ? col[i] : T(0);
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) {   // column-oriented forward substitution; i >= bs: never stored
+            if (k < bs) {
+#pragma unroll
+                for (int i = k + 1; i < NPB; ++i) u[i] -= top[i][k] * u[k];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NPB; ++i)
+            if (i < bs) col[i] = u[i];
     }
     unsigned long long bits = (unsigned long long)__double_as_longlong(lmax);
 #pragma unroll
@@ -235,11 +286,11 @@ __global__ __launch_bounds__(R) void k_leaf_np(int m, int n, T* __restrict__ A, 
         const unsigned long long o = __shfl_xor(bits, off);
         bits = o > bits ? o : bits;
     }
-    if ((tid & 63) == 0) red[tid >> 6] = bits;
+    if (lane == 0) red[tid >> 6] = bits;
     __syncthreads();
     if (tid == 0) {
         unsigned long long b = red[0];
-        for (int w = 1; w < NPR / 64; ++w) b = red[w] > b ? red[w] : b;
+        for (int w = 1; w < 4; ++w) b = red[w] > b ? red[w] : b;
         atomicMax(maxl, b);
     }
     if (blockIdx.x == 0) {
@@ -251,8 +302,59 @@ __global__ __launch_bounds__(R) void k_leaf_np(int m, int n, T* __restrict__ A, 
             }
         }
         __syncthreads();
-        if (tid < n)
-            for (int j = 0; j < n; ++j) A[(size_t)j * lda + tid] = top[tid][j];
+        for (int e = tid; e < bs * bs; e += 256) {
+            const int i = e % bs, j = e / bs;
+            A[(size_t)j * lda + i] = top[i][j];
+        }
+    }
+}
+
+// C (mrows x ncols) -= L (mrows x bs) U (bs x ncols), column-major, bs <= 32:
+// the rank-bs update that follows a k_np_step, one 64 x 64 tile per work-group.
+template <class T>
+__global__ __launch_bounds__(256) void k_np_update(int mrows, int ncols, int bs, const T* __restrict__ L,
+                                                   const T* __restrict__ U, T* __restrict__ C, int lda) {
+    __shared__ T ls[64][NPB + 1];
+    __shared__ T us[NPB][64 + 1];
+    const int tid = threadIdx.x;
+    const int ti = blockIdx.x * 64, tj = blockIdx.y * 64;
+    for (int e = tid; e < 64 * NPB; e += 256) {
+        const int i = e & 63, k = e >> 6;
+        ls[i][k] = (ti + i < mrows && k < bs) ? L[(size_t)k * lda + ti + i] : T(0);
+    }
+    for (int e = tid; e < NPB * 64; e += 256) {
+        const int k = e % NPB, j = e / NPB;
+        us[k][j] = (tj + j < ncols && k < bs) ? U[(size_t)(tj + j) * lda + k] : T(0);
+    }
+    __syncthreads();
+    const int ty = tid >> 4, tx = tid & 15;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+#pragma unroll 8
+    for (int k = 0; k < NPB; ++k) {
+        T lv[4], uv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lv[q] = ls[ty * 4 + q][k];
+            uv[q] = us[k][tx * 4 + q];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += lv[a] * uv[b];
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int c = tj + tx * 4 + b;
+        if (c >= ncols) continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int r = ti + ty * 4 + a;
+            if (r < mrows) C[(size_t)c * lda + r] -= acc[a][b];
+        }
     }
 }
 
@@ -517,50 +619,9 @@ static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipi
     }
 }
 
-// width of the speculative leaves: fewer, wider leaves make the panel's
-// chain of dependent launches shorter, and every launch of that chain waits
-// for compute units held by the concurrent trailing-update GEMM
-template <class T> static int leaf_width(const Solver* ctx) {
-    const int w = sizeof(T) == 8 ? ctx->leaf64 : ctx->leaf32;
-    return (sizeof(T) == 8 && w > 64) ? 64 : w;
-}
-
-template <class T>
-static void leaf_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
-    unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
-    unsigned int* err = readers + 1;
-    const int W = leaf_width<T>(ctx);
-    const int R = W == 32 ? 256 : 128;
-    const unsigned int P = (unsigned int)((m + R - 1) / R);
-    ctx->leaf_readers += P;
-    if (W == 32) k_leaf_np<T, 32, 256><<<P, 256, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
-    else if (W == 64) k_leaf_np<T, 64, 128><<<P, 128, 0, L.s>>>(m, n, A, lda, maxl, readers, ctx->leaf_readers, err);
-    else k_leaf_np<T, (sizeof(T) == 8 ? 64 : 128), 128><<<P, 128, 0, L.s>>>(m, n, A, lda, maxl, readers,
-                                                                             ctx->leaf_readers, err);
-    BH_HIP(hipGetLastError());
-}
-
-template <class T>
-static void panel_rec_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
-    const int NPL = leaf_width<T>(ctx);
-    if (n <= NPL) {
-        leaf_np(ctx, L, m, n, A, lda, maxl);
-        return;
-    }
-    int n1 = ((n / 2 + NPL - 1) / NPL) * NPL;
-    if (n1 >= n) n1 = n - NPL;
-    const int n2 = n - n1;
-    panel_rec_np(ctx, L, m, n1, A, lda, maxl);
-    T* A12 = A + (size_t)n1 * lda;
-    trsm_llnu(L.h, n1, n2, A, lda, A12, lda);
-    if (m > n1) {
-        gemm_minus(L.h, m - n1, n2, n1, A + n1, lda, A12, lda, A12 + n1, lda);
-        panel_rec_np(ctx, L, m - n1, n2, A12 + n1, lda, maxl);
-    }
-}
-
 // Interchange-free factorisation of an m x n panel: the top n x n block by
-// the recursive leaves, then every row below it at once, L21 = A21 U11^-1 (one
+// 32-column steps (k_np_step, then the rank-32 update of the rest of the top
+// block, k_np_update), then every row below it at once, L21 = A21 U11^-1 (one
 // large TRSM on the matrix cores), and max |l| over both parts into *maxl.
 // Without interchanges this is the same elimination as column-by-column
 // Gaussian elimination of the whole panel.  (A single grid-barrier launch for
@@ -568,7 +629,20 @@ static void panel_rec_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsig
 // units from the concurrent trailing-update GEMM.)
 template <class T>
 static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
-    panel_rec_np(ctx, L, std::min(m, n), n, A, lda, maxl);
+    unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
+    unsigned int* err = readers + 1;
+    for (int j0 = 0; j0 < n; j0 += NPB) {
+        const int bs = std::min(NPB, n - j0);
+        const int rows = n - j0, ncols = n - j0 - bs;
+        T* Ajj = A + (size_t)j0 * lda + j0;
+        const int P = std::max(1, std::max((rows - bs + 255) / 256, (ncols + 255) / 256));
+        ctx->leaf_readers += (unsigned int)P;
+        k_np_step<T><<<P, 256, 0, L.s>>>(rows, bs, ncols, Ajj, lda, maxl, readers, ctx->leaf_readers, err);
+        if (rows > bs && ncols > 0)
+            k_np_update<T><<<dim3((rows - bs + 63) / 64, (ncols + 63) / 64), 256, 0, L.s>>>(
+                rows - bs, ncols, bs, Ajj + bs, Ajj + (size_t)bs * lda, Ajj + (size_t)bs * lda + bs, lda);
+    }
+    BH_HIP(hipGetLastError());
     if (m > n) {
         trsm_runn(L.h, m - n, n, A, lda, A + n, lda);
         k_absmax_bits<T><<<dim3((m - n + 255) / 256, (n + 63) / 64), 256, 0, L.s>>>(m - n, n, A + n, lda, maxl);
@@ -637,13 +711,22 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     BH_HIP(hipGetLastError());
     unsigned int base = 0;
     const int lda = N;
+    // Block width for the panel starting with `rem` columns left: the panel's
+    // chain of small kernels costs about the same at any height, while the
+    // trailing update it hides behind shrinks with rem^2, so towards the end
+    // the width is halved (down to 256) whenever rem < nb_ratio * width.
+    auto width = [&](int rem) {
+        int nb = LU_NB;
+        while (ctx->nb_ratio > 0 && nb >= 2 * 256 && nb % 64 == 0 && rem < ctx->nb_ratio * nb) nb /= 2;
+        return std::min(nb, rem);
+    };
     // panel 0: nothing to overlap it with
-    const int jb0 = std::min(LU_NB, N);
+    const int jb0 = width(N);
     panel_start(ctx, main, N, jb0, A, lda, ipiv, 0, base);
     bool pivoted = panel_finish(ctx, main, N, jb0, A, lda, ipiv, 0, base);
     BH_HIP(hipEventRecord(ev_panel, main.s));
-    for (int k = 0; k < N; k += LU_NB) {
-        const int jb = std::min(LU_NB, N - k);
+    int k = 0, jb = jb0;
+    while (k < N) {
         T* Akk = A + (size_t)k * lda + k;
         BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));   // panel k factored
         if (pivoted) laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
@@ -652,7 +735,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         T* A12 = A + (size_t)(k + jb) * lda + k;
         if (pivoted) laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
         trsm_llnu(main.h, jb, rest, Akk, lda, A12, lda);
-        const int jb2 = std::min(LU_NB, rest);
+        const int jb2 = width(rest);
         if constexpr (std::is_same<T, float>::value)
             if (split16) split_prepare(ctx, main.s, rest, rest, jb, Akk + jb, lda, A12, lda);
         // next panel's columns first, then hand them to the side stream
@@ -672,6 +755,8 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         if (!overlap) panel_start(ctx, main, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
         pivoted = panel_finish(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
         BH_HIP(hipEventRecord(ev_panel, pl.s));
+        k += jb;
+        jb = jb2;
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     unsigned int flags[3] = {0, 0, 0};
@@ -1296,8 +1381,7 @@ void Solver::init(const Solver* cfg) {
         la_min_rest = cfg->la_min_rest;
         lookahead = cfg->lookahead;
         speculate = cfg->speculate;
-        leaf32 = cfg->leaf32;
-        leaf64 = cfg->leaf64;
+        nb_ratio = cfg->nb_ratio;
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;

@@ -151,8 +151,8 @@ struct Solver {
     hipEvent_t ev_la[3] = {};
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
-    int leaf32 = 32, leaf64 = 32;       // speculative leaf width (32, 64 or 128; FP64 at most 64)
     int nb64 = 512, nb32 = 512, nb_split = 1024;   // LU outer block per precision
+    int nb_ratio = 24;                              // halve the block while columns left < nb_ratio * block
     unsigned int leaf_readers = 0;                  // running target of the speculative-leaf hand-off
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;

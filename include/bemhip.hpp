@@ -131,6 +131,23 @@ public:
     std::vector<double> get_psi() const { std::vector<double> v(N_); state(nullptr, nullptr, v.data(), nullptr); return v; }
     bemhip_solve_stats last_stats() const { bemhip_solve_stats s; ctx_.check(bemhip_sim_last_stats(h_, &s)); return s; }
 
+    // Loss sum_n (R_n - target_n)^2 of the volume-equivalent radius over
+    // target.size() fixed RK4 steps and its gradient with respect to the
+    // current vertices and potential (state left unchanged); see
+    // bemhip_sim_adjoint_radius.
+    double adjoint_radius(double dt, const std::vector<double>& target, int every, std::vector<vec3d>* grad_x,
+                          std::vector<double>* grad_phi, std::vector<double>* radius = nullptr) {
+        double loss = 0.0;
+        if (grad_x) grad_x->assign(N_, vec3d{0.0, 0.0, 0.0});
+        if (grad_phi) grad_phi->assign(N_, 0.0);
+        if (radius) radius->assign(target.size(), 0.0);
+        ctx_.check(bemhip_sim_adjoint_radius(h_, (int32_t)target.size(), dt, target.data(), every, &loss,
+                                             radius ? radius->data() : nullptr,
+                                             grad_x ? (*grad_x)[0].data() : nullptr,
+                                             grad_phi ? grad_phi->data() : nullptr));
+        return loss;
+    }
+
 private:
     static std::vector<uint32_t> narrow(const std::vector<tri3>& trigs, std::size_t n) {
         std::vector<uint32_t> t;
