@@ -305,7 +305,6 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
-        if (const char* v = std::getenv("BEMHIP_NB_RATIO")) ctx->nb_ratio = std::max(0, std::atoi(v));
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));

@@ -182,27 +182,36 @@ __global__ __launch_bounds__(256) void k_laswp(int ncols, T* __restrict__ A, int
 // One 32-column step of the interchange-free factorisation of a panel's top
 // block.  The block column starts at A (its diagonal element) and has m rows
 // and bs <= 32 columns; U12 is the ncols columns to its right (rows 0..bs-1).
-// Every work-group
-//  * factors the bs x bs diagonal block in wave 0 (a row per lane, in
-//    registers, the pivot row broadcast by shuffles) and keeps it in LDS,
-//  * eliminates its 256 rows below the diagonal block (a row per thread,
-//    in registers),
-//  * solves its 256 columns of U12 = L11^-1 A12 (a column per thread),
-// with no inter-work-group synchronisation in the arithmetic.  max |l| over
-// all multipliers goes to *maxl (bits of a non-negative double; NaN sorts
-// above every number).  If max |l| <= 1, partial pivoting with first-maximum
-// ties (Eigen::PartialPivLU, LAPACK) would have chosen the diagonal at every
-// column, i.e. this is exactly its elimination.  Work-group 0 writes the
-// factored diagonal block back only after every work-group has read the
-// original one (monotone counter).
+// Every work-group factors the bs x bs diagonal block in wave 0 (a row per
+// lane, in registers, the pivot row read with v_readlane) and keeps it in
+// LDS; then work-groups [0, prow) eliminate 256 rows each below the diagonal
+// block (a row per thread, in registers) and the others solve 256 columns
+// each of U12 = L11^-1 A12 (a column per thread), with no inter-work-group
+// synchronisation at all.  Multipliers are formed with the reciprocal of the
+// pivot, as LAPACK's getf2 does.  max |l| over all multipliers goes to *maxl
+// (bits of a non-negative double; NaN sorts above every number).  If
+// max |l| <= 1, partial pivoting with first-maximum ties would have chosen
+// the diagonal at every column, i.e. this is exactly its elimination.  The
+// factored diagonal block goes to dscr (NPB x NPB, column-major), never into
+// A, because other work-groups may still be reading A's original one; the
+// next launch on the stream (k_np_update or k_np_diag_store) moves it there.
 constexpr int NPB = 32;
 
+__device__ inline float lane_value(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double lane_value(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned int)(b & 0xffffffffll), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
 template <class T>
-__global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __restrict__ A, int lda,
-                                                 unsigned long long* __restrict__ maxl,
-                                                 unsigned int* __restrict__ readers, unsigned int reader_target,
-                                                 unsigned int* __restrict__ err) {
+__global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, int prow, T* __restrict__ A, int lda,
+                                                 unsigned long long* __restrict__ maxl, T* __restrict__ dscr) {
     __shared__ T top[NPB][NPB + 1];
+    __shared__ T invd[NPB];
     __shared__ unsigned long long red[4];
     const int tid = threadIdx.x, lane = tid & 63;
     for (int e = tid; e < bs * bs; e += 256) {
@@ -210,7 +219,6 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __
         top[i][j] = A[(size_t)j * lda + i];
     }
     __syncthreads();
-    if (tid == 0) __hip_atomic_fetch_add(readers, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     double lmax = 0.0;
     if (tid < 64) {
         const bool row = lane < bs;
@@ -220,9 +228,9 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __
 #pragma unroll
         for (int j = 0; j < NPB; ++j) {
             if (j < bs) {
-                const T piv = __shfl(r[j], j);
+                const T inv = T(1) / lane_value(r[j], j);
                 const bool below = row && lane > j;
-                const T l = r[j] / piv;
+                const T l = r[j] * inv;
                 if (below) {
                     r[j] = l;
                     const double a = fabs((double)l);
@@ -230,9 +238,10 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __
                 }
 #pragma unroll
                 for (int k = j + 1; k < NPB; ++k) {
-                    const T pk = __shfl(r[k], j);
+                    const T pk = lane_value(r[k], j);
                     if (below) r[k] -= l * pk;
                 }
+                if (lane == j) invd[j] = inv;
             }
         }
 #pragma unroll
@@ -240,45 +249,47 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __
             if (row && k < bs) top[lane][k] = r[k];
     }
     __syncthreads();
-    // rows below the diagonal block
-    const int r = bs + blockIdx.x * 256 + tid;
-    if (r < m) {
-        T a[NPB];
+    if ((int)blockIdx.x < prow) {
+        // rows below the diagonal block
+        const int r = bs + blockIdx.x * 256 + tid;
+        if (r < m) {
+            T a[NPB];
 #pragma unroll
-        for (int j = 0; j < NPB; ++j) a[j] = j < bs ? A[(size_t)j * lda + r] : T(0);
+            for (int j = 0; j < NPB; ++j) a[j] = j < bs ? A[(size_t)j * lda + r] : T(0);
 #pragma unroll
-        for (int j = 0; j < NPB; ++j) {
-            if (j < bs) {
-                const T l = a[j] / top[j][j];
-                a[j] = l;
-                const double al = fabs((double)l);
-                if (!(al <= lmax)) lmax = al;
+            for (int j = 0; j < NPB; ++j) {
+                if (j < bs) {
+                    const T l = a[j] * invd[j];
+                    a[j] = l;
+                    const double al = fabs((double)l);
+                    if (!(al <= lmax)) lmax = al;
 #pragma unroll
-                for (int k = j + 1; k < NPB; ++k) a[k] -= l * top[j][k];   // k >= bs: never stored
+                    for (int k = j + 1; k < NPB; ++k) a[k] -= l * top[j][k];   // k >= bs: never stored
+                }
             }
+#pragma unroll
+            for (int j = 0; j < NPB; ++j)
+                if (j < bs) A[(size_t)j * lda + r] = a[j];
         }
+    } else {
+        // U12 = L11^-1 A12
+        const int c = (blockIdx.x - prow) * 256 + tid;
+        if (c < ncols) {
+            T* col = A + (size_t)(bs + c) * lda;
+            T u[NPB];
 #pragma unroll
-        for (int j = 0; j < NPB; ++j)
-            if (j < bs) A[(size_t)j * lda + r] = a[j];
-    }
-    __syncthreads();   // keeps the compiler from overlapping the two register-heavy phases
-    // U12 = L11^-1 A12
-    const int c = blockIdx.x * 256 + tid;
-    if (c < ncols) {
-        T* col = A + (size_t)(bs + c) * lda;
-        T u[NPB];
+            for (int i = 0; i < NPB; ++i) u[i] = i < bs ? col[i] : T(0);
 #pragma unroll
-        for (int i = 0; i < NPB; ++i) u[i] = i < bs ? col[i] : T(0);
+            for (int k = 0; k < NPB; ++k) {   // column-oriented forward substitution; i >= bs: never stored
+                if (k < bs) {
 #pragma unroll
-        for (int k = 0; k < NPB; ++k) {   // column-oriented forward substitution; i >= bs: never stored
-            if (k < bs) {
-#pragma unroll
-                for (int i = k + 1; i < NPB; ++i) u[i] -= top[i][k] * u[k];
+                    for (int i = k + 1; i < NPB; ++i) u[i] -= top[i][k] * u[k];
+                }
             }
-        }
 #pragma unroll
-        for (int i = 0; i < NPB; ++i)
-            if (i < bs) col[i] = u[i];
+            for (int i = 0; i < NPB; ++i)
+                if (i < bs) col[i] = u[i];
+        }
     }
     unsigned long long bits = (unsigned long long)__double_as_longlong(lmax);
 #pragma unroll
@@ -293,30 +304,38 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, T* __
         for (int w = 1; w < 4; ++w) b = red[w] > b ? red[w] : b;
         atomicMax(maxl, b);
     }
-    if (blockIdx.x == 0) {
-        if (tid == 0) {
-            unsigned int spins = 0;
-            while (__hip_atomic_load(readers, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < reader_target) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > (1u << 26)) { __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); break; }
-            }
-        }
-        __syncthreads();
+    if (blockIdx.x == 0)
         for (int e = tid; e < bs * bs; e += 256) {
             const int i = e % bs, j = e / bs;
-            A[(size_t)j * lda + i] = top[i][j];
+            dscr[j * NPB + i] = top[i][j];
         }
+}
+
+// the factored diagonal block from k_np_step's scratch into A
+template <class T>
+__device__ inline void np_diag_store(int bs, const T* __restrict__ dscr, T* __restrict__ Adiag, int lda) {
+    for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+        const int i = e % bs, j = e / bs;
+        Adiag[(size_t)j * lda + i] = dscr[j * NPB + i];
     }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_np_diag_store(int bs, const T* __restrict__ dscr, T* __restrict__ Adiag,
+                                                       int lda) {
+    np_diag_store(bs, dscr, Adiag, lda);
 }
 
 // C (mrows x ncols) -= L (mrows x bs) U (bs x ncols), column-major, bs <= 32:
 // the rank-bs update that follows a k_np_step, one 64 x 64 tile per work-group.
+// Work-group (0, 0) also stores the step's factored diagonal block.
 template <class T>
 __global__ __launch_bounds__(256) void k_np_update(int mrows, int ncols, int bs, const T* __restrict__ L,
-                                                   const T* __restrict__ U, T* __restrict__ C, int lda) {
+                                                   const T* __restrict__ U, T* __restrict__ C, int lda,
+                                                   const T* __restrict__ dscr, T* __restrict__ Adiag) {
     __shared__ T ls[64][NPB + 1];
     __shared__ T us[NPB][64 + 1];
     const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.y == 0) np_diag_store(bs, dscr, Adiag, lda);
     const int ti = blockIdx.x * 64, tj = blockIdx.y * 64;
     for (int e = tid; e < 64 * NPB; e += 256) {
         const int i = e & 63, k = e >> 6;
@@ -392,16 +411,90 @@ static void gemm_minus(rocblas_handle h, int m, int n, int k, const float* A, in
                           ldb, &beta, C, ldc));
 }
 
-// B := B U^-1 for the upper non-unit triangle U (the rows of L below a block)
-static void trsm_runn(rocblas_handle h, int m, int n, const double* U, int ldu, double* B, int ldb) {
-    const double one = 1.0;
-    BH_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                          rocblas_diagonal_non_unit, m, n, &one, U, ldu, B, ldb));
+
+// dst = src for a column-major rows x cols block (coalesced; the runtime's
+// rectangular copy measured 4-5x slower on these shapes)
+template <class T>
+__global__ __launch_bounds__(256) void k_copy2d(int rows, int cols, const T* __restrict__ src, int lds,
+                                                T* __restrict__ dst, int ldd) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const int c0 = blockIdx.y * 16, c1 = min(cols, c0 + 16);
+    for (int c = c0; c < c1; ++c) dst[(size_t)c * ldd + r] = src[(size_t)c * lds + r];
 }
-static void trsm_runn(rocblas_handle h, int m, int n, const float* U, int ldu, float* B, int ldb) {
-    const float one = 1.0f;
-    BH_BLAS(rocblas_strsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                          rocblas_diagonal_non_unit, m, n, &one, U, ldu, B, ldb));
+template <class T>
+static void copy2d(hipStream_t st, int rows, int cols, const T* src, int lds, T* dst, int ldd) {
+    if (rows <= 0 || cols <= 0) return;
+    k_copy2d<T><<<dim3((rows + 255) / 256, (cols + 15) / 16), 256, 0, st>>>(rows, cols, src, lds, dst, ldd);
+    BH_HIP(hipGetLastError());
+}
+
+// A stream with the rocBLAS handle bound to it.
+struct Lane {
+    hipStream_t s;
+    rocblas_handle h;
+};
+
+// ------------------------------------------- triangular solves by inverse --
+// Large blocked triangular solves as one GEMM with the explicit inverse of the
+// (at most NB x NB) triangle: rocBLAS's TRSM runs many small GEMMs per call.
+// The inverse of a unit lower triangle with |l| <= 1 (partial pivoting) or of
+// U11 of such a factorisation is well conditioned for these matrices, and the
+// factor only preconditions the FP64 refinement of the mixed solvers.
+static void trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, const double* A, int lda,
+                  double* W) {
+    BH_BLAS(rocblas_dtrtri(h, uplo, diag, n, A, lda, W, n));
+}
+static void trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, const float* A, int lda,
+                  float* W) {
+    BH_BLAS(rocblas_strtri(h, uplo, diag, n, A, lda, W, n));
+}
+static void gemm_nn(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
+                    double* C, int ldc) {
+    const double one = 1.0, zero = 0.0;
+    BH_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero,
+                          C, ldc));
+}
+static void gemm_nn(rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* B, int ldb,
+                    float* C, int ldc) {
+    const float one = 1.0f, zero = 0.0f;
+    BH_BLAS(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A, lda, B, ldb, &zero,
+                          C, ldc));
+}
+
+// the triangle not referenced by an inverse set to zero (and a unit diagonal
+// written out), so the inverse can enter a full GEMM
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_clean(int n, T* __restrict__ W, int lower, int unit) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= n) return;
+    T& w = W[(size_t)j * n + i];
+    if (lower ? (i < j) : (i > j)) w = T(0);
+    else if (unit && i == j) w = T(1);
+}
+
+// B (n x nrhs) := L^-1 B, L unit lower n x n
+template <class T>
+static void trsm_left_lower_unit(Solver* ctx, Lane L, int n, int nrhs, const T* Lt, int ldl, T* B, int ldb) {
+    T* W = static_cast<T*>(ctx->lu.tri_inv.ensure(sizeof(T) * (size_t)n * n));
+    T* C = static_cast<T*>(ctx->lu.tri_out.ensure(sizeof(T) * (size_t)n * nrhs));
+    trtri(L.h, rocblas_fill_lower, rocblas_diagonal_unit, n, Lt, ldl, W);
+    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, 1, 1);
+    BH_HIP(hipGetLastError());
+    gemm_nn(L.h, n, nrhs, n, W, n, B, ldb, C, n);
+    copy2d(L.s, n, nrhs, C, n, B, ldb);
+}
+
+// B (m x n) := B U^-1, U upper (non-unit) n x n
+template <class T>
+static void trsm_right_upper(Solver* ctx, Lane L, int m, int n, const T* U, int ldu, T* B, int ldb) {
+    T* W = static_cast<T*>(ctx->lu.tri_inv2.ensure(sizeof(T) * (size_t)n * n));
+    T* C = static_cast<T*>(ctx->lu.tri_out2.ensure(sizeof(T) * (size_t)m * n));
+    trtri(L.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, U, ldu, W);
+    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, 0, 0);
+    BH_HIP(hipGetLastError());
+    gemm_nn(L.h, m, n, n, B, ldb, W, n, C, m);
+    copy2d(L.s, m, n, C, m, B, ldb);
 }
 
 // max |x| over a column-major block, merged into *out as the bits of a
@@ -574,13 +667,8 @@ static void collect_gemm_timing(Solver* ctx) {
     ctx->gemm_ev_used = 0;
 }
 
-// ------------------------------------------------------------ factorise --
-// A stream with the rocBLAS handle bound to it.
-struct Lane {
-    hipStream_t s;
-    rocblas_handle h;
-};
 
+// ------------------------------------------------------------ factorise --
 template <class T>
 static void leaf(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base) {
     const int P = (m + PROWS - 1) / PROWS;
@@ -629,22 +717,22 @@ static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipi
 // units from the concurrent trailing-update GEMM.)
 template <class T>
 static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
-    unsigned int* readers = reinterpret_cast<unsigned int*>(maxl + 1);
-    unsigned int* err = readers + 1;
+    T* dscr = static_cast<T*>(ctx->lu.diag.ensure(sizeof(T) * NPB * NPB));
     for (int j0 = 0; j0 < n; j0 += NPB) {
         const int bs = std::min(NPB, n - j0);
         const int rows = n - j0, ncols = n - j0 - bs;
         T* Ajj = A + (size_t)j0 * lda + j0;
-        const int P = std::max(1, std::max((rows - bs + 255) / 256, (ncols + 255) / 256));
-        ctx->leaf_readers += (unsigned int)P;
-        k_np_step<T><<<P, 256, 0, L.s>>>(rows, bs, ncols, Ajj, lda, maxl, readers, ctx->leaf_readers, err);
+        const int prow = (rows - bs + 255) / 256, pcol = (ncols + 255) / 256;
+        k_np_step<T><<<std::max(1, prow + pcol), 256, 0, L.s>>>(rows, bs, ncols, prow, Ajj, lda, maxl, dscr);
         if (rows > bs && ncols > 0)
             k_np_update<T><<<dim3((rows - bs + 63) / 64, (ncols + 63) / 64), 256, 0, L.s>>>(
-                rows - bs, ncols, bs, Ajj + bs, Ajj + (size_t)bs * lda, Ajj + (size_t)bs * lda + bs, lda);
+                rows - bs, ncols, bs, Ajj + bs, Ajj + (size_t)bs * lda, Ajj + (size_t)bs * lda + bs, lda, dscr, Ajj);
+        else
+            k_np_diag_store<T><<<1, 256, 0, L.s>>>(bs, dscr, Ajj, lda);
     }
     BH_HIP(hipGetLastError());
     if (m > n) {
-        trsm_runn(L.h, m - n, n, A, lda, A + n, lda);
+        trsm_right_upper(ctx, L, m - n, n, A, lda, A + n, lda);
         k_absmax_bits<T><<<dim3((m - n + 255) / 256, (n + 63) / 64), 256, 0, L.s>>>(m - n, n, A + n, lda, maxl);
         BH_HIP(hipGetLastError());
     }
@@ -660,14 +748,12 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
                         unsigned int& base) {
     if (!ctx->speculate) { panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
     T* bk = static_cast<T*>(ctx->lu.backup.ensure(sizeof(T) * (size_t)m * n));
-    BH_HIP(hipMemcpy2DAsync(bk, sizeof(T) * m, A, sizeof(T) * lda, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
-    // layout (32-bit words): [0,1] max |l| bits, [2] leaf reader counter,
-    // [3] leaf hand-off error
+    copy2d(L.s, m, n, A, lda, bk, m);
+    // max |l| as the bits of a non-negative double
     unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
-    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 4, 0u);
-    ctx->leaf_readers = 0;
+    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 2, 0u);
     panel_np(ctx, L, m, n, A, lda, maxl);
-    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, 2 * sizeof(double), hipMemcpyDeviceToHost, L.s));
+    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, sizeof(double), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
 }
 
@@ -679,15 +765,13 @@ static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* 
     if (!ctx->speculate) { ctx->panels_pivoted += 1; return true; }
     BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
     const double ml = ctx->host_scal[32];
-    const unsigned int* words = reinterpret_cast<const unsigned int*>(ctx->host_scal + 32);
-    BH_CHECK(words[3] == 0u, BEMHIP_ERR_HIP, "speculative panel: top-block hand-off timed out");
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
         return false;
     }
     const T* bk = ctx->lu.backup.as<T>();
-    BH_HIP(hipMemcpy2DAsync(A, sizeof(T) * lda, bk, sizeof(T) * m, sizeof(T) * m, n, hipMemcpyDeviceToDevice, L.s));
+    copy2d(L.s, m, n, bk, m, A, lda);
     panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base);
     ctx->panels_pivoted += 1;
     return true;
@@ -711,15 +795,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     BH_HIP(hipGetLastError());
     unsigned int base = 0;
     const int lda = N;
-    // Block width for the panel starting with `rem` columns left: the panel's
-    // chain of small kernels costs about the same at any height, while the
-    // trailing update it hides behind shrinks with rem^2, so towards the end
-    // the width is halved (down to 256) whenever rem < nb_ratio * width.
-    auto width = [&](int rem) {
-        int nb = LU_NB;
-        while (ctx->nb_ratio > 0 && nb >= 2 * 256 && nb % 64 == 0 && rem < ctx->nb_ratio * nb) nb /= 2;
-        return std::min(nb, rem);
-    };
+    auto width = [&](int rem) { return std::min(LU_NB, rem); };
     // panel 0: nothing to overlap it with
     const int jb0 = width(N);
     panel_start(ctx, main, N, jb0, A, lda, ipiv, 0, base);
@@ -734,7 +810,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         if (rest <= 0) break;
         T* A12 = A + (size_t)(k + jb) * lda + k;
         if (pivoted) laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
-        trsm_llnu(main.h, jb, rest, Akk, lda, A12, lda);
+        trsm_left_lower_unit(ctx, main, jb, rest, Akk, lda, A12, lda);
         const int jb2 = width(rest);
         if constexpr (std::is_same<T, float>::value)
             if (split16) split_prepare(ctx, main.s, rest, rest, jb, Akk + jb, lda, A12, lda);
@@ -1381,7 +1457,6 @@ void Solver::init(const Solver* cfg) {
         la_min_rest = cfg->la_min_rest;
         lookahead = cfg->lookahead;
         speculate = cfg->speculate;
-        nb_ratio = cfg->nb_ratio;
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;

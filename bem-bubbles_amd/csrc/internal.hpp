@@ -131,7 +131,8 @@ struct DevTopology {
 
 // ------------------------------------------------------------- context --
 struct LuWork {
-    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl;
+    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;
+    DevBuf tri_inv, tri_out, tri_inv2, tri_out2;   // main / side stream triangular inverses and products
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
 };
 
@@ -151,9 +152,7 @@ struct Solver {
     hipEvent_t ev_la[3] = {};
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
-    int nb64 = 512, nb32 = 512, nb_split = 1024;   // LU outer block per precision
-    int nb_ratio = 24;                              // halve the block while columns left < nb_ratio * block
-    unsigned int leaf_readers = 0;                  // running target of the speculative-leaf hand-off
+    int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
     unsigned int trsv_epoch = 0;
