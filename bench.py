@@ -31,6 +31,10 @@ import meshgen  # noqa: E402
 
 METRIC = "ms/RK4 stage (assembly+solve) at N=40962; mixed-prec vs FP64 GPU @1e-8 rel"
 PEAK = {"lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
+# HBM bytes per trailing-update GEMM launch at nu = 64, from two rocprofv3 --pmc
+# runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain 1 GiB copy):
+# profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md
+TRAFFIC_NU64 = {"lu16x3_ir64": 6.50e9}
 
 
 def log(*a):
@@ -203,8 +207,21 @@ def main():
         achieved = gf / (gms * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "kernel": "LU trailing-update GEMM", "achieved": round(achieved, 2),
                            "peak": round(PEAK[args.solver], 1), "unit": "TFLOP/s",
-                           "frac": round(achieved / PEAK[args.solver], 4), "traffic": None,
+                           "frac": round(achieved / PEAK[args.solver], 4),
+                           "traffic": TRAFFIC_NU64.get(args.solver) if args.nu == 64 else None,
                            "launches": acc["gemm_launches"], "avg_launch_ms": round(gms / max(1, acc["gemm_launches"]), 4)}
+
+    # assembly: algorithmic output bytes (G in FP64, plus its FP32 copy for the
+    # mixed solvers) and algorithmic flops (35 per quadrature point, 7 points
+    # per collocation-row x triangle pair) over its device time
+    asm_ms = acc["ms_assemble"] / stages
+    g_bytes = (8.0 + (4.0 if args.solver != "lu64" else 0.0)) * N * N
+    asm_flops = 35.0 * 7.0 * N * M
+    out["assembly"] = {"kernels": "k_entry_prep + k_far + k_self", "ms_per_stage": round(asm_ms, 3),
+                       "achieved_GBps": round(g_bytes / (asm_ms * 1e-3) / 1e9, 1),
+                       "hbm_frac": round(g_bytes / (asm_ms * 1e-3) / 8e12, 4),
+                       "achieved_fp64_TFLOPs": round(asm_flops / (asm_ms * 1e-3) / 1e12, 2),
+                       "fp64_frac": round(asm_flops / (asm_ms * 1e-3) / 78.6e12, 4), "bound": "fp64 valu"}
 
     # plain FP64 path on the same state (timing + accuracy of the mixed psi)
     if not args.no_fp64:
