@@ -188,7 +188,7 @@ struct bemhip_ctx : bemhip::Solver {
     hipEvent_t ev[4] = {};
     // extra solvers for batches of independent problems (created on first use)
     std::vector<std::unique_ptr<bemhip::Solver>> lanes;
-    int max_lanes = 4;
+    int max_lanes = 8;
 };
 
 namespace bemhip {
