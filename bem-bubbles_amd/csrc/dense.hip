@@ -473,28 +473,14 @@ __global__ __launch_bounds__(256) void k_tri_clean(int n, T* __restrict__ W, int
     else if (unit && i == j) w = T(1);
 }
 
-// B (n x nrhs) := L^-1 B, L unit lower n x n
+// W (n x n) = the inverse of A's unit-lower (lower) or non-unit upper triangle,
+// the other triangle zeroed and a unit diagonal written out, ready for a GEMM
 template <class T>
-static void trsm_left_lower_unit(Solver* ctx, Lane L, int n, int nrhs, const T* Lt, int ldl, T* B, int ldb) {
-    T* W = static_cast<T*>(ctx->lu.tri_inv.ensure(sizeof(T) * (size_t)n * n));
-    T* C = static_cast<T*>(ctx->lu.tri_out.ensure(sizeof(T) * (size_t)n * nrhs));
-    trtri(L.h, rocblas_fill_lower, rocblas_diagonal_unit, n, Lt, ldl, W);
-    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, 1, 1);
+static void tri_inverse(Lane L, bool lower, int n, const T* A, int lda, T* W) {
+    trtri(L.h, lower ? rocblas_fill_lower : rocblas_fill_upper,
+          lower ? rocblas_diagonal_unit : rocblas_diagonal_non_unit, n, A, lda, W);
+    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, lower ? 1 : 0, lower ? 1 : 0);
     BH_HIP(hipGetLastError());
-    gemm_nn(L.h, n, nrhs, n, W, n, B, ldb, C, n);
-    copy2d(L.s, n, nrhs, C, n, B, ldb);
-}
-
-// B (m x n) := B U^-1, U upper (non-unit) n x n
-template <class T>
-static void trsm_right_upper(Solver* ctx, Lane L, int m, int n, const T* U, int ldu, T* B, int ldb) {
-    T* W = static_cast<T*>(ctx->lu.tri_inv2.ensure(sizeof(T) * (size_t)n * n));
-    T* C = static_cast<T*>(ctx->lu.tri_out2.ensure(sizeof(T) * (size_t)m * n));
-    trtri(L.h, rocblas_fill_upper, rocblas_diagonal_non_unit, n, U, ldu, W);
-    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, 0, 0);
-    BH_HIP(hipGetLastError());
-    gemm_nn(L.h, m, n, n, B, ldb, W, n, C, m);
-    copy2d(L.s, m, n, C, m, B, ldb);
 }
 
 // max |x| over a column-major block, merged into *out as the bits of a
@@ -542,43 +528,56 @@ __device__ inline int scale_exp(unsigned int bits) {
     frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
     return 12 - e;
 }
+// |L21| <= 1 for every panel that reaches a trailing update (partial pivoting,
+// or a speculative panel accepted because max |l| <= 1): its scale is fixed
+constexpr int SPLIT_EA = 11;
 
-__global__ void k_split_init(unsigned int* __restrict__ mx) {
-    if (threadIdx.x == 0) {
-        mx[0] = 0x3F800000u;   // 1.0f: the bound on |L21|
-        mx[1] = 0u;            // max |U12|, reduced next
-    }
-}
-
-__global__ void k_split_alpha(const unsigned int* __restrict__ mx, float* __restrict__ alpha) {
-    const int ea = scale_exp(mx[0]), eb = scale_exp(mx[1]);
-    alpha[0] = -ldexpf(1.0f, -(ea + eb));
-    alpha[1] = 1.0f;
-}
-
-// Acat (rows x 3K, ld rows) = [hi | hi | lo] of 2^ea A
+// Acat (rows x 3K, ld rows) = [hi | hi | lo] of 2^11 A.  With maxl set, max |a|
+// is merged into it as the bits of a non-negative double (NaN above every
+// number), the check that accepts a speculative panel.
 __global__ __launch_bounds__(256) void k_split_a(int rows, int K, const float* __restrict__ A, int lda,
-                                                 const unsigned int* __restrict__ mx, __half* __restrict__ out) {
+                                                 __half* __restrict__ out, unsigned long long* __restrict__ maxl) {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= rows) return;
-    const int ea = scale_exp(mx[0]);
     const int c0 = blockIdx.y * 64, c1 = min(K, c0 + 64);
-    for (int c = c0; c < c1; ++c) {
-        const float v = ldexpf(A[(size_t)c * lda + r], ea);
-        const __half h = __float2half_rn(v);
-        const __half l = __float2half_rn(v - __half2float(h));
-        out[(size_t)c * rows + r] = h;
-        out[(size_t)(c + K) * rows + r] = h;
-        out[(size_t)(c + 2 * K) * rows + r] = l;
+    unsigned int mb = 0;   // bits of max |a|: non-negative floats (and NaN above them) order as uints
+    if (r < rows)
+        for (int c = c0; c < c1; ++c) {
+            const float a = A[(size_t)c * lda + r];
+            const unsigned int ab = __float_as_uint(fabsf(a));
+            mb = ab > mb ? ab : mb;
+            const float v = ldexpf(a, SPLIT_EA);
+            const __half h = __float2half_rn(v);
+            const __half l = __float2half_rn(v - __half2float(h));
+            out[(size_t)c * rows + r] = h;
+            out[(size_t)(c + K) * rows + r] = h;
+            out[(size_t)(c + 2 * K) * rows + r] = l;
+        }
+    if (maxl == nullptr) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(mb, off);
+        mb = o > mb ? o : mb;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned long long bits = mb > 0x7F800000u ? 0x7FF8000000000000ull
+                                                         : (unsigned long long)__double_as_longlong((double)__uint_as_float(mb));
+        atomicMax(maxl, bits);
     }
 }
 
-// Bcat (3K x cols, ld 3K) = [hi; lo; hi] of 2^eb B (K rows)
+// Bcat (3K x cols, ld 3K) = [hi; lo; hi] of 2^eb B (K rows), eb from max |B|
+// in *mx; work-group (0, 0) also writes the GEMM's alpha = -2^-(11 + eb) and
+// beta = 1 to ab[0..1].
 __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* __restrict__ B, int ldb,
-                                                 const unsigned int* __restrict__ mx, __half* __restrict__ out) {
+                                                 const unsigned int* __restrict__ mx, __half* __restrict__ out,
+                                                 float* __restrict__ ab) {
     const int r = blockIdx.x * 256 + threadIdx.x;
+    const int eb = scale_exp(*mx);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        ab[0] = -ldexpf(1.0f, -(SPLIT_EA + eb));
+        ab[1] = 1.0f;
+    }
     if (r >= K) return;
-    const int eb = scale_exp(mx[1]);
     const size_t K3 = 3 * (size_t)K;
     const int c0 = blockIdx.y * 64, c1 = min(cols, c0 + 64);
     for (int c = c0; c < c1; ++c) {
@@ -591,44 +590,49 @@ __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* _
     }
 }
 
-// fp16 splits of one outer step's L21 (m x k) and U12 (k x n) blocks
-static void split_prepare(Solver* ctx, hipStream_t st, int m, int n, int k, const float* A, int lda, const float* B,
-                          int ldb) {
-    __half* acat = static_cast<__half*>(ctx->split_a.ensure(sizeof(__half) * 3 * (size_t)m * k));
-    __half* bcat = static_cast<__half*>(ctx->split_b.ensure(sizeof(__half) * 3 * (size_t)k * n));
-    unsigned int* mx = static_cast<unsigned int*>(ctx->split_mx.ensure(64));
-    float* alpha = reinterpret_cast<float*>(mx + 4);
-    // |L21| <= 1 for every panel that reaches a trailing update (partial
-    // pivoting, or a speculative panel accepted because max |l| <= 1), so its
-    // scale is fixed by the bound 1.0f instead of a reduction over the block
-    k_split_init<<<1, 64, 0, st>>>(mx);
-    k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + 1);
-    k_split_alpha<<<1, 1, 0, st>>>(mx, alpha);
-    k_split_a<<<dim3((m + 255) / 256, (k + 63) / 64), 256, 0, st>>>(m, k, A, lda, mx, acat);
-    k_split_b<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx, bcat);
+// fp16 splits of a panel's L21 (rows x k) into the step's A buffer (parity p)
+static void split_a(Solver* ctx, hipStream_t st, int rows, int k, const float* A, int lda, int p,
+                    unsigned long long* maxl) {
+    if (rows <= 0) return;
+    __half* acat = ctx->split_a[p].as<__half>();
+    k_split_a<<<dim3((rows + 255) / 256, (k + 63) / 64), 256, 0, st>>>(rows, k, A, lda, acat, maxl);
+    BH_HIP(hipGetLastError());
+}
+
+// fp16 splits of U12 columns [col0, col0 + n) (k x n, ld ldb) with their own
+// scale; `slot` (0: the next panel's columns, 1: the rest) selects the scale
+// word and the alpha pair, whose scale word the caller has zeroed.
+static void split_b(Solver* ctx, hipStream_t st, int k, int n, const float* B, int ldb, int col0, int slot) {
+    if (n <= 0) return;
+    unsigned int* mx = ctx->split_mx.as<unsigned int>();
+    __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
+    float* ab = reinterpret_cast<float*>(mx + 4) + 2 * slot;
+    k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot);
+    k_split_b<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot, bcat, ab);
     BH_HIP(hipGetLastError());
 }
 
 // C[:, col0:col0+n] -= L21 U12[:, col0:col0+n] from the prepared splits
-static void split_gemm(Solver* ctx, rocblas_handle h, int m, int n, int k, int col0, float* C, int ldc) {
-    const __half* acat = ctx->split_a.as<__half>();
+static void split_gemm(Solver* ctx, rocblas_handle h, int m, int n, int k, int col0, float* C, int ldc, int p,
+                       int slot) {
+    const __half* acat = ctx->split_a[p].as<__half>();
     const __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
-    float* alpha = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4);
+    float* ab = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4) + 2 * slot;
     BH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
-    rocblas_status s = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, alpha,
-                                       acat, rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, alpha + 1,
+    rocblas_status s = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, ab,
+                                       acat, rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, ab + 1,
                                        C, rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
                                        rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
     BH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
     BH_BLAS(s);
 }
 
-// timed trailing update C -= A B; with split16 the fp16 splits of A and of
-// the full B block row were prepared by split_prepare and `col0` selects the
-// first column of B used here.
+// timed trailing update C -= A B; with split16 the fp16 splits of A (buffer
+// parity p) and of B's columns col0.. (scale slot `slot`) were prepared by
+// split_a / split_b and A, B are not read here.
 template <class T>
 static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m, int n, int k, const T* A, int lda,
-                            const T* B, int ldb, T* C, int ldc, bool split16, int col0 = 0) {
+                            const T* B, int ldb, T* C, int ldc, bool split16, int col0, int p, int slot) {
     if (m <= 0 || n <= 0 || k <= 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -647,7 +651,7 @@ static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m
         BH_HIP(hipEventRecord(e0, st));
     }
     if constexpr (std::is_same<T, float>::value) {
-        if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc);
+        if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc, p, slot);
         else gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
         gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
@@ -709,14 +713,16 @@ static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipi
 
 // Interchange-free factorisation of an m x n panel: the top n x n block by
 // 32-column steps (k_np_step, then the rank-32 update of the rest of the top
-// block, k_np_update), then every row below it at once, L21 = A21 U11^-1 (one
-// large TRSM on the matrix cores), and max |l| over both parts into *maxl.
-// Without interchanges this is the same elimination as column-by-column
-// Gaussian elimination of the whole panel.  (A single grid-barrier launch for
-// the top block was measured slower: its spinning work-groups take compute
-// units from the concurrent trailing-update GEMM.)
+// block, k_np_update), then every row below it at once, L21 = A21 U11^-1 (the
+// explicit inverse of U11 in one GEMM) written to `l21` ((m - n) x n, ld
+// m - n), not into A: A21 keeps its values until the panel is accepted, so a
+// rejected panel only has to restore its top block.  Without interchanges
+// this is the same elimination as column-by-column Gaussian elimination of the
+// whole panel.  (A single grid-barrier launch for the top block was measured
+// slower: its spinning work-groups take compute units from the concurrent
+// trailing-update GEMM.)
 template <class T>
-static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl) {
+static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl, T* l21) {
     T* dscr = static_cast<T*>(ctx->lu.diag.ensure(sizeof(T) * NPB * NPB));
     for (int j0 = 0; j0 < n; j0 += NPB) {
         const int bs = std::min(NPB, n - j0);
@@ -732,55 +738,97 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
     }
     BH_HIP(hipGetLastError());
     if (m > n) {
-        trsm_right_upper(ctx, L, m - n, n, A, lda, A + n, lda);
-        k_absmax_bits<T><<<dim3((m - n + 255) / 256, (n + 63) / 64), 256, 0, L.s>>>(m - n, n, A + n, lda, maxl);
+        T* W = ctx->lu.winv_u.as<T>();
+        tri_inverse(L, false, n, A, lda, W);
+        gemm_nn(L.h, m - n, n, n, A + n, lda, W, n, l21, m - n);
+    }
+}
+
+// What a panel's lane leaves for the main stream's update of its step: the
+// inverse of the panel's unit-lower diagonal block (U12 = L11^-1 A12 is one
+// GEMM with it) and, for the split-fp16 update, the fp16 splits of L21 (whose
+// pass also merges max |l| into maxl when that is set).  Both alternate
+// between two buffers by step parity p: the side stream makes step k+1's
+// while the main stream still reads step k's.
+template <class T>
+static void panel_products(Solver* ctx, Lane L, int m, int n, const T* A, int lda, const T* l21, int ldl, int p,
+                           bool split16, unsigned long long* maxl) {
+    if (m <= n) return;   // the last panel: no update follows
+    tri_inverse(L, true, n, A, lda, ctx->lu.winv_l[p].as<T>());
+    if constexpr (std::is_same<T, float>::value) {
+        if (split16) {
+            split_a(ctx, L.s, m - n, n, l21, ldl, p, maxl);
+            return;
+        }
+    }
+    if (maxl) {
+        k_absmax_bits<T><<<dim3((m - n + 255) / 256, (n + 63) / 64), 256, 0, L.s>>>(m - n, n, l21, ldl, maxl);
         BH_HIP(hipGetLastError());
     }
 }
 
-// Panel of one outer step.  With speculation on, the panel is saved, factored
-// without interchanges and the largest multiplier is copied to the host; the
-// caller enqueues independent work and then calls panel_finish, which keeps the
-// result when max |l| <= 1 (identical to GEPP's choices) and otherwise
-// restores the panel and factors it with partial pivoting.
+// Panel of one outer step (m x n at A).  With speculation on, the top block is
+// saved, the panel factored without interchanges and the largest multiplier
+// copied to the host; the caller enqueues independent work and then calls
+// panel_finish, which keeps the result when max |l| <= 1 (identical to GEPP's
+// choices) and otherwise restores the top block and factors the panel with
+// partial pivoting.
 template <class T>
-static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
-                        unsigned int& base) {
-    if (!ctx->speculate) { panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base); return; }
-    T* bk = static_cast<T*>(ctx->lu.backup.ensure(sizeof(T) * (size_t)m * n));
-    copy2d(L.s, m, n, A, lda, bk, m);
+static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base,
+                        int p, bool split16) {
+    if (!ctx->speculate) {
+        panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base);
+        panel_products(ctx, L, m, n, A, lda, A + n, lda, p, split16, nullptr);
+        return;
+    }
+    copy2d(L.s, n, n, A, lda, ctx->lu.backup.as<T>(), n);
     // max |l| as the bits of a non-negative double
     unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
     k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 2, 0u);
-    panel_np(ctx, L, m, n, A, lda, maxl);
+    T* l21 = ctx->lu.l21[p].as<T>();
+    panel_np(ctx, L, m, n, A, lda, maxl, l21);
+    panel_products(ctx, L, m, n, A, lda, l21, m - n, p, split16, maxl);
     BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, sizeof(double), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
 }
 
-// returns true when the panel was factored with row interchanges
+// Returns true when the panel was factored with row interchanges.  *l21 and
+// *ldl receive where the step's L21 is: the step's l21 buffer for an accepted
+// speculative panel (the caller copies it into A once the panel is handed
+// over) or A itself.
 template <class T>
-static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff,
-                         unsigned int& base) {
+static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base,
+                         int p, bool split16, const T** l21, int* ldl) {
     ctx->panels += 1;
-    if (!ctx->speculate) { ctx->panels_pivoted += 1; return true; }
+    *l21 = A + n;
+    *ldl = lda;
+    if (!ctx->speculate) {
+        ctx->panels_pivoted += 1;
+        return true;
+    }
     BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
     const double ml = ctx->host_scal[32];
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
+        *l21 = ctx->lu.l21[p].as<T>();
+        *ldl = std::max(1, m - n);
         return false;
     }
-    const T* bk = ctx->lu.backup.as<T>();
-    copy2d(L.s, m, n, bk, m, A, lda);
+    copy2d(L.s, n, n, ctx->lu.backup.as<T>(), n, A, lda);
     panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base);
+    panel_products(ctx, L, m, n, A, lda, A + n, lda, p, split16, nullptr);
     ctx->panels_pivoted += 1;
     return true;
 }
 
-// Blocked right-looking LU with partial pivoting and one block of look-ahead:
-// while the main stream applies step k's update to the trailing matrix, the
-// panel of step k+1 (whose columns were updated first) is factored on the
-// second stream.
+// Blocked right-looking LU with partial pivoting and one block of look-ahead.
+// Per outer step k (panel factored, its L21 and L11^-1 ready): the main stream
+// forms U12 = L11^-1 A12 and updates the next panel's columns first, hands them
+// to the second stream, which factors panel k+1 (and prepares its L11^-1 and
+// L21 splits) while the main stream forms the rest of U12 and applies the
+// rest of the update.  Only the narrow part of step k lies on the chain of
+// dependent work from panel to panel.
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16) {
     // outer block: the split-fp16 update runs at fp16 MFMA speed, so its K must
     // be larger for the C read/write of each update to stay below the compute
@@ -788,9 +836,24 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     BH_CHECK(LU_NB >= LEAF && LU_NB % LEAF == 0, BEMHIP_ERR_ARG, "LU block must be a multiple of 32");
     ctx->lu.panel_ws.ensure(PanelWs::BYTES);
     char* ws = ctx->lu.panel_ws.as<char>();
+    // work buffers at their largest (the first step's) before anything is
+    // enqueued: growing one later would free memory under running kernels
+    const size_t nb = (size_t)std::min(LU_NB, N);
+    for (int p = 0; p < 2; ++p) {
+        ctx->lu.winv_l[p].ensure(sizeof(T) * nb * nb);
+        ctx->lu.l21[p].ensure(sizeof(T) * (size_t)N * nb);
+        if (split16) ctx->split_a[p].ensure(sizeof(__half) * 3 * (size_t)N * nb);
+    }
+    ctx->lu.winv_u.ensure(sizeof(T) * nb * nb);
+    ctx->lu.u12.ensure(sizeof(T) * nb * (size_t)N);
+    ctx->lu.backup.ensure(sizeof(T) * nb * nb);
+    if (split16) {
+        ctx->split_b.ensure(sizeof(__half) * 3 * nb * (size_t)N);
+        ctx->split_mx.ensure(64);
+    }
     const Lane main{ctx->stream, ctx->blas};
     const Lane side = ctx->lookahead ? Lane{ctx->stream2, ctx->blas2} : main;
-    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1];
+    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1], ev_copied = ctx->ev_la[3];
     k_fill_u32<<<1, 64, 0, main.s>>>(reinterpret_cast<unsigned int*>(ws), 16, 0u);
     BH_HIP(hipGetLastError());
     unsigned int base = 0;
@@ -798,24 +861,36 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     auto width = [&](int rem) { return std::min(LU_NB, rem); };
     // panel 0: nothing to overlap it with
     const int jb0 = width(N);
-    panel_start(ctx, main, N, jb0, A, lda, ipiv, 0, base);
-    bool pivoted = panel_finish(ctx, main, N, jb0, A, lda, ipiv, 0, base);
+    int p = 0;
+    const T* l21 = nullptr;
+    int ldl = 1;
+    panel_start(ctx, main, N, jb0, A, lda, ipiv, 0, base, p, split16);
+    bool pivoted = panel_finish(ctx, main, N, jb0, A, lda, ipiv, 0, base, p, split16, &l21, &ldl);
     BH_HIP(hipEventRecord(ev_panel, main.s));
+    if (!pivoted) copy2d(main.s, N - jb0, jb0, l21, ldl, A + jb0, lda);
+    bool side_copy = false;   // an L21 copy was enqueued on the side stream (ev_copied follows the latest)
     int k = 0, jb = jb0;
     while (k < N) {
-        T* Akk = A + (size_t)k * lda + k;
         BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));   // panel k factored
+        // the row swaps below reach into the previous panel's L21
+        if (pivoted && side_copy) BH_HIP(hipStreamWaitEvent(main.s, ev_copied, 0));
         if (pivoted) laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
         const int rest = N - k - jb;
         if (rest <= 0) break;
         T* A12 = A + (size_t)(k + jb) * lda + k;
         if (pivoted) laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
-        trsm_left_lower_unit(ctx, main, jb, rest, Akk, lda, A12, lda);
         const int jb2 = width(rest);
+        const T* W = ctx->lu.winv_l[p].as<T>();
+        T* U = ctx->lu.u12.as<T>();   // U12 = L11^-1 A12, jb x rest, ld jb
+        if (split16) {
+            k_fill_u32<<<1, 64, 0, main.s>>>(ctx->split_mx.as<unsigned int>(), 2, 0u);
+            BH_HIP(hipGetLastError());
+        }
+        // the next panel's columns first, then hand them to the side stream
+        gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
         if constexpr (std::is_same<T, float>::value)
-            if (split16) split_prepare(ctx, main.s, rest, rest, jb, Akk + jb, lda, A12, lda);
-        // next panel's columns first, then hand them to the side stream
-        trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, Akk + jb, lda, A12, lda, A12 + jb, lda, split16, 0);
+            if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
+        trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
         // the panel's small kernels only hide well behind a long update; a
         // short one is cheaper to run first and then factor the panel alone
         const bool overlap = ctx->lookahead && rest - jb2 >= ctx->la_min_rest;
@@ -823,18 +898,41 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         if (overlap) {
             BH_HIP(hipEventRecord(ev_ready, main.s));
             BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-            panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+            panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
         }
-        if (rest > jb2)
-            trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, Akk + jb, lda, A12 + (size_t)jb2 * lda,
-                               lda, A12 + (size_t)jb2 * lda + jb, lda, split16, jb2);
-        if (!overlap) panel_start(ctx, main, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
-        pivoted = panel_finish(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base);
+        if (rest > jb2) {
+            T* Uw = U + (size_t)jb2 * jb;
+            gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+            if constexpr (std::is_same<T, float>::value)
+                if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
+            trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
+                               A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
+        }
+        copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
+        if (!overlap) panel_start(ctx, main, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
+        const T* l21n = nullptr;
+        int ldln = 1;
+        pivoted = panel_finish(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16, &l21n,
+                               &ldln);
         BH_HIP(hipEventRecord(ev_panel, pl.s));
+        // an accepted panel's L21 into the factor, after the hand-over (the
+        // main stream reads it from the step buffers); the next panel's
+        // hand-over event on this lane comes after the copy
+        if (!pivoted) {
+            copy2d(pl.s, rest - jb2, jb2, l21n, ldln, A12 + jb + jb2, lda);
+            if (pl.s != main.s) {
+                BH_HIP(hipEventRecord(ev_copied, pl.s));
+                side_copy = true;
+            }
+        }
         k += jb;
         jb = jb2;
+        p ^= 1;
+        l21 = l21n;
+        ldl = ldln;
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
+    if (side_copy) BH_HIP(hipStreamWaitEvent(main.s, ev_copied, 0));
     unsigned int flags[3] = {0, 0, 0};
     BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, main.s));
     BH_HIP(hipStreamSynchronize(main.s));
@@ -1215,6 +1313,14 @@ __global__ __launch_bounds__(256) void k_mul(int n, const double* __restrict__ a
     if (i < n) out[i] = a[i] * b[i];
 }
 
+// Default refinement tolerance of the mixed solvers: stop once the correction
+// is below IR_TOL |x|.  The error left is about that times the contraction of
+// a sweep, measured at 3e-7 .. 3e-6 on sphere systems of N = 10242 and 40962,
+// so two sweeps leave ~1e-16.  The FP64 residual itself is only accurate to
+// ~1e-11 .. 1e-10 of |x| at N = 40962, so a tolerance at that floor (1e-10)
+// costs a third sweep at random.
+constexpr double IR_TOL = 1e-9;
+
 // Eigen 3.4 bicgstab() with DiagonalPreconditioner, zero initial guess
 static void bicgstab(Solver* ctx, int N, const double* A, const double* rhs, double* x, double tol, int maxit,
                      bemhip_solve_stats* st) {
@@ -1327,9 +1433,7 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         int* perm = make_perm(ctx, N, ipiv);
         if (ctx->trsv_persistent) lu_prepare_solve<float>(ctx, N, G32);
         BH_HIP(hipEventRecord(e1, sm));
-        // stop once the correction is below tol |x|: the error left is about
-        // tol times the contraction factor of a sweep (1e-4 .. 1e-3 here)
-        if (tol <= 0.0) tol = 1e-10;
+        if (tol <= 0.0) tol = IR_TOL;
         if (maxit <= 0) maxit = 30;
         BH_HIP(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, sm));
         lu_solve<float>(ctx, N, G32, perm, x);
@@ -1422,7 +1526,7 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
     }
     BH_CHECK(solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64, BEMHIP_ERR_ARG,
              "transposed solve needs an LU-based solver");
-    if (tol <= 0.0) tol = 1e-10;
+    if (tol <= 0.0) tol = IR_TOL;
     if (maxit <= 0) maxit = 30;
     lu_solve_t32(ctx, N, G32, perm, b, x);
     double* r = static_cast<double*>(ctx->vec[6].ensure(sizeof(double) * N));

@@ -132,7 +132,9 @@ struct DevTopology {
 // ------------------------------------------------------------- context --
 struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;
-    DevBuf tri_inv, tri_out, tri_inv2, tri_out2;   // main / side stream triangular inverses and products
+    // per outer step: inverse of the panel's unit-lower diagonal block and its
+    // L21 (two of each, alternating by step parity), inverse of U11, U12
+    DevBuf winv_l[2], l21[2], winv_u, u12;
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
 };
 
@@ -149,7 +151,7 @@ struct Solver {
     // many columns besides the next panel's (0: always; measured best at
     // N = 40962 in FP64 and mixed precision)
     int la_min_rest = 0;
-    hipEvent_t ev_la[3] = {};
+    hipEvent_t ev_la[4] = {};   // LU: next panel's columns ready, panel done, max |l| copied, L21 copied
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
     int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
@@ -159,7 +161,7 @@ struct Solver {
     // workspace
     DevBuf vec[12], scal, gemv_part;
     LuWork lu;
-    DevBuf split_a, split_b, split_mx, perm;
+    DevBuf split_a[2], split_b, split_mx, perm;   // split_a alternates by LU step parity
     std::vector<int> h_ipiv, h_perm;
     double* host_scal = nullptr;  // pinned
     // instrumentation of trailing updates

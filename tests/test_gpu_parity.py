@@ -57,6 +57,25 @@ def test_lu64_with_pivoting(ctx):
         assert rel(xm, np.linalg.solve(A, b)) < 1e-10 and stm["panels_pivoted"] >= 1
 
 
+def test_lu_accepted_then_pivoted_panels(ctx):
+    """Columns [0, 1536) diagonally dominant, the rest random: the first panels
+    are accepted without interchanges and later ones fall back to partial
+    pivoting, whose row swaps reach into L21 blocks that the second stream
+    copied into the factor."""
+    say("lu accepted then pivoted N=3000")
+    rng = np.random.default_rng(11)
+    n = 3000
+    A = rng.standard_normal((n, n))
+    A[np.arange(1536), np.arange(1536)] += 100.0
+    b = rng.standard_normal(n)
+    ref = np.linalg.solve(A, b)
+    for solver in ("lu64", "lu32_ir64", "lu16x3_ir64"):
+        x, st = ctx.solve(A, b, solver)
+        say(f"  {solver}: panels {st['panels']} pivoted {st['panels_pivoted']}")
+        assert rel(x, ref) < 1e-10
+        assert 0 < st["panels_pivoted"] < st["panels"]
+
+
 def test_speculative_panels_equal_gepp(oracle_lib, monkeypatch):
     """On BEM matrices partial pivoting keeps the diagonal: the interchange-free
     panels are accepted and give GEPP's factorisation (checked against a
