@@ -302,6 +302,8 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (la && la[0] == '0') ctx->lookahead = false;
         const char* sp = std::getenv("BEMHIP_LU_SPECULATE");
         if (sp && sp[0] == '0') ctx->speculate = false;
+        const char* nt = std::getenv("BEMHIP_NP_TOP");
+        if (nt && nt[0] == '0') ctx->np_top = false;
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);

@@ -311,6 +311,391 @@ __global__ __launch_bounds__(256) void k_np_step(int m, int bs, int ncols, int p
         }
 }
 
+// One launch factors a panel's whole top block (n x n) without interchanges:
+// one work-group per 32-column stripe and one thread per row of the block.
+// Stripes are taken in dispatch order through a ticket, so every stripe a
+// work-group waits for belongs to a work-group that is already running (no
+// co-residency assumption).  Work-group s applies the updates of stripes
+// 0..s-1 in order as they are published: the 32 U rows of its stripe by
+// forward substitution with stripe j's unit-lower diagonal block, then the
+// rank-32 update of its rows below them; then it factors its own diagonal
+// block (one wave, rows in registers, pivot row by v_readlane, reciprocal
+// pivots as getf2) and the rows below it, merges max |l| into sync[0..1] and
+// publishes the stripe (flag = epoch).  Published values go through
+// agent-scope atomic stores and loads (coherent across the chip's L2 caches)
+// with a release / acquire pair on the flag.  A waiting work-group gives up
+// after ~0.2 s and raises sync[3], so the host reports an error instead of
+// hanging.  One launch per panel replaces 2 n / 32 dependent launches, each
+// of which had to find compute-unit slots behind the concurrent
+// trailing-update GEMM.
+constexpr int TOP_STRIPES = 64;
+constexpr int TOP_SYNC_WORDS = 16 + TOP_STRIPES;   // [0,1] max |l| bits, [2] ticket, [3] error, [16..] flags
+template <class T> constexpr int top_max_rows() { return sizeof(T) == 8 ? 768 : 1024; }
+// FP32: 32-row blocks on the matrix cores, up to TOP_RBW per wave (8 waves at
+// most, two per SIMD, so that a thread may hold 256 registers); FP64: a thread
+// per row.  (Giving wave 0 the diagonal row block alone, so that it could
+// factor it while the others still update, measured slower: sharing its SIMD
+// with an updating wave doubled the factorisation's latency.)
+constexpr int TOP_RBW = 4;
+template <class T> constexpr int top_max_threads() { return sizeof(T) == 8 ? 768 : 512; }
+template <class T> inline int top_threads(int n) {
+    if (sizeof(T) == 8) return std::max(64, (n + 63) / 64 * 64);
+    return 64 * std::max(1, ((n + NPB - 1) / NPB + TOP_RBW - 1) / TOP_RBW);
+}
+static_assert((1024 / NPB + TOP_RBW - 1) / TOP_RBW <= 8, "FP32 top block of 1024 rows needs more than 8 waves");
+
+template <class T>
+__device__ inline T ld_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ inline void st_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// An SGPR zero the compiler can neither fold nor hoist: LDS tables read
+// through `base + zero` keep one base register, where constant addresses are
+// materialised one register each.
+__device__ inline int opaque_zero() {
+    int z;
+    __asm__ volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+// Ends one step of a long unrolled multiply-add sweep: the running values are
+// pinned to registers here and LDS reads may not move across.  Without it the
+// compiler issues all the sweep's LDS reads first and sinks the arithmetic to
+// the end, spilling hundreds of registers.
+template <class T, int N>
+__device__ inline void step_fence(T (&v)[N]) {
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int i = 0; i < N; ++i) __asm__ volatile("" : "+v"(v[i]));
+}
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// FP32 32-row blocks on the matrix cores: acc (32 x 32, v_mfma_f32_32x32x2f32
+// layout) += W^T X^T over k = 0..31, i.e. acc[c][i] = sum_k W[k][c] X[i][k],
+// where the lane of row i (i = lane % 32) supplies X[i][k] for k = 2 kk +
+// lane / 32 in xv[kk] and W is a 32 x 32 LDS table.  Lane l, register q of the
+// result holds column c = 8 (q / 4) + 4 (l / 32) + q % 4 of row i.
+__device__ inline floatx16 mfma_rows32(const float* __restrict__ W, const float (&xv)[16], int lane) {
+    floatx16 acc = {};
+    const int c = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(W[(2 * kk + kh) * NPB + c], xv[kk], acc, 0, 0, 0);
+    return acc;
+}
+
+// Unit-lower inverse (lane = column c < 32 of L^-1, forward substitution) of
+// the 32 x 32 LU block in Lb (row stride NPB + 1), written transposed to
+// W[c * NPB + i] = (L^-1)[i][c] through `st`.
+template <class T, class St>
+__device__ inline void unit_lower_inverse(const T* __restrict__ Lb, int c, St st) {
+    T x[NPB];
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) {
+        T v = i == c ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < i; ++k) v -= Lb[i * (NPB + 1) + k] * x[k];
+        x[i] = v;
+        step_fence(x);
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) st(c * NPB + i, x[i]);
+}
+// Upper inverse (lane = column c < 32 of U^-1, back substitution with the
+// reciprocal pivots), written to W[i * NPB + c] = (U^-1)[i][c].
+template <class T>
+__device__ inline void upper_inverse(const T* __restrict__ Lb, const T* __restrict__ invd, int c, T* __restrict__ W) {
+    T x[NPB];
+#pragma unroll
+    for (int i = NPB - 1; i >= 0; --i) {
+        T v = i == c ? T(1) : T(0);
+#pragma unroll
+        for (int k = i + 1; k < NPB; ++k) v -= Lb[i * (NPB + 1) + k] * x[k];
+        x[i] = v * invd[i];
+        step_fence(x);
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) W[i * NPB + c] = x[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __restrict__ A, int lda,
+                                                                  unsigned int* __restrict__ sync,
+                                                                  unsigned int epoch, T* __restrict__ linv) {
+    constexpr int MAXW = top_max_threads<T>() / 64;
+    // FP32 runs its block products on the matrix cores (32-row blocks, up to
+    // TOP_RBW per wave) and publishes L^-1 of every diagonal block, so that a
+    // stripe's U rows are one product too; FP64 keeps one row per thread on
+    // the vector units and forward substitution
+    constexpr bool MF = sizeof(T) == 4;
+    __shared__ T Ld[NPB][NPB + 1];   // stripe j's unit-lower diagonal block (FP64) / own factored block
+    __shared__ T Us[NPB][NPB];       // this stripe's 32 U rows of step j
+    __shared__ T Wt[NPB * NPB];      // FP32: published L^-1 of stripe j (transposed) / own U^-1
+    __shared__ T invd[NPB];
+    __shared__ unsigned long long red[MAXW];
+    __shared__ int s_stripe, s_go;
+    unsigned int* ticket = sync + 2;
+    unsigned int* err = sync + 3;
+    unsigned int* flags = sync + 16;
+    const int tid = threadIdx.x, lane = tid & 63, nt = blockDim.x;
+    const int wv = tid >> 6, nw = nt >> 6;
+    if (tid == 0) s_stripe = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int s = s_stripe;
+    const int c0 = s * NPB, w = min(NPB, n - c0);
+    T* S = A + (size_t)c0 * lda;   // this stripe: columns c0 .. c0 + w - 1, all n rows
+    const int r = tid;              // FP64: this thread's row
+    // FP32 row blocks: slot h of a wave holds row block rbk(h) (or none: -1)
+    auto rbk = [&](int h) -> int {
+        const int rb = wv + h * nw;
+        return rb * NPB < n ? rb : -1;
+    };
+    // own diagonal block (rows c0 .., w x w), factored by wave 0 in registers
+    // (pivot row by v_readlane); rows at or above the pivot get a zero
+    // multiplier instead of a branch.  Ld and invd are zero beyond w.
+    // max |l| as the bits of a non-negative double: such doubles (and NaN
+    // above them) order as unsigned integers; exact for FP32 and FP64 alike
+    unsigned long long lbits = 0;
+    auto factor_diag = [&]() {
+        const bool row = lane < w;
+        T* Sd = S;
+        T rw[NPB];
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) rw[k] = (row && k < w) ? Sd[(size_t)k * lda + c0 + lane] : T(0);
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            if (j < w) {
+                const T piv = lane_value(rw[j], j);
+                T inv;
+                if constexpr (MF) inv = __builtin_amdgcn_rcpf(piv);
+                else inv = T(1) / piv;
+                const bool below = row && lane > j;
+                const T l = below ? rw[j] * inv : T(0);
+                if (below) rw[j] = l;
+                const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)l));
+                lbits = lb > lbits ? lb : lbits;
+                T pk[NPB];
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
+                if (lane == j) invd[j] = inv;
+            } else if (lane == j) {
+                invd[j] = T(0);
+            }
+        }
+        if (lane < NPB) {
+#pragma unroll
+            for (int k = 0; k < NPB; ++k) {
+                Ld[lane][k] = (row && k < w) ? rw[k] : T(0);
+                if (row && k < w) st_agent(&Sd[(size_t)k * lda + c0 + lane], rw[k]);
+            }
+        }
+    };
+    for (int j = 0; j < s; ++j) {
+        const int r0 = j * NPB;
+        if (tid == 0) {
+            int go = 1;
+            unsigned int spins = 0;
+            while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    go = 0;
+                    break;
+                }
+            }
+            // one acquire for the work-group (the barrier below extends it)
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;   // error raised: results invalid, every work-group leaves
+        const T* Lj = A + (size_t)r0 * lda;   // stripe j (published)
+        // FP32: the multipliers of stripe j for this wave's row blocks, loaded
+        // now so that their latency overlaps the U rows
+        float xb[TOP_RBW][16];
+        if constexpr (MF) {
+#pragma unroll
+            for (int h = 0; h < TOP_RBW; ++h) {
+                const int rb = rbk(h), i = min(max(rb, 0) * NPB + (lane & 31), n - 1);
+                const bool act = rb > j;
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk)
+                    xb[h][kk] = act ? ld_agent(reinterpret_cast<const float*>(Lj) + (size_t)(2 * kk + (lane >> 5)) * lda + i)
+                                    : 0.0f;
+            }
+        }
+        for (int e = tid; e < NPB * NPB; e += nt) {
+            const int i = e & (NPB - 1), k = e / NPB;
+            if constexpr (MF) Wt[e] = ld_agent(linv + (size_t)j * NPB * NPB + e);
+            else Ld[i][k] = ld_agent(&Lj[(size_t)k * lda + r0 + i]);
+            Us[i][k] = k < w ? S[(size_t)k * lda + r0 + i] : T(0);
+        }
+        __syncthreads();
+        // U rows r0.. of this stripe = L_jj^-1 A[r0.., stripe]
+        if constexpr (MF) {
+            if (wv == 0) {
+                // acc[i][c] = sum_k (L^-1)[i][k] A[k][c]: lane c supplies A[k][c]
+                float xv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) xv[kk] = reinterpret_cast<const float*>(&Us[0][0])[(2 * kk + (lane >> 5)) * NPB + (lane & 31)];
+                const floatx16 acc = mfma_rows32(reinterpret_cast<const float*>(Wt), xv, lane);
+                const int c = lane & 31;
+                float* Sf = reinterpret_cast<float*>(S);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int i = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    if (c < w) {
+                        reinterpret_cast<float*>(&Us[0][0])[i * NPB + c] = acc[q];
+                        Sf[(size_t)c * lda + r0 + i] = acc[q];
+                    }
+                }
+            }
+        } else if (tid < w) {
+            // a column per lane of wave 0
+            T u[NPB];
+#pragma unroll
+            for (int i = 0; i < NPB; ++i) u[i] = Us[i][tid];
+            const T* Lb = &Ld[0][0] + opaque_zero();
+#pragma unroll
+            for (int k = 0; k < NPB; ++k) {
+#pragma unroll
+                for (int i = k + 1; i < NPB; ++i) u[i] -= Lb[i * (NPB + 1) + k] * u[k];
+                step_fence(u);
+            }
+#pragma unroll
+            for (int i = 0; i < NPB; ++i) {
+                Us[i][tid] = u[i];
+                S[(size_t)tid * lda + r0 + i] = u[i];
+            }
+        }
+        __syncthreads();
+        // rank-32 update of this stripe's rows below: a[r, :] -= L_j[r, :] U
+        if constexpr (MF) {
+#pragma unroll
+            for (int h = 0; h < TOP_RBW; ++h) {
+                const int rb = rbk(h);
+                if (rb > j) {
+                    const floatx16 acc = mfma_rows32(reinterpret_cast<const float*>(&Us[0][0]), xb[h], lane);
+                    const int i = rb * NPB + (lane & 31);
+                    float* Sf = reinterpret_cast<float*>(S) + opaque_zero();   // not hoisted out of the step loop
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const int c = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                        if (i < n && c < w) Sf[(size_t)c * lda + i] -= acc[q];
+                    }
+                }
+                __asm__ volatile("" ::: "memory");   // one row block at a time (registers)
+            }
+        } else if (r >= r0 + NPB && r < n) {
+            // opaque bases: the stripe's 32 column addresses would otherwise be
+            // hoisted out of the step loop and held in 64 registers throughout
+            const T* Ub = &Us[0][0] + opaque_zero();
+            T l[NPB];
+#pragma unroll
+            for (int k = 0; k < NPB; ++k) l[k] = ld_agent(Lj + (size_t)k * lda + r);
+            // in two passes of 16 columns (registers)
+            constexpr int CB = 16;
+#pragma unroll
+            for (int cb = 0; cb < NPB; cb += CB) {
+                T* So = S + opaque_zero();
+                T a[CB];
+#pragma unroll
+                for (int q = 0; q < CB; ++q) {
+                    const T v = (So + (size_t)min(cb + q, w - 1) * lda)[r];   // clamped: in bounds for a narrow stripe
+                    a[q] = cb + q < w ? v : T(0);
+                }
+#pragma unroll
+                for (int k = 0; k < NPB; ++k) {
+#pragma unroll
+                    for (int q = 0; q < CB; ++q) a[q] -= l[k] * Ub[k * NPB + cb + q];
+                    step_fence(a);
+                }
+#pragma unroll
+                for (int q = 0; q < CB; ++q)
+                    if (cb + q < w) (So + (size_t)(cb + q) * lda)[r] = a[q];
+            }
+        }
+        __syncthreads();   // Ld / Us / Wt are refilled by the next step
+    }
+    if (tid < 64) factor_diag();
+    __syncthreads();
+    if constexpr (MF) {
+        // L^-1 of the diagonal block (published for the later stripes' U rows)
+        // and U^-1 (for the rows below, L = A U^-1), by two waves at once
+        const bool below = c0 + NPB < n;
+        const float* Lb = reinterpret_cast<const float*>(&Ld[0][0]) + opaque_zero();
+        if (wv == (1 % nw) && lane < NPB && w == NPB)
+            unit_lower_inverse(Lb, lane, [&](int e, float v) { st_agent(reinterpret_cast<float*>(linv) + (size_t)s * NPB * NPB + e, v); });
+        if (wv == (2 % nw) && lane < NPB && below)
+            upper_inverse(Lb, reinterpret_cast<const float*>(invd), lane, reinterpret_cast<float*>(Wt));
+        __syncthreads();
+        // rows below the diagonal block on the matrix cores
+        float* Sf = reinterpret_cast<float*>(S);
+#pragma unroll
+        for (int h = 0; h < TOP_RBW; ++h) {
+            const int rb = rbk(h);
+            if (below && rb > s) {
+                const int i = rb * NPB + (lane & 31), ic = min(i, n - 1);
+                float xv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) xv[kk] = Sf[(size_t)(2 * kk + (lane >> 5)) * lda + ic];
+                const floatx16 acc = mfma_rows32(reinterpret_cast<const float*>(Wt), xv, lane);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int c = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    if (i < n) {
+                        const float lq = acc[q];
+                        st_agent(Sf + (size_t)c * lda + i, lq);
+                        const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)lq));
+                        lbits = lb > lbits ? lb : lbits;
+                    }
+                }
+            }
+        }
+    } else if (r >= c0 + w && r < n) {
+        // rows below the diagonal block, a row per thread
+        const T* Lb = &Ld[0][0] + opaque_zero();
+        T a[NPB];
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            const T v = (S + (size_t)min(j, w - 1) * lda)[r];
+            a[j] = j < w ? v : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            const T l = a[j] * invd[j];
+            a[j] = l;
+            const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)l));
+            lbits = lb > lbits ? lb : lbits;
+#pragma unroll
+            for (int k = j + 1; k < NPB; ++k) a[k] -= l * Lb[j * (NPB + 1) + k];
+            step_fence(a);
+        }
+#pragma unroll
+        for (int j = 0; j < NPB; ++j)
+            if (j < w) st_agent(S + (size_t)j * lda + r, a[j]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(lbits, off);
+        lbits = o > lbits ? o : lbits;
+    }
+    if (lane == 0) red[wv] = lbits;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long bits = red[0];
+        for (int q = 1; q < nw; ++q) bits = red[q] > bits ? red[q] : bits;
+        atomicMax(reinterpret_cast<unsigned long long*>(sync), bits);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // the factored diagonal block from k_np_step's scratch into A
 template <class T>
 __device__ inline void np_diag_store(int bs, const T* __restrict__ dscr, T* __restrict__ Adiag, int lda) {
@@ -722,8 +1107,31 @@ static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipi
 // slower: its spinning work-groups take compute units from the concurrent
 // trailing-update GEMM.)
 template <class T>
-static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned long long* maxl, T* l21) {
-    T* dscr = static_cast<T*>(ctx->lu.diag.ensure(sizeof(T) * NPB * NPB));
+static void np_steps(Lane L, int n, T* A, int lda, unsigned long long* maxl, T* dscr);
+
+template <class T>
+static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned int* sync, T* l21) {
+    unsigned long long* maxl = reinterpret_cast<unsigned long long*>(sync);
+    const int stripes = (n + NPB - 1) / NPB;
+    if (ctx->np_top && stripes <= TOP_STRIPES && n <= top_max_rows<T>()) {
+        T* linv = ctx->lu.diag.as<T>();
+        k_np_top<T><<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, linv);
+    } else {
+        T* dscr = ctx->lu.diag.as<T>();
+        np_steps(L, n, A, lda, maxl, dscr);
+    }
+    BH_HIP(hipGetLastError());
+    if (m > n) {
+        T* W = ctx->lu.winv_u.as<T>();
+        tri_inverse(L, false, n, A, lda, W);
+        gemm_nn(L.h, m - n, n, n, A + n, lda, W, n, l21, m - n);
+    }
+}
+
+// the top block by 32-column steps: k_np_step, then the rank-32 update of the
+// rest of the top block (k_np_update)
+template <class T>
+static void np_steps(Lane L, int n, T* A, int lda, unsigned long long* maxl, T* dscr) {
     for (int j0 = 0; j0 < n; j0 += NPB) {
         const int bs = std::min(NPB, n - j0);
         const int rows = n - j0, ncols = n - j0 - bs;
@@ -735,12 +1143,6 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
                 rows - bs, ncols, bs, Ajj + bs, Ajj + (size_t)bs * lda, Ajj + (size_t)bs * lda + bs, lda, dscr, Ajj);
         else
             k_np_diag_store<T><<<1, 256, 0, L.s>>>(bs, dscr, Ajj, lda);
-    }
-    BH_HIP(hipGetLastError());
-    if (m > n) {
-        T* W = ctx->lu.winv_u.as<T>();
-        tri_inverse(L, false, n, A, lda, W);
-        gemm_nn(L.h, m - n, n, n, A + n, lda, W, n, l21, m - n);
     }
 }
 
@@ -782,13 +1184,13 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
         return;
     }
     copy2d(L.s, n, n, A, lda, ctx->lu.backup.as<T>(), n);
-    // max |l| as the bits of a non-negative double
-    unsigned long long* maxl = static_cast<unsigned long long*>(ctx->lu.maxl.ensure(64));
-    k_fill_u32<<<1, 64, 0, L.s>>>(reinterpret_cast<unsigned int*>(maxl), 2, 0u);
+    // sync words: max |l| (bits of a non-negative double), ticket, error, flags
+    unsigned int* sync = ctx->lu.maxl.as<unsigned int>();
+    k_fill_u32<<<1, 64, 0, L.s>>>(sync, 4, 0u);
     T* l21 = ctx->lu.l21[p].as<T>();
-    panel_np(ctx, L, m, n, A, lda, maxl, l21);
-    panel_products(ctx, L, m, n, A, lda, l21, m - n, p, split16, maxl);
-    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, maxl, sizeof(double), hipMemcpyDeviceToHost, L.s));
+    panel_np(ctx, L, m, n, A, lda, sync, l21);
+    panel_products(ctx, L, m, n, A, lda, l21, m - n, p, split16, reinterpret_cast<unsigned long long*>(sync));
+    BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, sync, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
 }
 
@@ -808,6 +1210,8 @@ static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* 
     }
     BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
     const double ml = ctx->host_scal[32];
+    BH_CHECK(reinterpret_cast<const unsigned int*>(ctx->host_scal + 33)[1] == 0u, BEMHIP_ERR_HIP,
+             "panel top-block factorisation gave up waiting");
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
@@ -847,6 +1251,16 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     ctx->lu.winv_u.ensure(sizeof(T) * nb * nb);
     ctx->lu.u12.ensure(sizeof(T) * nb * (size_t)N);
     ctx->lu.backup.ensure(sizeof(T) * nb * nb);
+    // published diagonal-block inverses (persistent top-block launches) or the
+    // step kernels' diagonal block, at the largest size for either precision
+    ctx->lu.diag.ensure(sizeof(double) * NPB * NPB * TOP_STRIPES);
+    if (ctx->lu.maxl.bytes < sizeof(unsigned int) * TOP_SYNC_WORDS) {
+        // flags start below every publication stamp
+        ctx->lu.maxl.ensure(sizeof(unsigned int) * TOP_SYNC_WORDS);
+        k_fill_u32<<<1, 128, 0, ctx->stream>>>(ctx->lu.maxl.as<unsigned int>(), TOP_SYNC_WORDS, 0u);
+        BH_HIP(hipGetLastError());
+        ctx->np_epoch = 0;
+    }
     if (split16) {
         ctx->split_b.ensure(sizeof(__half) * 3 * nb * (size_t)N);
         ctx->split_mx.ensure(64);
@@ -895,11 +1309,10 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         // short one is cheaper to run first and then factor the panel alone
         const bool overlap = ctx->lookahead && rest - jb2 >= ctx->la_min_rest;
         const Lane pl = overlap ? side : main;
-        if (overlap) {
-            BH_HIP(hipEventRecord(ev_ready, main.s));
-            BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-            panel_start(ctx, side, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
-        }
+        if (overlap) BH_HIP(hipEventRecord(ev_ready, main.s));
+        // the rest of the step is enqueued before the next panel's ~70 small
+        // launches: enqueued after them, it would reach the device only once
+        // the host had issued them all, and the main stream would idle meanwhile
         if (rest > jb2) {
             T* Uw = U + (size_t)jb2 * jb;
             gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
@@ -909,7 +1322,8 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
                                A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
         }
         copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
-        if (!overlap) panel_start(ctx, main, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
+        if (overlap) BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+        panel_start(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
         const T* l21n = nullptr;
         int ldln = 1;
         pivoted = panel_finish(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16, &l21n,
@@ -1561,6 +1975,7 @@ void Solver::init(const Solver* cfg) {
         la_min_rest = cfg->la_min_rest;
         lookahead = cfg->lookahead;
         speculate = cfg->speculate;
+        np_top = cfg->np_top;
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;

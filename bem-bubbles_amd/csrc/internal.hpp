@@ -131,7 +131,7 @@ struct DevTopology {
 
 // ------------------------------------------------------------- context --
 struct LuWork {
-    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;
+    DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
     // per outer step: inverse of the panel's unit-lower diagonal block and its
     // L21 (two of each, alternating by step parity), inverse of U11, U12
     DevBuf winv_l[2], l21[2], winv_u, u12;
@@ -154,6 +154,8 @@ struct Solver {
     hipEvent_t ev_la[4] = {};   // LU: next panel's columns ready, panel done, max |l| copied, L21 copied
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
+    bool np_top = true;                 // a panel's top block in one persistent launch (else 32-column steps)
+    unsigned int np_epoch = 0;          // publication stamp of the persistent top-block launches
     int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;

@@ -94,6 +94,14 @@ def test_speculative_panels_equal_gepp(oracle_lib, monkeypatch):
     assert st1["panels_pivoted"] == 0 and st2["panels_pivoted"] == st2["panels"] > 0
     assert rel(x1, x2) < 1e-12
     assert rel(x1, oracle_lib.solve(G, b)) < 1e-10
+    # the top blocks by 32-column step launches instead of one persistent launch
+    monkeypatch.setenv("BEMHIP_LU_SPECULATE", "1")
+    monkeypatch.setenv("BEMHIP_NP_TOP", "0")
+    c3 = bemhip.Context(0)
+    for solver, tol in (("lu64", 1e-12), ("lu16x3_ir64", 1e-10)):
+        x3, st3 = c3.solve(G, b, solver)
+        assert st3["panels_pivoted"] == 0 and rel(x3, x1) < tol
+    c3.close()
 
 
 @pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu32_ir64", 1e-8), ("lu16x3_ir64", 1e-8),
