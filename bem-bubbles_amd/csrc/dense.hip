@@ -421,6 +421,12 @@ __device__ inline void upper_inverse(const T* __restrict__ Lb, const T* __restri
     for (int i = 0; i < NPB; ++i) W[i * NPB + c] = x[i];
 }
 
+// FP32 row block in slot h of wave wv (nw waves), or -1 past the block
+__device__ inline int top_rb(int wv, int h, int nw, int n) {
+    const int rb = wv + h * nw;
+    return rb * NPB < n ? rb : -1;
+}
+
 template <class T>
 __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __restrict__ A, int lda,
                                                                   unsigned int* __restrict__ sync,
@@ -448,53 +454,9 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
     const int c0 = s * NPB, w = min(NPB, n - c0);
     T* S = A + (size_t)c0 * lda;   // this stripe: columns c0 .. c0 + w - 1, all n rows
     const int r = tid;              // FP64: this thread's row
-    // FP32 row blocks: slot h of a wave holds row block rbk(h) (or none: -1)
-    auto rbk = [&](int h) -> int {
-        const int rb = wv + h * nw;
-        return rb * NPB < n ? rb : -1;
-    };
-    // own diagonal block (rows c0 .., w x w), factored by wave 0 in registers
-    // (pivot row by v_readlane); rows at or above the pivot get a zero
-    // multiplier instead of a branch.  Ld and invd are zero beyond w.
     // max |l| as the bits of a non-negative double: such doubles (and NaN
     // above them) order as unsigned integers; exact for FP32 and FP64 alike
     unsigned long long lbits = 0;
-    auto factor_diag = [&]() {
-        const bool row = lane < w;
-        T* Sd = S;
-        T rw[NPB];
-#pragma unroll
-        for (int k = 0; k < NPB; ++k) rw[k] = (row && k < w) ? Sd[(size_t)k * lda + c0 + lane] : T(0);
-#pragma unroll
-        for (int j = 0; j < NPB; ++j) {
-            if (j < w) {
-                const T piv = lane_value(rw[j], j);
-                T inv;
-                if constexpr (MF) inv = __builtin_amdgcn_rcpf(piv);
-                else inv = T(1) / piv;
-                const bool below = row && lane > j;
-                const T l = below ? rw[j] * inv : T(0);
-                if (below) rw[j] = l;
-                const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)l));
-                lbits = lb > lbits ? lb : lbits;
-                T pk[NPB];
-#pragma unroll
-                for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
-#pragma unroll
-                for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
-                if (lane == j) invd[j] = inv;
-            } else if (lane == j) {
-                invd[j] = T(0);
-            }
-        }
-        if (lane < NPB) {
-#pragma unroll
-            for (int k = 0; k < NPB; ++k) {
-                Ld[lane][k] = (row && k < w) ? rw[k] : T(0);
-                if (row && k < w) st_agent(&Sd[(size_t)k * lda + c0 + lane], rw[k]);
-            }
-        }
-    };
     for (int j = 0; j < s; ++j) {
         const int r0 = j * NPB;
         if (tid == 0) {
@@ -521,7 +483,7 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
         if constexpr (MF) {
 #pragma unroll
             for (int h = 0; h < TOP_RBW; ++h) {
-                const int rb = rbk(h), i = min(max(rb, 0) * NPB + (lane & 31), n - 1);
+                const int rb = top_rb(wv, h, nw, n), i = min(max(rb, 0) * NPB + (lane & 31), n - 1);
                 const bool act = rb > j;
 #pragma unroll
                 for (int kk = 0; kk < 16; ++kk)
@@ -578,7 +540,7 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
         if constexpr (MF) {
 #pragma unroll
             for (int h = 0; h < TOP_RBW; ++h) {
-                const int rb = rbk(h);
+                const int rb = top_rb(wv, h, nw, n);
                 if (rb > j) {
                     const floatx16 acc = mfma_rows32(reinterpret_cast<const float*>(&Us[0][0]), xb[h], lane);
                     const int i = rb * NPB + (lane & 31);
@@ -622,7 +584,55 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
         }
         __syncthreads();   // Ld / Us / Wt are refilled by the next step
     }
-    if (tid < 64) factor_diag();
+    // own diagonal block (rows c0 .., w x w), factored by wave 0 in registers
+    // (pivot row by v_readlane); rows at or above the pivot get a zero
+    // multiplier instead of a branch.  Ld and invd are zero beyond w.
+    if (tid < 64) {
+        const bool row = lane < w;
+        T rw[NPB];
+        unsigned int fbits = 0;   // FP32: max |l| as float bits (cheaper on the elimination's chain)
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) rw[k] = (row && k < w) ? S[(size_t)k * lda + c0 + lane] : T(0);
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            if (j < w) {
+                const T piv = lane_value(rw[j], j);
+                T inv;
+                if constexpr (MF) inv = __builtin_amdgcn_rcpf(piv);
+                else inv = T(1) / piv;
+                const bool below = row && lane > j;
+                const T l = below ? rw[j] * inv : T(0);
+                if (below) rw[j] = l;
+                if constexpr (MF) {
+                    const unsigned int lb = __float_as_uint(fabsf(l));
+                    fbits = lb > fbits ? lb : fbits;
+                } else {
+                    const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)l));
+                    lbits = lb > lbits ? lb : lbits;
+                }
+                T pk[NPB];
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
+                if (lane == j) invd[j] = inv;
+            } else if (lane == j) {
+                invd[j] = T(0);
+            }
+        }
+        if (lane < NPB) {
+#pragma unroll
+            for (int k = 0; k < NPB; ++k) {
+                Ld[lane][k] = (row && k < w) ? rw[k] : T(0);
+                if (row && k < w) st_agent(&S[(size_t)k * lda + c0 + lane], rw[k]);
+            }
+        }
+        if constexpr (MF) {
+            const unsigned long long fd = fbits > 0x7F800000u ? 0x7FF8000000000000ull
+                                                              : (unsigned long long)__double_as_longlong((double)__uint_as_float(fbits));
+            lbits = fd > lbits ? fd : lbits;
+        }
+    }
     __syncthreads();
     if constexpr (MF) {
         // L^-1 of the diagonal block (published for the later stripes' U rows)
@@ -636,9 +646,10 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
         __syncthreads();
         // rows below the diagonal block on the matrix cores
         float* Sf = reinterpret_cast<float*>(S);
+        unsigned int fb = 0;   // bits of max |l| as a float (non-negative floats and NaN order as uints)
 #pragma unroll
         for (int h = 0; h < TOP_RBW; ++h) {
-            const int rb = rbk(h);
+            const int rb = top_rb(wv, h, nw, n);
             if (below && rb > s) {
                 const int i = rb * NPB + (lane & 31), ic = min(i, n - 1);
                 float xv[16];
@@ -651,12 +662,15 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
                     if (i < n) {
                         const float lq = acc[q];
                         st_agent(Sf + (size_t)c * lda + i, lq);
-                        const unsigned long long lb = (unsigned long long)__double_as_longlong(fabs((double)lq));
-                        lbits = lb > lbits ? lb : lbits;
+                        const unsigned int lb = __float_as_uint(fabsf(lq));
+                        fb = lb > fb ? lb : fb;
                     }
                 }
             }
         }
+        const unsigned long long fbd = fb > 0x7F800000u ? 0x7FF8000000000000ull
+                                                        : (unsigned long long)__double_as_longlong((double)__uint_as_float(fb));
+        lbits = fbd > lbits ? fbd : lbits;
     } else if (r >= c0 + w && r < n) {
         // rows below the diagonal block, a row per thread
         const T* Lb = &Ld[0][0] + opaque_zero();
