@@ -1493,11 +1493,26 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     const int r = tid & (TS - 1), h = tid >> 7;
     // the diagonal-block inverse does not depend on x: load this thread's half
     // row of it now so the final product is off the dependency chain
-    double ivr[TS / 2];
+    // (held in the factor's precision: an FP32 factor only preconditions the
+    // FP64 refinement; with the prefetched tile below the FP32 kernel runs one
+    // work-group per compute unit, which measured faster than two with spills)
+    T ivr[TS / 2];
     {
         const double* iv = inv + (size_t)ib * TS * TS + (size_t)(h * (TS / 2)) * TS + r;
 #pragma unroll
-        for (int c = 0; c < TS / 2; ++c) ivr[c] = iv[(size_t)c * TS];
+        for (int c = 0; c < TS / 2; ++c) ivr[c] = (T)iv[(size_t)c * TS];
+    }
+    // FP32: the tile of the last block this one depends on: once that block
+    // is published its product is all that is left on the dependency chain,
+    // so its half row is loaded now as well (FP64 lacks the registers)
+    const int tl = t - 1;                      // that block's position in the solve order
+    const int jl = LOWER ? tl : nb - 1 - tl;   // and its index
+    const bool pre = sizeof(T) == 4 && t > 0 && r < bs && N - jl * TS >= TS;
+    T apre[TS / 2];
+    if (pre) {
+        const T* col = A + (size_t)(jl * TS + h * (TS / 2)) * N + i0 + r;
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) apre[c] = col[(size_t)c * N];
     }
     double acc = 0.0;
     unsigned int spins = 0;
@@ -1538,6 +1553,11 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
         __syncthreads();
         if (r < bs) {
             for (int b = 0; b < nr; ++b) {
+                if (pre && q + b == tl) {
+#pragma unroll
+                    for (int c = 0; c < TS / 2; ++c) acc += (double)apre[c] * xs[b][h * (TS / 2) + c];
+                    continue;
+                }
                 const int jb = LOWER ? q + b : nb - 1 - (q + b);
                 const int j0 = jb * TS, js = min(TS, N - j0);
                 const int c0 = h * (TS / 2), c1 = min(js, c0 + TS / 2);
