@@ -20,6 +20,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -1240,6 +1244,139 @@ static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* 
     return true;
 }
 
+// The look-ahead loop of lu_factor (after panel 0) with two host threads:
+// this one enqueues the main stream's work, a second one drives the panels on
+// the side stream.  rocBLAS's gemm_ex with alpha in device memory (the
+// split-fp16 update) returns only when the stream has drained up to it, so
+// with one host thread the next panel could be enqueued only once the update
+// before it had started on the device.  Hand-over per step k: this thread
+// records ev_ready after the narrow update and posts panel k+1; the other
+// waits for ev_ready on the side stream, factors the panel, records ev_panel
+// and reports its pivoting and where its L21 is.
+template <class T>
+static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, int LU_NB, unsigned int& base,
+                           bool pivoted, const T* l21, int ldl) {
+    const Lane main{ctx->stream, ctx->blas}, side{ctx->stream2, ctx->blas2};
+    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1];
+    const int lda = N;
+    auto width = [&](int rem) { return std::min(LU_NB, rem); };
+    struct Job { int k, jb, p; bool stop; };
+    struct Done { bool pivoted; const T* l21; int ldl; };
+    std::mutex mu;
+    std::condition_variable cv;
+    bool have_job = false, have_done = false, failed = false;
+    Job job{0, 0, 0, false};
+    Done done{false, nullptr, 1};
+    std::exception_ptr err;
+    int device = 0;
+    BH_HIP(hipGetDevice(&device));
+    auto fail = [&](std::exception_ptr e) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = e;
+            failed = true;
+        }
+        cv.notify_all();
+    };
+    std::thread panels([&] {
+        try {
+            BH_HIP(hipSetDevice(device));
+            for (;;) {
+                Job jo;
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return have_job || failed; });
+                    if (failed) return;
+                    jo = job;
+                    have_job = false;
+                }
+                if (jo.stop) return;
+                const int k1 = jo.k + jo.jb, rest = N - k1, jb2 = width(rest);
+                T* Ap = A + (size_t)k1 * lda + k1;
+                BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+                panel_start(ctx, side, rest, jb2, Ap, lda, ipiv + k1, k1, base, jo.p, split16);
+                Done d{false, nullptr, 1};
+                d.pivoted = panel_finish(ctx, side, rest, jb2, Ap, lda, ipiv + k1, k1, base, jo.p, split16, &d.l21,
+                                         &d.ldl);
+                BH_HIP(hipEventRecord(ev_panel, side.s));
+                // an accepted panel's L21 into the factor after the hand-over;
+                // the side stream's next panel comes after this copy, and so
+                // does every later row swap on the main stream
+                if (!d.pivoted) copy2d(side.s, rest - jb2, jb2, d.l21, d.ldl, Ap + jb2, lda);
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    done = d;
+                    have_done = true;
+                }
+                cv.notify_all();
+            }
+        } catch (...) {
+            fail(std::current_exception());
+        }
+    });
+    auto post = [&](Job j) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = j;
+            have_job = true;
+        }
+        cv.notify_all();
+    };
+    try {
+        int k = 0, jb = width(N), p = 0;
+        for (;;) {
+            BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));   // panel k factored (and the L21 copies before it)
+            if (pivoted) laswp(main, k, A + k, lda, ipiv + k, 0, jb, k);
+            const int rest = N - k - jb;
+            if (rest <= 0) break;
+            T* A12 = A + (size_t)(k + jb) * lda + k;
+            if (pivoted) laswp(main, rest, A12, lda, ipiv + k, 0, jb, k);
+            const int jb2 = width(rest);
+            const T* W = ctx->lu.winv_l[p].as<T>();
+            T* U = ctx->lu.u12.as<T>();   // U12 = L11^-1 A12, jb x rest, ld jb
+            if (split16) {
+                k_fill_u32<<<1, 64, 0, main.s>>>(ctx->split_mx.as<unsigned int>(), 2, 0u);
+                BH_HIP(hipGetLastError());
+            }
+            // the next panel's columns first, then hand them over
+            gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+            if constexpr (std::is_same<T, float>::value)
+                if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
+            trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
+            BH_HIP(hipEventRecord(ev_ready, main.s));
+            post(Job{k, jb, p ^ 1, false});
+            if (rest > jb2) {
+                T* Uw = U + (size_t)jb2 * jb;
+                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+                if constexpr (std::is_same<T, float>::value)
+                    if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
+                trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
+                                   A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
+            }
+            copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
+            Done d;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return have_done || failed; });
+                if (failed) break;
+                d = done;
+                have_done = false;
+            }
+            pivoted = d.pivoted;
+            l21 = d.l21;
+            ldl = d.ldl;
+            k += jb;
+            jb = jb2;
+            p ^= 1;
+        }
+    } catch (...) {
+        fail(std::current_exception());
+    }
+    post(Job{0, 0, 0, true});
+    panels.join();
+    if (err) std::rethrow_exception(err);
+}
+
 // Blocked right-looking LU with partial pivoting and one block of look-ahead.
 // Per outer step k (panel factored, its L21 and L11^-1 ready): the main stream
 // forms U12 = L11^-1 A12 and updates the next panel's columns first, hands them
@@ -1296,6 +1433,9 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     bool pivoted = panel_finish(ctx, main, N, jb0, A, lda, ipiv, 0, base, p, split16, &l21, &ldl);
     BH_HIP(hipEventRecord(ev_panel, main.s));
     if (!pivoted) copy2d(main.s, N - jb0, jb0, l21, ldl, A + jb0, lda);
+    if (ctx->lookahead && ctx->la_min_rest == 0 && N > jb0) {
+        lu_two_threads(ctx, N, A, ipiv, split16, LU_NB, base, pivoted, l21, ldl);
+    } else {
     bool side_copy = false;   // an L21 copy was enqueued on the side stream (ev_copied follows the latest)
     int k = 0, jb = jb0;
     while (k < N) {
@@ -1361,6 +1501,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     if (side_copy) BH_HIP(hipStreamWaitEvent(main.s, ev_copied, 0));
+    }
     unsigned int flags[3] = {0, 0, 0};
     BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, main.s));
     BH_HIP(hipStreamSynchronize(main.s));
