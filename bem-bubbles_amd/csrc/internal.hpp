@@ -69,7 +69,7 @@ struct DevBuf {
 
 // ----------------------------------------------------- mesh topology --
 // Host-built, topology-only tables (rebuilt when the connectivity changes).
-constexpr int PATCH = 32;          // columns per assembly patch
+constexpr int PATCH = 16;          // columns per assembly patch (16: 5 waves per SIMD; 32: 2.5, 25% slower)
 constexpr int ASM_ROWS = 128;      // collocation rows per assembly work-group
 
 // One dense BEM problem inside a mesh: all vertices of the mesh (coupled
