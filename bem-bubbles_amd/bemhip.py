@@ -114,6 +114,8 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_get_parts.argtypes = [vp, C.POINTER(C.c_int32)]
     L.bemhip_sim_get_part_volumes.argtypes = [vp, f64p, C.c_int32]
     L.bemhip_sim_set_part_physics.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
+    L.bemhip_exterior_potential.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p]
+    L.bemhip_sim_exterior_pot.argtypes = [vp, C.c_int64, f64p, f64p]
     L.bemhip_sim_adjoint_radius.argtypes = [vp, C.c_int32, C.c_double, f64p, C.c_int32, C.POINTER(C.c_double),
                                             vp, vp, vp]
     _lib = L
@@ -198,6 +200,18 @@ class Context:
                                              np.ascontiguousarray(phi, dtype=np.float64),
                                              flags, _solver_id(solver), tol, maxit, psi, C.byref(st)))
         return psi, st.as_dict()
+
+    # compute_exterior_pot (LinLinSim.cpp:440-477)
+    def exterior_potential(self, verts, tris, phi, psi, points):
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        self.check(self.L.bemhip_exterior_potential(self.h, v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1),
+                                                    np.ascontiguousarray(phi, dtype=np.float64),
+                                                    np.ascontiguousarray(psi, dtype=np.float64),
+                                                    pts.shape[0], pts.reshape(-1), out))
+        return out
 
 
 class ColocSim:
@@ -346,6 +360,13 @@ class ColocSim:
     def compute_psi(self):
         self._check(self.ctx.L.bemhip_sim_compute_psi(self.h))
         return self.get_psi()
+
+    # LinLinSim::exterior_pot (LinLinSim.cpp:478-483)
+    def exterior_pot(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        self._check(self.ctx.L.bemhip_sim_exterior_pot(self.h, pts.shape[0], pts.reshape(-1), out))
+        return out
 
     def evolve_system_RK4(self, dp, fixdt=False):
         dt = C.c_double()

@@ -672,6 +672,58 @@ int bemhip_sim_compute_psi(bemhip_sim* s) {
     });
 }
 
+int bemhip_exterior_potential(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                              const double* phi, const double* psi, int64_t P, const double* points,
+                              double* out) {
+    if (!ctx) return BEMHIP_ERR_ARG;
+    return guarded(ctx, [&] {
+        BH_CHECK(N > 0 && M > 0 && P >= 0, BEMHIP_ERR_ARG, "bad sizes");
+        BH_CHECK(xyz && tri && phi && psi && (P == 0 || (points && out)), BEMHIP_ERR_ARG, "null argument");
+        for (int64_t i = 0; i < 3 * M; ++i) BH_CHECK(tri[i] < (uint64_t)N, BEMHIP_ERR_BAD_MESH, "vertex index out of range");
+        check_finite(phi, N, "phi");
+        check_finite(psi, N, "psi");
+        check_finite(points, 3 * P, "points");
+        if (P == 0) return;
+        double* dx = dev<double>(ctx->xyz, 3 * N);
+        double* dphi = dev<double>(ctx->phi, N);
+        double* dpsi = dev<double>(ctx->ext_psi, N);
+        uint32_t* dtri = dev<uint32_t>(ctx->ext_tri, 3 * M);
+        double* dtg = dev<double>(ctx->ext_tgt, 3 * P);
+        double* dout = dev<double>(ctx->ext_out, P);
+        h2d(dx, xyz, sizeof(double) * 3 * N, ctx->stream);
+        h2d(dphi, phi, sizeof(double) * N, ctx->stream);
+        h2d(dpsi, psi, sizeof(double) * N, ctx->stream);
+        h2d(dtri, tri, sizeof(uint32_t) * 3 * M, ctx->stream);
+        h2d(dtg, points, sizeof(double) * 3 * P, ctx->stream);
+        exterior_device(ctx, M, dx, dtri, dphi, dpsi, P, dtg, dout);
+        d2h(out, dout, sizeof(double) * P, ctx->stream);
+    });
+}
+
+int bemhip_sim_exterior_pot(bemhip_sim* s, int64_t P, const double* points, double* out) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(P >= 0 && (P == 0 || (points && out)), BEMHIP_ERR_ARG, "bad arguments");
+        check_finite(points, 3 * P, "points");
+        if (P == 0) return;
+        bemhip_ctx* ctx = s->ctx;
+        const int64_t N = s->N;
+        // psi for the current state into scratch, the simulation's psi untouched
+        double* hphi = dev<double>(s->hphi, N);
+        double* psi = dev<double>(ctx->ext_psi, N);
+        bemhip_solve_stats st{};
+        psi_device(ctx, s->topo, s->x.as<double>(), s->phi.as<double>(), s->P.flags, s->P.solver, s->P.tol,
+                   s->P.maxit, hphi, psi, &st);
+        BH_CHECK(st.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
+        double* dtg = dev<double>(ctx->ext_tgt, 3 * P);
+        double* dout = dev<double>(ctx->ext_out, P);
+        h2d(dtg, points, sizeof(double) * 3 * P, ctx->stream);
+        exterior_device(ctx, s->M, s->x.as<double>(), s->topo.tri.as<uint32_t>(), s->phi.as<double>(), psi, P, dtg,
+                        dout);
+        d2h(out, dout, sizeof(double) * P, ctx->stream);
+    });
+}
+
 int bemhip_sim_evolve_rk4(bemhip_sim* s, double dp, int fixdt, double* dt_out) {
     if (!s) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {

@@ -188,6 +188,7 @@ struct bemhip_ctx : bemhip::Solver {
     int cus = 0;
     // assembly and per-problem buffers
     bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H;
+    bemhip::DevBuf ext_rec, ext_part, ext_tgt, ext_out, ext_tri, ext_psi;   // exterior potential
     bemhip::DevBuf adj_coef, adj_tc, adj_tpart, adj_rpart, adj_sc, adj_zero;   // adjoint partials
     hipEvent_t ev[4] = {};
     // extra solvers for batches of independent problems (created on first use)
@@ -209,6 +210,10 @@ void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_x
 
 // ---------------------------------------------------------- dense LA --
 // LU with partial pivoting (column-major, lda = N), pivots as absolute rows.
+// potential at P points off the surface (compute_exterior_pot, LinLinSim.cpp:440-477);
+// all pointers on the device, out[P]
+void exterior_device(bemhip_ctx* ctx, int64_t M, const double* xyz, const uint32_t* tri, const double* phi,
+                     const double* psi, int64_t P, const double* tgt, double* out);
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
 template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* ipiv, double* x);

@@ -42,6 +42,27 @@ constexpr double TQ_X[7] = {1.0 / 3.0, TQ_A1, TQ_B1, TQ_B1, TQ_A2, TQ_B2, TQ_B2}
 constexpr double TQ_Y[7] = {1.0 / 3.0, TQ_B1, TQ_A1, TQ_B1, TQ_B2, TQ_A2, TQ_B2};
 constexpr double TQ_W[7] = {TQ_W0, TQ_W1, TQ_W1, TQ_W1, TQ_W2, TQ_W2, TQ_W2};
 
+// Degree-9 19-point triangle rule (Dunavant 1985) on the same unit triangle,
+// as the reference lists it for the exterior potential (quadrature_19,
+// quadrature.hpp:126-146; weights there are written w * 0.5): the centroid,
+// four orbits (a, b), (b, a), (b, b) and one six-point orbit, in the
+// reference's order so that sums run in the same sequence.
+constexpr double Q19_C = 0.33333333333333331;
+constexpr double Q19_A1 = 2.06349616025259287e-2, Q19_B1 = 0.48968251919873701;
+constexpr double Q19_A2 = 0.12582081701412900, Q19_B2 = 0.43708959149293553;
+constexpr double Q19_A3 = 0.62359292876193562, Q19_B3 = 0.18820353561903219;
+constexpr double Q19_A4 = 0.91054097321109406, Q19_B4 = 4.47295133944529688e-2;
+constexpr double Q19_E1 = 0.74119859878449801, Q19_E2 = 3.68384120547362581e-2, Q19_E3 = 0.22196298916076573;
+constexpr double Q19_W0 = 9.71357962827961025e-2 * 0.5, Q19_W1 = 3.13347002271398278e-2 * 0.5,
+                 Q19_W2 = 7.78275410047754301e-2 * 0.5, Q19_W3 = 7.96477389272090969e-2 * 0.5,
+                 Q19_W4 = 2.55776756586981006e-2 * 0.5, Q19_W5 = 4.32835393772893970e-2 * 0.5;
+constexpr double Q19_X[19] = {Q19_C,  Q19_A1, Q19_B1, Q19_B1, Q19_A2, Q19_B2, Q19_B2, Q19_A3, Q19_B3, Q19_B3,
+                              Q19_A4, Q19_B4, Q19_B4, Q19_E1, Q19_E1, Q19_E2, Q19_E2, Q19_E3, Q19_E3};
+constexpr double Q19_Y[19] = {Q19_C,  Q19_B1, Q19_A1, Q19_B1, Q19_B2, Q19_A2, Q19_B2, Q19_B3, Q19_A3, Q19_B3,
+                              Q19_B4, Q19_A4, Q19_B4, Q19_E2, Q19_E3, Q19_E1, Q19_E3, Q19_E1, Q19_E2};
+constexpr double Q19_W[19] = {Q19_W0, Q19_W1, Q19_W1, Q19_W1, Q19_W2, Q19_W2, Q19_W2, Q19_W3, Q19_W3, Q19_W3,
+                              Q19_W4, Q19_W4, Q19_W4, Q19_W5, Q19_W5, Q19_W5, Q19_W5, Q19_W5, Q19_W5};
+
 constexpr double PI = 3.14159265358979323846264338;
 
 }  // namespace bemhip

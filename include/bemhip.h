@@ -170,6 +170,21 @@ int bemhip_sim_last_stats(bemhip_sim* sim, bemhip_solve_stats* stats);
  * last reset; residual / correction hold the maxima */
 int bemhip_sim_accum_stats(bemhip_sim* sim, bemhip_solve_stats* stats, int reset);
 
+/* ---- potential off the surface ----
+ * phi_ext(y) = 1/(4 pi) sum_T [ -sum_k G_k(y,T) psi_k + sum_k H_k(y,T) phi_k ]
+ * for P points y (points: P x 3), with the disjoint-panel integrals under the
+ * 19-point degree-9 rule and no mirror image; phi and psi hold N values.
+ * Replaces the free function compute_exterior_pot (LinLinSim.cpp:440-477,
+ * declared LinLinSim.hpp:17; per triangle Integrator.cpp:287-305).  Points
+ * on the surface are singular, as in the reference. */
+int bemhip_exterior_potential(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                              const double* phi, const double* psi, int64_t P, const double* points,
+                              double* out);
+/* LinLinSim::exterior_pot (LinLinSim.cpp:478-483): psi for the current mesh
+ * and phi (the simulation's solver and mirror flag; the simulation's psi is
+ * left unchanged), then the potential at the P points from (phi, psi). */
+int bemhip_sim_exterior_pot(bemhip_sim* sim, int64_t P, const double* points, double* out);
+
 /* ---- adjoint of a radius trajectory (no counterpart in the reference) ----
  * From the simulation's current state (x0, phi0, time), runs `nsteps` RK4
  * steps of fixed size dt (LinLinSim::evolve_system_RK4 with fixdt, no wall)

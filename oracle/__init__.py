@@ -70,6 +70,9 @@ def lib():
                                     C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.bo_euler_step.restype = C.c_int
         L.bo_compute_psi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
+        L.bo_quadrature19.argtypes = [f64p] * 3
+        L.bo_exterior_panel.argtypes = [f64p] * 6
+        L.bo_exterior_pot.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -82,6 +85,27 @@ def quadrature():
     out = [np.zeros(7) for _ in range(5)]
     lib().bo_quadrature(*out)
     return {"gauss_x": out[0], "gauss_w": out[1], "tri_x": out[2], "tri_y": out[3], "tri_w": out[4]}
+
+
+def quadrature19():
+    out = [np.zeros(19) for _ in range(3)]
+    lib().bo_quadrature19(*out)
+    return {"tri_x": out[0], "tri_y": out[1], "tri_w": out[2]}
+
+
+def exterior_panel(y, a, b, c):
+    g, h = np.zeros(3), np.zeros(3)
+    lib().bo_exterior_panel(_c(y, np.float64), _c(a, np.float64), _c(b, np.float64), _c(c, np.float64), g, h)
+    return g, h
+
+
+def exterior_pot(verts, tris, phi, psi, points, nthreads=0):
+    v, t = _c(verts, np.float64), _c(tris, np.uint32)
+    pts = _c(points, np.float64).reshape(-1, 3)
+    out = np.zeros(pts.shape[0])
+    lib().bo_exterior_pot(v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1), _c(phi, np.float64),
+                          _c(psi, np.float64), pts.shape[0], pts.reshape(-1), out, nthreads)
+    return out
 
 
 def panel(x, a, b, c, self_vertex=-1, mirror=False):

@@ -83,6 +83,40 @@ static void init_quadrature(void) {
     quad_ready = 1;
 }
 
+/* 19-point degree-9 triangle rule (Dunavant 1985), listed in the reference as
+ * quadrature_19 (quadrature.hpp:126-146): centroid, four orbits
+ * (a, b), (b, a), (b, b) and one six-point orbit, in the reference's order;
+ * the reference writes each weight as w * 0.5 (the triangle's area). */
+static const double q19_orb[5][3] = {   /* a, b, weight */
+    {2.06349616025259287e-2, 0.48968251919873701, 3.13347002271398278e-2},
+    {0.12582081701412900, 0.43708959149293553, 7.78275410047754301e-2},
+    {0.62359292876193562, 0.18820353561903219, 7.96477389272090969e-2},
+    {0.91054097321109406, 4.47295133944529688e-2, 2.55776756586981006e-2},
+};
+static double t19x[19], t19y[19], t19w[19];
+static int q19_ready = 0;
+static void init_q19(void) {
+    if (q19_ready) return;
+    int q = 0;
+    t19x[q] = 0.33333333333333331; t19y[q] = 0.33333333333333331; t19w[q] = 9.71357962827961025e-2 * 0.5; ++q;
+    for (int o = 0; o < 4; ++o) {
+        const double a = q19_orb[o][0], b = q19_orb[o][1], w = q19_orb[o][2] * 0.5;
+        t19x[q] = a; t19y[q] = b; t19w[q] = w; ++q;
+        t19x[q] = b; t19y[q] = a; t19w[q] = w; ++q;
+        t19x[q] = b; t19y[q] = b; t19w[q] = w; ++q;
+    }
+    const double e1 = 0.74119859878449801, e2 = 3.68384120547362581e-2, e3 = 0.22196298916076573;
+    const double we = 4.32835393772893970e-2 * 0.5;
+    const double ex[6] = {e1, e1, e2, e2, e3, e3}, ey[6] = {e2, e3, e1, e3, e1, e2};
+    for (int k = 0; k < 6; ++k) { t19x[q] = ex[k]; t19y[q] = ey[k]; t19w[q] = we; ++q; }
+    q19_ready = 1;
+}
+
+void bo_quadrature19(double* tx, double* ty, double* tw) {
+    init_q19();
+    for (int q = 0; q < 19; ++q) { tx[q] = t19x[q]; ty[q] = t19y[q]; tw[q] = t19w[q]; }
+}
+
 void bo_quadrature(double* gx, double* gw, double* tx, double* ty, double* tw) {
     init_quadrature();
     for (int q = 0; q < 7; ++q) {
@@ -195,6 +229,55 @@ void bo_panel(const double* x, const double* a, const double* b, const double* c
 }
 
 /* ColocSim::assemble_matrices (ColocSim.cpp:21-110) */
+/* integrate_disjoint_coloc (Integrator.hpp:471-491) under the 19-point rule */
+static void disjoint19(v3 x, const interp_t* I, double* g3, double* h3) {
+    double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+    for (int q = 0; q < 19; ++q) {
+        const double u = t19x[q] + t19y[q], v = t19y[q];
+        v3 y = interp_at(I, u, v);
+        double g, h;
+        kernel_pair(vsub(y, x), I->n, &g, &h);
+        const double phi[3] = {1.0 - u, u - v, v};
+        for (int k = 0; k < 3; ++k) {
+            G[k] += phi[k] * g * t19w[q];
+            H[k] += phi[k] * h * t19w[q];
+        }
+    }
+    for (int k = 0; k < 3; ++k) { g3[k] = G[k] * I->area; h3[k] = H[k] * I->area; }
+}
+
+void bo_exterior_panel(const double* y, const double* a, const double* b, const double* c, double* g3, double* h3) {
+    init_q19();
+    interp_t I = make_interp(V(a), V(b), V(c));
+    disjoint19(V(y), &I, g3, h3);
+}
+
+/* compute_exterior_pot (LinLinSim.cpp:440-477) with
+ * Integrator::get_exterior_potential (Integrator.cpp:287-305): per target a
+ * sum over triangles of (-sum G_k psi_k + sum H_k phi_k) / (4 pi) */
+void bo_exterior_pot(int64_t N, const double* xyz, int64_t M, const uint32_t* tri, const double* phi,
+                     const double* psi, int64_t P, const double* pts, double* out, int nthreads) {
+    (void)N;
+    init_q19();
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < P; ++i) {
+        const v3 y = V(pts + 3 * i);
+        double val = 0.0;
+        for (int64_t t = 0; t < M; ++t) {
+            const uint32_t ia = tri[3 * t], ib = tri[3 * t + 1], ic = tri[3 * t + 2];
+            interp_t I = make_interp(V(xyz + 3 * (size_t)ia), V(xyz + 3 * (size_t)ib), V(xyz + 3 * (size_t)ic));
+            double g[3], h[3];
+            disjoint19(y, &I, g, h);
+            double integral = -g[0] * psi[ia] - g[1] * psi[ib] - g[2] * psi[ic]
+                              + h[0] * phi[ia] + h[1] * phi[ib] + h[2] * phi[ic];
+            integral *= 1.0 / (4.0 * M_PI);
+            val += integral;
+        }
+        out[i] = val;
+    }
+}
+
 void bo_assemble(int64_t N, const double* xyz, int64_t M, const uint32_t* tri, int mirror,
                  double* G, double* H, int nthreads) {
     init_quadrature();

@@ -25,6 +25,14 @@ extern "C" {
 void bo_quadrature(double* gauss7_x, double* gauss7_w,      /* 7 + 7 on (0,1)        */
                    double* tri7_x, double* tri7_y, double* tri7_w); /* 7 x 3 on unit triangle */
 
+/* the 19-point degree-9 rule of the exterior potential (quadrature.hpp:126-146) */
+void bo_quadrature19(double* tri19_x, double* tri19_y, double* tri19_w);
+/* one disjoint panel under the 19-point rule (Integrator.hpp:471-491) */
+void bo_exterior_panel(const double* y, const double* a, const double* b, const double* c, double* g3, double* h3);
+/* compute_exterior_pot (LinLinSim.cpp:440-477): out[P] at the points pts (P x 3) */
+void bo_exterior_pot(int64_t N, const double* xyz, int64_t M, const uint32_t* tri, const double* phi,
+                     const double* psi, int64_t P, const double* pts, double* out, int nthreads);
+
 /* single (collocation point, triangle) panel integrals, 3 G + 3 H values in
  * triangle-vertex order (Integrator.cpp:237-283) */
 void bo_panel(const double* x, const double* a, const double* b, const double* c,

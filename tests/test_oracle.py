@@ -56,6 +56,20 @@ def test_quadrature_exactness(oracle_lib):
         assert abs(np.sum(q["gauss_w"] * q["gauss_x"] ** k) - 1.0 / (k + 1)) < 1e-15, k
 
 
+def test_quadrature19_matches_reference_table(oracle_lib):
+    ref = json.load(open(os.path.join(GOLD, "quadrature_reference.json")))
+    q = oracle_lib.quadrature19()
+    assert np.max(np.abs(q["tri_x"] - np.array(ref["quadrature_19_x"]))) < 2e-16
+    assert np.max(np.abs(q["tri_y"] - np.array(ref["quadrature_19_y"]))) < 2e-16
+    assert np.max(np.abs(q["tri_w"] - 0.5 * np.array(ref["quadrature_19_w_unhalved"]))) < 2e-17
+    # degree 9 (Dunavant): every monomial of degree <= 9 exactly, degree 10 not
+    for a in range(10):
+        for b in range(10 - a):
+            exact = fact(a) * fact(b) / fact(a + b + 2)
+            assert abs(np.sum(q["tri_w"] * q["tri_x"] ** a * q["tri_y"] ** b) - exact) < 1e-15, (a, b)
+    assert abs(np.sum(q["tri_w"] * q["tri_x"] ** 10) - fact(10) / fact(12)) > 1e-9
+
+
 # ---------------------------------------------------------------- panels --
 
 def _sub_rule(a, b, c, level):
@@ -262,3 +276,76 @@ def test_rk4_follows_rayleigh_plesset(oracle_lib):
     assert radii[-1] > 1.005          # the bubble grows: sign of the coupling
     # psi after a step is the averaged velocity projected on the new normals
     assert np.allclose(sim.psi, sim.psi.mean(), rtol=1e-2)
+
+
+# ------------------------------------------------- potential off the surface --
+
+def test_exterior_panel_against_brute_force(oracle_lib):
+    a, b, c = [0.1, 0.0, 0.05], [0.9, 0.2, -0.1], [0.3, 0.7, 0.2]
+    # (point, tolerance on G, on H): the rule's own error grows as y nears the panel
+    for y, tg, th in (([0.5, -1.2, 1.6], 1e-10, 1e-9), ([0.4, 0.3, 0.6], 1e-5, 1e-4)):
+        g, h = oracle_lib.exterior_panel(y, a, b, c)
+        P, W = _sub_rule(a, b, c, 5)
+        phi = _barycentric(P, a, b, c)
+        z = P - np.array(y)
+        r = np.linalg.norm(z, axis=1)
+        n = np.cross(np.array(b) - a, np.array(c) - np.array(b))
+        n /= np.linalg.norm(n)
+        gb = (phi * (W / r)[:, None]).sum(0)
+        hb = (phi * (W * (-(z @ n)) / r ** 3)[:, None]).sum(0)
+        assert np.allclose(g, gb, rtol=tg, atol=0), (y, g, gb)
+        assert np.allclose(h, hb, rtol=th, atol=0), (y, h, hb)
+        # the degree-9 rule beats the degree-5 rule of the matrices by far
+        g7, h7 = oracle_lib.panel(y, a, b, c, -1, False)
+        assert np.max(np.abs(g - gb)) * 50 < np.max(np.abs(g7 - gb))
+        assert np.max(np.abs(h - hb)) * 50 < np.max(np.abs(h7 - hb))
+
+
+def _sphere_points(rng, count, rmin, rmax):
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return d * rng.uniform(rmin, rmax, size=(count, 1))
+
+
+def test_exterior_potential_green_identities_on_sphere(oracle_lib):
+    """Green's representation on the unit sphere: with n pointing out of the
+    sphere, 1/(4 pi) int [phi d_n(1/r) - psi / r] dS equals u(y) outside for a
+    decaying exterior field u and -u(y) inside for a field harmonic inside,
+    and vanishes on the other side; the discrete values converge as O(h^2)."""
+    rng = np.random.default_rng(5)
+    out_pts = _sphere_points(rng, 12, 1.5, 3.0)
+    in_pts = _sphere_points(rng, 12, 0.0, 0.5)
+    errs = []
+    for nu in (4, 8):
+        v, t = meshgen.icosphere(nu, 1.0)
+        one = np.ones(len(v))
+        # u = 1/|y|: phi = 1, psi = d_r u = -1
+        e_out = oracle_lib.exterior_pot(v, t, one, -one, out_pts) - 1.0 / np.linalg.norm(out_pts, axis=1)
+        e_in0 = oracle_lib.exterior_pot(v, t, one, -one, in_pts)
+        # constant phi, psi = 0: 0 outside, -1 inside (solid angle)
+        c_out = oracle_lib.exterior_pot(v, t, one, 0 * one, out_pts)
+        c_in = oracle_lib.exterior_pot(v, t, one, 0 * one, in_pts) + 1.0
+        # u = z + 2 x y (harmonic inside): phi = u, psi = grad u . n; -u inside, 0 outside
+        u = v[:, 2] + 2 * v[:, 0] * v[:, 1]
+        du = v[:, 2] + 4 * v[:, 0] * v[:, 1]                # r d_r of a degree-1 plus a degree-2 term
+        l_in = oracle_lib.exterior_pot(v, t, u, du, in_pts) + (in_pts[:, 2] + 2 * in_pts[:, 0] * in_pts[:, 1])
+        l_out = oracle_lib.exterior_pot(v, t, u, du, out_pts)
+        errs.append([np.max(np.abs(e)) for e in (e_out, e_in0, c_out, c_in, l_in, l_out)])
+    e4, e8 = np.array(errs[0]), np.array(errs[1])
+    assert np.all(e8 < 5e-3), e8
+    assert np.all(e4 / e8 > 3.0), (e4, e8)                   # O(h^2)
+    # the flat panels' solid angles close exactly: only the rule's error is left
+    assert np.all(e4[2:4] < 1e-9) and np.all(e8[2:4] < 1e-12), (e4, e8)
+
+
+def test_exterior_potential_is_sum_of_panels(oracle_lib):
+    v, t = meshgen.icosphere(2, 1.1, (0.1, 0.0, -0.2))
+    rng = np.random.default_rng(2)
+    phi, psi = rng.normal(size=len(v)), rng.normal(size=len(v))
+    y = np.array([[1.7, -0.4, 0.9]])
+    want = 0.0
+    for (ia, ib, ic) in t:
+        g, h = oracle_lib.exterior_panel(y[0], v[ia], v[ib], v[ic])
+        want += (-g @ psi[[ia, ib, ic]] + h @ phi[[ia, ib, ic]]) / (4 * math.pi)
+    got = oracle_lib.exterior_pot(v, t, phi, psi, y, nthreads=1)
+    assert abs(got[0] - want) < 1e-14 * max(1.0, abs(want))
