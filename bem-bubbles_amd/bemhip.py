@@ -27,7 +27,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbemhip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "bemhip.h")
 
-OK, ERR_ARG, ERR_BAD_MESH, ERR_NEG_X, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_NAN, ERR_OOM, ERR_HIP, ERR_NO_DEVICE = range(10)
+(OK, ERR_ARG, ERR_BAD_MESH, ERR_NEG_X, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_NAN, ERR_OOM, ERR_HIP, ERR_NO_DEVICE,
+ ERR_IO) = range(11)
 MIRROR_X0 = 0x1
 INDEPENDENT_PARTS = 0x2
 SOLVERS = {"lu64": 0, "lu32_ir64": 1, "bicgstab64": 2, "lu16x3_ir64": 3}
@@ -116,10 +117,74 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_set_part_physics.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
     L.bemhip_exterior_potential.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p]
     L.bemhip_sim_exterior_pot.argtypes = [vp, C.c_int64, f64p, f64p]
+    L.bemhip_sim_get_triangles.argtypes = [vp, u32p]
+    L.bemhip_sim_export_ply.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.bemhip_pin_boundary.argtypes = [C.c_int64, f64p, C.c_int64, u32p, vp, vp, C.POINTER(C.c_int64)]
+    L.bemhip_sim_create_pinned.argtypes = [vp, C.POINTER(SimParams), C.c_int64, f64p, C.c_int64, u32p, vp, vp,
+                                           C.POINTER(vp), C.POINTER(C.c_int64)]
+    L.bemhip_sim_set_pinned.argtypes = [vp, C.c_int64]
+    L.bemhip_sim_get_pinned.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.bemhip_io_last_error.argtypes = []
+    L.bemhip_io_last_error.restype = C.c_char_p
+    L.bemhip_ply_write.argtypes = [C.c_char_p, C.c_int64, f64p, C.c_int64, u32p, vp, vp, C.c_int32]
+    L.bemhip_ply_info.argtypes = [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]
+    L.bemhip_ply_read.argtypes = [C.c_char_p, C.c_int64, f64p, C.c_int64, u32p, vp, vp]
     L.bemhip_sim_adjoint_radius.argtypes = [vp, C.c_int32, C.c_double, f64p, C.c_int32, C.POINTER(C.c_double),
                                             vp, vp, vp]
     _lib = L
     return L
+
+
+def _io_check(L, rc):
+    if rc != OK:
+        raise BemHipError(rc, L.bemhip_io_last_error().decode())
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data
+
+
+def write_ply(fname, verts, tris, phi=None, psi=None, precision=8):
+    """export_ply_double / export_ply_float (MeshIO.cpp:263-335) and the one-value
+    and bare-mesh variants (MeshIO.cpp:67-100, 191-261); no device needed."""
+    L = load()
+    v = np.ascontiguousarray(verts, dtype=np.float64)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    ph = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+    ps = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+    _io_check(L, L.bemhip_ply_write(os.fsencode(fname), v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1),
+                                    _vp(ph), _vp(ps), int(precision)))
+
+
+def read_ply(fname):
+    """import_ply (MeshIO.cpp:701-903): (verts, tris, phi, psi); phi / psi are
+    None when the file holds no such values."""
+    L = load()
+    N, M, nv, pr = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+    path = os.fsencode(fname)
+    _io_check(L, L.bemhip_ply_info(path, C.byref(N), C.byref(M), C.byref(nv), C.byref(pr)))
+    v = np.zeros((N.value, 3))
+    t = np.zeros((M.value, 3), dtype=np.uint32)
+    phi = np.zeros(N.value) if nv.value >= 1 else None
+    psi = np.zeros(N.value) if nv.value >= 2 else None
+    _io_check(L, L.bemhip_ply_read(path, N.value, v.reshape(-1), M.value, t.reshape(-1), _vp(phi), _vp(psi)))
+    return v, t, phi, psi
+
+
+def pin_boundary(verts, tris, phi=None, psi=None):
+    """ColocSimPin::set_x_boundary (ColocSimPin.cpp:222-287) on copies:
+    (verts, tris, phi, psi, n_pin); no device needed."""
+    L = load()
+    v = np.array(verts, dtype=np.float64, copy=True)
+    t = np.array(tris, dtype=np.uint32, copy=True)
+    ph = None if phi is None else np.array(phi, dtype=np.float64, copy=True)
+    ps = None if psi is None else np.array(psi, dtype=np.float64, copy=True)
+    n = C.c_int64()
+    rc = L.bemhip_pin_boundary(v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1), _vp(ph), _vp(ps), C.byref(n))
+    if rc != OK:
+        raise BemHipError(rc, "pin_boundary: bad arguments or mesh")
+    return v, t, ph, ps, n.value
 
 
 def quadrature():
@@ -235,10 +300,14 @@ class ColocSim:
         v = np.ascontiguousarray(verts, dtype=np.float64)
         t = np.ascontiguousarray(tris, dtype=np.uint32)
         self.N, self.M = v.shape[0], t.shape[0]
-        h = C.c_void_p()
-        ctx.check(L.bemhip_sim_create(ctx.h, C.byref(p), self.N, v.reshape(-1), self.M, t.reshape(-1), C.byref(h)))
-        self.h = h
+        self.h = self._create(L, p, v, t)
         ctx._sims.add(self)
+
+    def _create(self, L, p, v, t):
+        h = C.c_void_p()
+        self.ctx.check(L.bemhip_sim_create(self.ctx.h, C.byref(p), self.N, v.reshape(-1), self.M, t.reshape(-1),
+                                           C.byref(h)))
+        return h
 
     def close(self):
         if getattr(self, "h", None):
@@ -361,6 +430,26 @@ class ColocSim:
         self._check(self.ctx.L.bemhip_sim_compute_psi(self.h))
         return self.get_psi()
 
+    # Simulation::export_mesh (Simulation.cpp:94-96) writes floats; precision=8
+    # is export_ply_double (MeshIO.cpp:300-335), which restores the state exactly
+    def export_mesh(self, fname, precision=4):
+        self._check(self.ctx.L.bemhip_sim_export_ply(self.h, os.fsencode(fname), int(precision)))
+
+    def get_triangles(self):
+        t = np.zeros((self.M, 3), dtype=np.uint32)
+        self._check(self.ctx.L.bemhip_sim_get_triangles(self.h, t.reshape(-1)))
+        return t
+
+    @property
+    def N_pin(self):   # ColocSimPin.hpp:57
+        n = C.c_int64(0)
+        self._check(self.ctx.L.bemhip_sim_get_pinned(self.h, C.byref(n)))
+        return n.value
+
+    @N_pin.setter
+    def N_pin(self, value):
+        self._check(self.ctx.L.bemhip_sim_set_pinned(self.h, int(value)))
+
     # LinLinSim::exterior_pot (LinLinSim.cpp:478-483)
     def exterior_pot(self, points):
         pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
@@ -387,3 +476,23 @@ class ColocSim:
         st = SolveStats()
         self._check(self.ctx.L.bemhip_sim_accum_stats(self.h, C.byref(st), int(reset)))
         return st.as_dict()
+
+
+class ColocSimPin(ColocSim):
+    """Drop-in for Bem::ColocSimPin (ColocSimPin.hpp:13-68): a bubble on the wall
+    x = 0 whose contact-line vertices (|x| < 1e-5) are pinned; the vertices are
+    reordered on construction (pinned last), as in the reference.  Wall mode
+    (mirror) is on by default, as the reference compiles it (LinLinSim.hpp:13)."""
+
+    def __init__(self, ctx: Context, verts, tris, *args, phi=None, psi=None, mirror=True, **kw):
+        self._phi0 = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        self._psi0 = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        super().__init__(ctx, verts, tris, *args, mirror=mirror, **kw)
+
+    def _create(self, L, p, v, t):
+        h = C.c_void_p()
+        n = C.c_int64()
+        self.ctx.check(L.bemhip_sim_create_pinned(self.ctx.h, C.byref(p), self.N, v.reshape(-1), self.M,
+                                                  t.reshape(-1), _vp(self._phi0), _vp(self._psi0), C.byref(h),
+                                                  C.byref(n)))
+        return h

@@ -34,6 +34,8 @@ struct bemhip_sim {
     int64_t N = 0, M = 0;
     double time = 0.0;
     DevBuf x, phi, psi, x1, p1, xs, ps, kx, kp, avg, vn, tnorm, tgrad, tvol, pvol, tk, hphi, psil, pphys;
+    int64_t n_pin = 0;   // ColocSimPin: the last n_pin vertices are pinned on the wall
+    DevBuf pz;           // stage potential with the pinned entries zeroed
     // per-part V_0, field amplitude, frequency, phase; NaN = the scalar in P
     std::vector<double> part_v0, part_amp, part_freq, part_phase;
     // adjoint work space: start state, checkpoints, segment states, cotangents
@@ -268,6 +270,74 @@ void stage(bemhip_sim* s, const double* xs, const double* ps, double t, double* 
     vertex_dpot(ctx, s->topo, stage_params(s), xs, kx, tk, pvol, s->pphys.as<double>(), t, kp);
 }
 
+// ColocSimPin (ColocSimPin.cpp:21-118) + solve_system: the unknowns are psi at
+// the free vertices and phi at the pinned ones; pot holds phi at the free
+// vertices and the prescribed psi at the pinned ones.  H pot is formed in the
+// assembly over the free columns (pinned entries zeroed), and the column swap
+// adds the pinned columns' part.
+void psi_pinned(bemhip_sim* s, const double* xs, const double* pot, double* rhs, double* x,
+                bemhip_solve_stats* st, bool report_residual = true) {
+    bemhip_ctx* ctx = s->ctx;
+    const int64_t N = s->N, np = s->n_pin;
+    BH_CHECK(N <= 65536, BEMHIP_ERR_ARG, "dense solve supports at most 65536 unknowns per problem");
+    const size_t nn = (size_t)N * N;
+    double* G64 = dev<double>(ctx->G64, nn);
+    float* G32 = mixed(s->P.solver) ? dev<float>(ctx->G32, nn) : nullptr;
+    double* H = dev<double>(ctx->H, nn);
+    double* pz = dev<double>(s->pz, N);
+    pin_zero_tail(ctx, N, np, pot, pz);
+    hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+    BH_HIP(hipEventRecord(e0, ctx->stream));
+    assemble_device(ctx, s->topo, xs, pz, s->P.flags, AsmOut{G64, G32, H, rhs});
+    pin_swap_columns(ctx, N, np, G64, G32, H, pot, rhs);
+    BH_HIP(hipEventRecord(e1, ctx->stream));
+    const float ams = event_ms(e0, e1);
+    solve_device(ctx, (int)N, G64, G32, rhs, x, s->P.solver, s->P.tol, s->P.maxit, st, report_residual);
+    st->ms_assemble = ams;
+}
+
+// ColocSimPin::position_t (ColocSimPin.cpp:139-194) + pot_t_multi.  As in the
+// reference, ps is rewritten: its pinned entries take the state's psi for the
+// solve and the solved phi afterwards.
+void stage_pinned(bemhip_sim* s, const double* xs, double* ps, double t, double* kx, double* kp) {
+    bemhip_ctx* ctx = s->ctx;
+    const int64_t N = s->N, M = s->M;
+    double* tnorm = dev<double>(s->tnorm, 3 * M);
+    double* tgrad = dev<double>(s->tgrad, 3 * M);
+    double* tvol = dev<double>(s->tvol, M);
+    tri_geometry(ctx, s->topo, xs, ps, tnorm, tgrad, tvol);   // tangential gradients of the stage phi first
+    pin_copy_tail(ctx, N, s->n_pin, s->psi.as<double>(), ps);
+    double* hphi = dev<double>(s->hphi, N);
+    double* psil = dev<double>(s->psil, N);
+    psi_pinned(s, xs, ps, hphi, psil, &s->last, false);
+    accumulate(s->accum, s->last);
+    vertex_velocity(ctx, s->topo, tnorm, tgrad, psil, kx);
+    double* vn = dev<double>(s->vn, 3 * N);
+    vertex_normals(ctx, s->topo, xs, vn);
+    pin_drift(ctx, N, s->n_pin, vn, psil, kx, ps);
+    if (!kp) return;
+    double* pvol = dev<double>(s->pvol, s->topo.nparts);
+    part_volumes(ctx, s->topo, tvol, pvol);
+    double* tk = nullptr;
+    if (s->P.sigma != 0.0) {
+        tk = dev<double>(s->tk, M);
+        tri_curvature(ctx, s->topo, xs, vn, tk);
+    }
+    vertex_dpot(ctx, s->topo, stage_params(s), xs, kx, tk, pvol, s->pphys.as<double>(), t, kp);
+}
+
+// a stage of the time steppers: the pinned variant may rewrite ps
+void stage_step(bemhip_sim* s, const double* xs, double* ps, double t, double* kx, double* kp) {
+    if (s->n_pin > 0) stage_pinned(s, xs, ps, t, kx, kp);
+    else stage(s, xs, ps, t, kx, kp);
+}
+
+// psi of the state (x, pot) by the simulation's own formulation
+void sim_psi(bemhip_sim* s, const double* x, const double* pot, double* rhs, double* psi, bemhip_solve_stats* st) {
+    if (s->n_pin > 0) psi_pinned(s, x, pot, rhs, psi, st);
+    else psi_device(s->ctx, s->topo, x, pot, s->P.flags, s->P.solver, s->P.tol, s->P.maxit, rhs, psi, st);
+}
+
 double get_dt(bemhip_sim* s, double dp, const double* kx, const double* kp) {  // Simulation.cpp:77-92
     double m[2];
     dt_reduce(s->ctx, s->N, kx, kp, m);
@@ -479,6 +549,61 @@ int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, co
     return BEMHIP_OK;
 }
 
+int bemhip_sim_create_pinned(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, const double* xyz, int64_t M,
+                             const uint32_t* tri, const double* phi, const double* psi, bemhip_sim** out,
+                             int64_t* npin) {
+    if (!ctx || !out || !p || !xyz || !tri || N <= 0 || M <= 0) return BEMHIP_ERR_ARG;
+    *out = nullptr;
+    if (p->flags & BEMHIP_INDEPENDENT_PARTS) {
+        ctx->err = "pinned vertices need the coupled problem";
+        return BEMHIP_ERR_ARG;
+    }
+    // Simulation(initial): V_0 from the mesh as given (Simulation.hpp:42)
+    bemhip_sim* s = nullptr;
+    int rc = bemhip_sim_create(ctx, p, N, xyz, M, tri, &s);
+    if (rc != BEMHIP_OK) return rc;
+    rc = guarded(ctx, [&] {
+        std::vector<double> x(xyz, xyz + 3 * N), ph(N, 0.0), ps(N, 0.0);
+        std::vector<uint32_t> t(tri, tri + 3 * M);
+        if (phi) { check_finite(phi, N, "phi"); ph.assign(phi, phi + N); }
+        if (psi) { check_finite(psi, N, "psi"); ps.assign(psi, psi + N); }
+        int64_t np = 0;
+        const int r = bemhip_pin_boundary(N, x.data(), M, t.data(), ph.data(), ps.data(), &np);
+        BH_CHECK(r == BEMHIP_OK, r, "bad mesh");
+        const double time = s->time;
+        const int r2 = bemhip_sim_set_mesh(s, N, x.data(), M, t.data());
+        BH_CHECK(r2 == BEMHIP_OK, r2, ctx->err);
+        h2d(s->phi.as<double>(), ph.data(), sizeof(double) * N, ctx->stream);
+        h2d(s->psi.as<double>(), ps.data(), sizeof(double) * N, ctx->stream);
+        BH_HIP(hipStreamSynchronize(ctx->stream));   // the host vectors end with this scope
+        s->time = time;
+        s->n_pin = np;
+        if (npin) *npin = np;
+    });
+    if (rc != BEMHIP_OK) {
+        bemhip_sim_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_set_pinned(bemhip_sim* s, int64_t npin) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(npin >= 0 && npin <= s->N, BEMHIP_ERR_ARG, "pinned count out of range");
+        BH_CHECK(npin == 0 || !(s->P.flags & BEMHIP_INDEPENDENT_PARTS), BEMHIP_ERR_ARG,
+                 "pinned vertices need the coupled problem");
+        s->n_pin = npin;
+    });
+}
+
+int bemhip_sim_get_pinned(bemhip_sim* s, int64_t* npin) {
+    if (!s || !npin) return BEMHIP_ERR_ARG;
+    *npin = s->n_pin;
+    return BEMHIP_OK;
+}
+
 int bemhip_sim_destroy(bemhip_sim* s) {
     if (!s) return BEMHIP_OK;
     (void)hipStreamSynchronize(s->ctx->stream);
@@ -584,6 +709,7 @@ int bemhip_sim_set_mesh(bemhip_sim* s, int64_t N, const double* xyz, int64_t M, 
     return guarded(s->ctx, [&] {
         BH_CHECK(xyz && tri, BEMHIP_ERR_ARG, "null argument");
         bemhip_ctx* ctx = s->ctx;
+        BH_CHECK(s->n_pin <= N, BEMHIP_ERR_ARG, "the new mesh has fewer vertices than are pinned");
         Topology th = build_topology(N, xyz, M, tri, (s->P.flags & BEMHIP_INDEPENDENT_PARTS) != 0);
         const int old_parts = s->topo.nparts;
         s->th = std::move(th);
@@ -614,6 +740,22 @@ int bemhip_sim_get_state(bemhip_sim* s, double* xyz, double* phi, double* psi, d
     });
 }
 
+int bemhip_sim_get_triangles(bemhip_sim* s, uint32_t* tri) {
+    if (!s || !tri) return BEMHIP_ERR_ARG;
+    std::copy(s->th.tri.begin(), s->th.tri.end(), tri);
+    return BEMHIP_OK;
+}
+
+int bemhip_sim_export_ply(bemhip_sim* s, const char* path, int32_t precision) {
+    if (!s || !path) return BEMHIP_ERR_ARG;
+    std::vector<double> x(3 * s->N), ph(s->N), ps(s->N);
+    int rc = bemhip_sim_get_state(s, x.data(), ph.data(), ps.data(), nullptr);
+    if (rc != BEMHIP_OK) return rc;
+    rc = bemhip_ply_write(path, s->N, x.data(), s->M, s->th.tri.data(), ph.data(), ps.data(), precision);
+    if (rc != BEMHIP_OK) s->ctx->err = bemhip_io_last_error();
+    return rc;
+}
+
 int bemhip_sim_get_volume(bemhip_sim* s, double* vol) {
     if (!s || !vol) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {
@@ -628,7 +770,10 @@ int bemhip_sim_position_t(bemhip_sim* s, double* vel, double* psi) {
     if (!s) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {
         double* kx = dev<double>(s->kx, 12 * s->N);
-        stage(s, s->x.as<double>(), s->phi.as<double>(), s->time, kx, nullptr);
+        // on a copy of phi: the pinned stage rewrites its potential
+        double* ps = dev<double>(s->ps, s->N);
+        BH_HIP(hipMemcpyAsync(ps, s->phi.p, sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->ctx->stream));
+        stage_step(s, s->x.as<double>(), ps, s->time, kx, nullptr);
         if (vel) d2h(vel, kx, sizeof(double) * 3 * s->N, s->ctx->stream);
         if (psi) d2h(psi, s->psil.p, sizeof(double) * s->N, s->ctx->stream);
         BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
@@ -664,8 +809,7 @@ int bemhip_sim_compute_psi(bemhip_sim* s) {
     if (!s) return BEMHIP_ERR_ARG;
     return guarded(s->ctx, [&] {
         double* hphi = dev<double>(s->hphi, s->N);
-        psi_device(s->ctx, s->topo, s->x.as<double>(), s->phi.as<double>(), s->P.flags, s->P.solver, s->P.tol,
-                   s->P.maxit, hphi, s->psi.as<double>(), &s->last);
+        sim_psi(s, s->x.as<double>(), s->phi.as<double>(), hphi, s->psi.as<double>(), &s->last);
         accumulate(s->accum, s->last);
         BH_HIP(hipStreamSynchronize(s->ctx->stream));
         BH_CHECK(s->last.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
@@ -712,8 +856,7 @@ int bemhip_sim_exterior_pot(bemhip_sim* s, int64_t P, const double* points, doub
         double* hphi = dev<double>(s->hphi, N);
         double* psi = dev<double>(ctx->ext_psi, N);
         bemhip_solve_stats st{};
-        psi_device(ctx, s->topo, s->x.as<double>(), s->phi.as<double>(), s->P.flags, s->P.solver, s->P.tol,
-                   s->P.maxit, hphi, psi, &st);
+        sim_psi(s, s->x.as<double>(), s->phi.as<double>(), hphi, psi, &st);
         BH_CHECK(st.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance");
         double* dtg = dev<double>(ctx->ext_tgt, 3 * P);
         double* dout = dev<double>(ctx->ext_out, P);
@@ -744,7 +887,7 @@ int bemhip_sim_evolve_rk4(bemhip_sim* s, double dp, int fixdt, double* dt_out) {
         BH_HIP(hipMemcpyAsync(p1, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
         const double t0 = s->time;
         bool conv = true;
-        stage(s, x1, p1, t0, kx, kp);
+        stage_step(s, x1, p1, t0, kx, kp);
         conv = conv && s->last.converged;
         const double dt = fixdt ? dp : get_dt(s, dp, kx, kp);
         const bool frozen = s->P.frozen_stage_time != 0;
@@ -752,10 +895,13 @@ int bemhip_sim_evolve_rk4(bemhip_sim* s, double dp, int fixdt, double* dt_out) {
         const double ts[3] = {frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + 0.5 * dt, frozen ? t0 : t0 + dt};
         for (int k = 1; k < 4; ++k) {
             axpy_state(ctx, N, cs[k - 1], kx + (k - 1) * n3, x1, xs, kp + (size_t)(k - 1) * N, p1, ps);
-            stage(s, xs, ps, ts[k - 1], kx + k * n3, kp + (size_t)k * N);
+            stage_step(s, xs, ps, ts[k - 1], kx + k * n3, kp + (size_t)k * N);
             conv = conv && s->last.converged;
         }
         double* avg = dev<double>(s->avg, n3);
+        // pf = p1 + dt/6 (...) (LinLinSim.cpp:238) with p1 as the first stage left
+        // it: a pinned stage rewrote its pinned entries (ColocSimPin.cpp:186)
+        pin_copy_tail(ctx, N, s->n_pin, p1, phi);
         rk4_final(ctx, N, dt, kx, kp, x1, x, phi, avg, s->P.wall_guards, s->P.wall_eps);
         double* vn = dev<double>(s->vn, n3);
         vertex_normals(ctx, s->topo, x, vn);
@@ -776,7 +922,7 @@ int bemhip_sim_evolve_euler(bemhip_sim* s, double dp, int fixdt, double* dt_out)
         const int64_t N = s->N;
         double* kx = dev<double>(s->kx, 12 * (size_t)N);
         double* kp = dev<double>(s->kp, 4 * (size_t)N);
-        stage(s, s->x.as<double>(), s->phi.as<double>(), s->time, kx, kp);
+        stage_step(s, s->x.as<double>(), s->phi.as<double>(), s->time, kx, kp);
         const double dt = fixdt ? dp : get_dt(s, dp, kx, kp);
         euler_update(ctx, N, dt, kx, kp, s->x.as<double>(), s->phi.as<double>());
         BH_HIP(hipStreamSynchronize(ctx->stream));
@@ -852,6 +998,7 @@ int bemhip_sim_adjoint_radius(bemhip_sim* s, int32_t nsteps, double dt, const do
         BH_CHECK(!(s->P.flags & (BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)) && !s->P.wall_guards, BEMHIP_ERR_ARG,
                  "the adjoint supports the free-space, coupled problem only");
         BH_CHECK(s->P.solver != BEMHIP_SOLVE_BICGSTAB64, BEMHIP_ERR_ARG, "the adjoint needs an LU-based solver");
+        BH_CHECK(s->n_pin == 0, BEMHIP_ERR_ARG, "the adjoint does not support pinned vertices");
         bemhip_ctx* ctx = s->ctx;
         hipStream_t st = ctx->stream;
         const int64_t N = s->N;

@@ -262,6 +262,13 @@ void euler_update(bemhip_ctx* ctx, int64_t N, double dt, const double* vel, cons
                   double* x, double* phi);
 void dt_reduce(bemhip_ctx* ctx, int64_t N, const double* vel, const double* dpot, double* out2);
 int any_negative_x(bemhip_ctx* ctx, int64_t N, const double* xyz);
+// pinned contact line (ColocSimPin): the last npin vertices
+void pin_copy_tail(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* src, double* dst);
+void pin_zero_tail(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* pot, double* pz);
+void pin_swap_columns(bemhip_ctx* ctx, int64_t N, int64_t npin, double* G, float* G32, const double* H,
+                      const double* pot, double* rhs);
+void pin_drift(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* vn, const double* psil, double* vel,
+               double* pot);
 
 // ----------------------------------------------------------- adjoint --
 // forward quantities of one stage that its reverse sweep reads

@@ -283,7 +283,91 @@ __global__ __launch_bounds__(256) void k_maxes(int N, const double* __restrict__
     if (threadIdx.x == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = s2[0]; }
 }
 
+// ---- pinned contact line (ColocSimPin) ----
+// The last N - first vertices are pinned on the wall x = 0.
+
+// dst[i] = src[i] for the pinned vertices
+__global__ __launch_bounds__(256) void k_pin_copy_tail(int N, int first, const double* __restrict__ src,
+                                                       double* __restrict__ dst) {
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) dst[i] = src[i];
+}
+
+// pz = pot with the pinned entries zeroed: the fused H pz is then the sum over
+// the free columns only
+__global__ __launch_bounds__(256) void k_pin_zero_tail(int N, int first, const double* __restrict__ pot,
+                                                       double* __restrict__ pz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) pz[i] = i < first ? pot[i] : 0.0;
+}
+
+// ColocSimPin::assemble_matrices (ColocSimPin.cpp:113-117) applied to the
+// system G x = H pot: pinned column j of G becomes -H(:, j) (also in the FP32
+// copy) and, with H(:, j) = -G_old(:, j), rhs_i gains -sum_j G_old(i, j) pot_j.
+// One thread per row; the column loop reads consecutive rows (coalesced).
+__global__ __launch_bounds__(256) void k_pin_swap(int N, int first, double* __restrict__ G, float* __restrict__ G32,
+                                                  const double* __restrict__ H, const double* __restrict__ pot,
+                                                  double* __restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double acc = rhs[i];
+    for (int j = first; j < N; ++j) {
+        const size_t e = (size_t)j * N + i;
+        const double g = G[e], h = H[e];
+        G[e] = -h;
+        if (G32) G32[e] = (float)(-h);
+        acc += -g * pot[j];
+    }
+    rhs[i] = acc;
+}
+
+// ColocSimPin::position_t (ColocSimPin.cpp:176-188): a pinned vertex moves
+// along the wall, in the direction of its normal with the x part removed, at
+// the drift velocity of cos(theta) = n.x (ColocSimPin.cpp:126-136:
+// beta (cos_eq - c) + c_nl c^3 / (c^2 - 1)); its potential entry takes the
+// solved value.
+__global__ __launch_bounds__(256) void k_pin_drift(int N, int first, double beta, double cos_eq, double c_nl,
+                                                   const double* __restrict__ vn, const double* __restrict__ psil,
+                                                   double* __restrict__ vel, double* __restrict__ pot) {
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    d3 n = ld3(vn + 3 * (size_t)i);
+    const double c = n.x;
+    n.x = 0.0;
+    n = scl(1.0 / sqrt(dot(n, n)), n);
+    const double drift = beta * (cos_eq - c) + c_nl * (c * c * c) / (c * c - 1.0);
+    n = scl(drift, n);
+    vel[3 * (size_t)i] = n.x;
+    vel[3 * (size_t)i + 1] = n.y;
+    vel[3 * (size_t)i + 2] = n.z;
+    pot[i] = psil[i];
+}
+
 static inline int nb(int64_t n) { return (int)((n + 255) / 256); }
+
+void pin_copy_tail(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* src, double* dst) {
+    if (npin <= 0) return;
+    k_pin_copy_tail<<<nb(npin), 256, 0, ctx->stream>>>((int)N, (int)(N - npin), src, dst);
+    BH_HIP(hipGetLastError());
+}
+void pin_zero_tail(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* pot, double* pz) {
+    k_pin_zero_tail<<<nb(N), 256, 0, ctx->stream>>>((int)N, (int)(N - npin), pot, pz);
+    BH_HIP(hipGetLastError());
+}
+void pin_swap_columns(bemhip_ctx* ctx, int64_t N, int64_t npin, double* G, float* G32, const double* H,
+                      const double* pot, double* rhs) {
+    if (npin <= 0) return;
+    k_pin_swap<<<nb(N), 256, 0, ctx->stream>>>((int)N, (int)(N - npin), G, G32, H, pot, rhs);
+    BH_HIP(hipGetLastError());
+}
+void pin_drift(bemhip_ctx* ctx, int64_t N, int64_t npin, const double* vn, const double* psil, double* vel,
+               double* pot) {
+    if (npin <= 0) return;
+    // the reference's constants (ColocSimPin.cpp:126-130); cos(130 deg) on the host
+    const double beta = 0.2, cos_eq = std::cos(130.0 / 180.0 * PI), c_nl = 0.01;
+    k_pin_drift<<<nb(npin), 256, 0, ctx->stream>>>((int)N, (int)(N - npin), beta, cos_eq, c_nl, vn, psil, vel, pot);
+    BH_HIP(hipGetLastError());
+}
 
 void tri_geometry(bemhip_ctx* ctx, const DevTopology& topo, const double* xyz, const double* pot, double* tnorm,
                   double* tgrad, double* tvol) {

@@ -291,3 +291,36 @@ void DevTopology::upload(const Topology& t, hipStream_t s) {
 }
 
 }  // namespace bemhip
+
+// ColocSimPin::set_x_boundary (ColocSimPin.cpp:222-287): a vertex with
+// |x| < 1e-5 is put on the wall (x = 0) and swapped with the last vertex not
+// yet visited, which is looked at next; triangles are relabelled through the
+// inverse permutation, and phi / psi (either may be null) move with their
+// vertices.  The visiting order fixes the resulting vertex order, which the
+// pinned columns and every per-vertex array depend on.
+extern "C" int bemhip_pin_boundary(int64_t N, double* xyz, int64_t M, uint32_t* tri, double* phi, double* psi,
+                                   int64_t* npin) {
+    if (N <= 0 || M < 0 || !xyz || (M > 0 && !tri) || !npin) return BEMHIP_ERR_ARG;
+    for (int64_t e = 0; e < 3 * M; ++e)
+        if (tri[e] >= (uint64_t)N) return BEMHIP_ERR_BAD_MESH;
+    std::vector<int64_t> perm(N), inv(N);
+    std::iota(perm.begin(), perm.end(), 0);
+    int64_t k = N, pinned = 0;
+    for (int64_t i = 0; i < k; ++i) {
+        if (std::fabs(xyz[3 * i]) < 1e-5) {
+            xyz[3 * i] = 0.0;
+            const int64_t l = k - 1;
+            std::swap(perm[i], perm[l]);
+            for (int d = 0; d < 3; ++d) std::swap(xyz[3 * i + d], xyz[3 * l + d]);
+            if (phi) std::swap(phi[i], phi[l]);
+            if (psi) std::swap(psi[i], psi[l]);
+            --i;
+            --k;
+            ++pinned;
+        }
+    }
+    for (int64_t i = 0; i < N; ++i) inv[perm[i]] = i;
+    for (int64_t e = 0; e < 3 * M; ++e) tri[e] = (uint32_t)inv[tri[e]];
+    *npin = pinned;
+    return BEMHIP_OK;
+}

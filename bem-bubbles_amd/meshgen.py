@@ -97,6 +97,49 @@ def icosphere(nu: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
     return np.ascontiguousarray(v), np.ascontiguousarray(t)
 
 
+def hemisphere(nu: int, radius: float = 1.0):
+    """Geodesic hemisphere x >= 0 on the wall x = 0: the four faces of the
+    octahedron around +x, each split into nu^2 triangles and projected on the
+    sphere.  Its open rim (4 nu vertices) lies exactly in the plane x = 0, so the
+    wall image closes it into a sphere.  Returns (verts, tris), outward faces."""
+    if nu < 1:
+        raise ValueError("nu must be >= 1")
+    apex = np.array([1.0, 0.0, 0.0])
+    rim = [np.array(p, dtype=np.float64) for p in ((0, 1, 0), (0, 0, 1), (0, -1, 0), (0, 0, -1))]
+    index, verts, tris = {}, [], []
+
+    def vid(pos):
+        key = tuple(np.round(pos * 2 * nu).astype(int))
+        j = index.get(key)
+        if j is None:
+            j = len(verts)
+            index[key] = j
+            verts.append(pos)
+        return j
+
+    for f in range(4):
+        A, B, Cc = apex, rim[f], rim[(f + 1) % 4]
+        lat = {}
+        for i in range(nu + 1):
+            for j in range(nu + 1 - i):
+                lat[(i, j)] = vid(((nu - i - j) * A + i * B + j * Cc) / nu)
+        for i in range(nu):
+            for j in range(nu - i):
+                tris.append((lat[(i, j)], lat[(i + 1, j)], lat[(i, j + 1)]))
+                if i + j + 1 < nu:
+                    tris.append((lat[(i + 1, j)], lat[(i + 1, j + 1)], lat[(i, j + 1)]))
+    v = np.array(verts, dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v *= float(radius)
+    t = np.array(tris, dtype=np.uint32)
+    # outward orientation
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    t[flip] = t[flip][:, [0, 2, 1]]
+    assert v.shape[0] == 2 * nu * nu + 2 * nu + 1 and t.shape[0] == 4 * nu * nu
+    return np.ascontiguousarray(v), np.ascontiguousarray(t)
+
+
 def join_meshes(parts):
     """Concatenate meshes (Mesh.cpp join_meshes semantics: offsets added)."""
     vs, ts, off = [], [], 0

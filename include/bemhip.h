@@ -34,7 +34,8 @@ typedef enum {
     BEMHIP_ERR_NAN = 6,             /* non-finite value in inputs or results            */
     BEMHIP_ERR_OOM = 7,             /* device allocation failed                         */
     BEMHIP_ERR_HIP = 8,             /* HIP / rocBLAS runtime error                      */
-    BEMHIP_ERR_NO_DEVICE = 9        /* no usable gfx950 device                          */
+    BEMHIP_ERR_NO_DEVICE = 9,       /* no usable gfx950 device                          */
+    BEMHIP_ERR_IO = 10              /* file missing, unreadable or not a supported PLY  */
 } bemhip_status;
 
 /* assembly flags */
@@ -138,6 +139,11 @@ int bemhip_sim_set_params(bemhip_sim* sim, const bemhip_sim_params* p);
 int bemhip_sim_set_mesh(bemhip_sim* sim, int64_t N, const double* xyz, int64_t M, const uint32_t* tri);
 int bemhip_sim_get_state(bemhip_sim* sim, double* xyz, double* phi, double* psi, double* time);
 int bemhip_sim_get_volume(bemhip_sim* sim, double* vol);               /* Simulation.hpp:128-130 */
+/* the triangles in the simulation's vertex numbering (the public mesh.trigs) */
+int bemhip_sim_get_triangles(bemhip_sim* sim, uint32_t* tri);
+/* Simulation::export_mesh (Simulation.cpp:94-96; float, precision 4) and its
+ * lossless double variant (precision 8): see bemhip_ply_write */
+int bemhip_sim_export_ply(bemhip_sim* sim, const char* path, int32_t precision);
 /* number of loose parts (split_by_loose_parts, Mesh.cpp:382-440) and their
  * signed volumes, part p = the part holding the smallest vertex id not in
  * parts 0..p-1 */
@@ -184,6 +190,50 @@ int bemhip_exterior_potential(bemhip_ctx* ctx, int64_t N, const double* xyz, int
  * and phi (the simulation's solver and mirror flag; the simulation's psi is
  * left unchanged), then the potential at the P points from (phi, psi). */
 int bemhip_sim_exterior_pot(bemhip_sim* sim, int64_t P, const double* points, double* out);
+
+/* ---- pinned contact line (ColocSimPin, ColocSimPin.hpp:13-68) -----------
+ * A bubble sitting on the wall x = 0 whose contact-line vertices are pinned to
+ * the wall: they are the last n_pin vertices, psi is prescribed there and phi
+ * is solved for (the pinned columns of G and H trade places after the 4-pi
+ * rule, ColocSimPin.cpp:113-117), and they move along the wall with the drift
+ * law of the contact angle (ColocSimPin.cpp:126-136, 176-188).  Wall mode
+ * (BEMHIP_MIRROR_X0) is what the reference compiles it with.
+ *
+ * bemhip_pin_boundary: ColocSimPin::set_x_boundary (ColocSimPin.cpp:222-287)
+ * on host arrays, in place: vertices with |x| < 1e-5 get x = 0 and move to the
+ * end, triangles are relabelled, phi / psi (may be null) move along; *npin
+ * receives their number.  Needs no device. */
+int bemhip_pin_boundary(int64_t N, double* xyz, int64_t M, uint32_t* tri, double* phi, double* psi,
+                        int64_t* npin);
+/* the two ColocSimPin constructors (ColocSimPin.hpp:17-52): the mesh (and phi,
+ * psi when non-null; zero otherwise) reordered by bemhip_pin_boundary, V_0
+ * (when NaN in p) from the mesh as given, before the reordering; *npin (may be
+ * null) receives the pinned count.  Not with BEMHIP_INDEPENDENT_PARTS. */
+int bemhip_sim_create_pinned(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, const double* xyz,
+                             int64_t M, const uint32_t* tri, const double* phi, const double* psi,
+                             bemhip_sim** out, int64_t* npin);
+/* the public member N_pin (ColocSimPin.hpp:57): set after bemhip_sim_set_mesh
+ * with a mesh whose last npin vertices are the pinned ones (0 = ColocSim) */
+int bemhip_sim_set_pinned(bemhip_sim* sim, int64_t npin);
+int bemhip_sim_get_pinned(bemhip_sim* sim, int64_t* npin);
+
+/* ---- state files: binary little-endian PLY (MeshIO.cpp) ------------------
+ * Need no device and no context; on failure bemhip_io_last_error() (per host
+ * thread) holds the message.
+ * bemhip_ply_write: export_ply_double (precision 8) / export_ply_float
+ * (precision 4) with phi and psi (MeshIO.cpp:263-335), with one value named w
+ * when psi is null (MeshIO.cpp:191-261), or the bare mesh when both are null
+ * (MeshIO.cpp:67-100).  Simulation::export_mesh (Simulation.cpp:94-96) writes
+ * precision 4, which rounds the state; precision 8 restores it exactly.
+ * bemhip_ply_info / bemhip_ply_read: import_ply (MeshIO.cpp:701-903): counts,
+ * the number of per-vertex values (0, 1 = phi, 2 = phi and psi) and their
+ * precision, then the data (phi / psi may be null; missing values read as 0). */
+const char* bemhip_io_last_error(void);
+int bemhip_ply_write(const char* path, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                     const double* phi, const double* psi, int32_t precision);
+int bemhip_ply_info(const char* path, int64_t* N, int64_t* M, int32_t* nvalues, int32_t* precision);
+int bemhip_ply_read(const char* path, int64_t N, double* xyz, int64_t M, uint32_t* tri, double* phi,
+                    double* psi);
 
 /* ---- adjoint of a radius trajectory (no counterpart in the reference) ----
  * From the simulation's current state (x0, phi0, time), runs `nsteps` RK4

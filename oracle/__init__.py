@@ -28,7 +28,7 @@ class Params(C.Structure):
         ("field_amp", C.c_double), ("field_freq", C.c_double), ("field_phase", C.c_double),
         ("dp_balance", C.c_double), ("min_dt", C.c_double), ("wall_eps", C.c_double),
         ("mirror", C.c_int), ("wall_guards", C.c_int), ("frozen_stage_time", C.c_int),
-        ("solver", C.c_int), ("nthreads", C.c_int),
+        ("solver", C.c_int), ("nthreads", C.c_int), ("n_pin", C.c_int64),
     ]
 
 
@@ -66,8 +66,12 @@ def lib():
         L.bo_rk4_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
                                   C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.bo_rk4_step.restype = C.c_int
-        L.bo_euler_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p,
+        L.bo_euler_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
                                     C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.bo_pin_boundary.argtypes = [C.c_int64, f64p, C.c_int64, u32p, C.c_void_p, C.c_void_p]
+        L.bo_pin_boundary.restype = C.c_int64
+        L.bo_drift_velocity.argtypes = [C.c_double]
+        L.bo_drift_velocity.restype = C.c_double
         L.bo_euler_step.restype = C.c_int
         L.bo_compute_psi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
         L.bo_quadrature19.argtypes = [f64p] * 3
@@ -210,7 +214,8 @@ def part_volumes(verts, tris):
 
 
 def make_params(physics=None, *, V_0=None, mirror=False, wall_guards=False, frozen_stage_time=True,
-                solver="lu", nthreads=0, dp_balance=3.0, min_dt=-1.0, wall_eps=1e-2, verts=None, tris=None):
+                solver="lu", nthreads=0, dp_balance=3.0, min_dt=-1.0, wall_eps=1e-2, verts=None, tris=None,
+                n_pin=0):
     p = Params()
     ph = physics
     p.p_inf = getattr(ph, "p_inf", 1.0)
@@ -229,7 +234,23 @@ def make_params(physics=None, *, V_0=None, mirror=False, wall_guards=False, froz
     p.mirror, p.wall_guards, p.frozen_stage_time = int(mirror), int(wall_guards), int(frozen_stage_time)
     p.solver = {"lu": 0, "bicgstab": 1}[solver]
     p.nthreads = nthreads
+    p.n_pin = n_pin
     return p
+
+
+def pin_boundary(verts, tris, phi=None, psi=None):
+    """ColocSimPin::set_x_boundary: reordered copies (verts, tris, phi, psi) and the pinned count"""
+    v = np.array(verts, dtype=np.float64, copy=True)
+    t = np.array(tris, dtype=np.uint32, copy=True)
+    ph = None if phi is None else np.array(phi, dtype=np.float64, copy=True)
+    ps = None if psi is None else np.array(psi, dtype=np.float64, copy=True)
+    n = lib().bo_pin_boundary(v.shape[0], v.reshape(-1), t.shape[0], t.reshape(-1),
+                              None if ph is None else ph.ctypes.data, None if ps is None else ps.ctypes.data)
+    return v, t, ph, ps, int(n)
+
+
+def drift_velocity(costheta):
+    return lib().bo_drift_velocity(float(costheta))
 
 
 def compute_psi(params, verts, tris, phi):
@@ -281,6 +302,6 @@ class Sim:
         tm = C.c_double(self.time)
         dt = C.c_double(0.0)
         lib().bo_euler_step(C.byref(self.p), self.x.shape[0], self.x.reshape(-1), self.t.shape[0],
-                            self.t.reshape(-1), self.phi, C.byref(tm), float(dp), int(fixdt), C.byref(dt))
+                            self.t.reshape(-1), self.phi, self.psi, C.byref(tm), float(dp), int(fixdt), C.byref(dt))
         self.time = tm.value
         return dt.value

@@ -699,6 +699,18 @@ static void psi_of(const bo_params* P, int64_t N, const double* xyz, int64_t M,
     double* H = (double*)malloc(sizeof(double) * (size_t)N * N);
     double* b = (double*)malloc(sizeof(double) * N);
     bo_assemble(N, xyz, M, tri, P->mirror, G, H, P->nthreads);
+    /* ColocSimPin::assemble_matrices (ColocSimPin.cpp:113-117): the columns of
+     * the pinned vertices trade places, G(:, j) = -H(:, j), H(:, j) = -G(:, j)
+     * (both column-major) */
+    for (int64_t j = N - P->n_pin; j < N; ++j) {
+        double* g = G + (size_t)j * N;
+        double* h = H + (size_t)j * N;
+        for (int64_t i = 0; i < N; ++i) {
+            const double t = g[i];
+            g[i] = -h[i];
+            h[i] = -t;
+        }
+    }
     bo_gemv(N, H, pot, b, P->nthreads);  /* H * phi (LinLinSim.cpp:32) */
     free(H);
     solve_dense(P, N, G, b, psi);
@@ -710,14 +722,49 @@ void bo_compute_psi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
     psi_of(P, N, xyz, M, tri, phi, psi);
 }
 
-void bo_position_t(const bo_params* P, int64_t N, const double* xyz, int64_t M,
-                   const uint32_t* tri, const double* pot, double* vel, double* psi_out) {
+int64_t bo_pin_boundary(int64_t N, double* xyz, int64_t M, uint32_t* tri, double* phi, double* psi) {
+    int64_t* perm = (int64_t*)malloc(sizeof(int64_t) * N);
+    int64_t* inv = (int64_t*)malloc(sizeof(int64_t) * N);
+    for (int64_t i = 0; i < N; ++i) perm[i] = i;
+    int64_t k = N, npin = 0;
+    for (int64_t i = 0; i < k; ++i) {
+        if (fabs(xyz[3 * i]) < 1e-5) {
+            xyz[3 * i] = 0.0;
+            const int64_t l = k - 1;
+            int64_t tp = perm[i]; perm[i] = perm[l]; perm[l] = tp;
+            for (int d = 0; d < 3; ++d) { double t = xyz[3 * i + d]; xyz[3 * i + d] = xyz[3 * l + d]; xyz[3 * l + d] = t; }
+            if (phi) { double t = phi[i]; phi[i] = phi[l]; phi[l] = t; }
+            if (psi) { double t = psi[i]; psi[i] = psi[l]; psi[l] = t; }
+            --i;    /* look at the vertex swapped in */
+            --k;
+            ++npin;
+        }
+    }
+    for (int64_t i = 0; i < N; ++i) inv[perm[i]] = i;
+    for (int64_t e = 0; e < 3 * M; ++e) tri[e] = (uint32_t)inv[tri[e]];
+    free(perm); free(inv);
+    return npin;
+}
+
+double bo_drift_velocity(double c) {
+    const double beta = 0.2, c_eq = cos(130.0 / 180.0 * M_PI), c_nl = 0.01;
+    return beta * (c_eq - c) + c_nl * (c * c * c) / (c * c - 1.0);
+}
+
+/* LinLinSim::position_t (LinLinSim.cpp:26-83) and, with n_pin > 0,
+ * ColocSimPin::position_t (ColocSimPin.cpp:139-194): pot's pinned entries
+ * take the state's psi for the solve and the solved potential afterwards
+ * (in place, as in the reference); the pinned vertices move with the drift
+ * law along the wall. */
+static void position_impl(const bo_params* P, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
+                          double* pot, const double* psi_state, double* vel, double* psi_out) {
     double* psi = (double*)malloc(sizeof(double) * N);
-    psi_of(P, N, xyz, M, tri, pot, psi);
     double* tn = (double*)malloc(sizeof(double) * 3 * M);
     double* tg = (double*)malloc(sizeof(double) * 3 * M);
     bo_triangle_normals(N, xyz, M, tri, tn);
     bo_tangent_gradients(N, xyz, M, tri, pot, tg);
+    for (int64_t i = N - P->n_pin; i < N; ++i) pot[i] = psi_state[i];
+    psi_of(P, N, xyz, M, tri, pot, psi);
     int64_t *ptr, *idx;
     incidence(N, M, tri, &ptr, &idx);
     for (int64_t i = 0; i < N; ++i) {   /* LinLinSim.cpp:48-59 */
@@ -731,8 +778,32 @@ void bo_position_t(const bo_params* P, int64_t N, const double* xyz, int64_t M,
         g = vscale(1.0 / num, g);
         vel[3 * i] = g.x; vel[3 * i + 1] = g.y; vel[3 * i + 2] = g.z;
     }
+    if (P->n_pin > 0) {
+        double* vn = (double*)malloc(sizeof(double) * 3 * N);
+        bo_vertex_normals(N, xyz, M, tri, vn);
+        for (int64_t i = N - P->n_pin; i < N; ++i) {
+            v3 n = V(vn + 3 * i);
+            const double c = n.x;
+            n.x = 0.0;
+            n = vscale(1.0 / vnorm(n), n);      /* tangential to the wall */
+            n = vscale(bo_drift_velocity(c), n);
+            vel[3 * i] = n.x; vel[3 * i + 1] = n.y; vel[3 * i + 2] = n.z;
+            pot[i] = psi[i];
+        }
+        free(vn);
+    }
     if (psi_out) for (int64_t i = 0; i < N; ++i) psi_out[i] = psi[i];
     free(psi); free(tn); free(tg); free(ptr); free(idx);
+}
+
+void bo_position_t(const bo_params* P, int64_t N, const double* xyz, int64_t M,
+                   const uint32_t* tri, const double* pot, double* vel, double* psi_out) {
+    bo_params Q = *P;
+    Q.n_pin = 0;
+    double* p = (double*)malloc(sizeof(double) * N);
+    memcpy(p, pot, sizeof(double) * N);
+    position_impl(&Q, N, xyz, M, tri, p, NULL, vel, psi_out);
+    free(p);
 }
 
 void bo_pot_t_multi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
@@ -788,7 +859,8 @@ int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uin
     const double t0 = *time;
     double dt;
     /* stage 1 */
-    bo_position_t(P, N, x1, M, tri, phi, kx[0], NULL);
+    /* phi is the stage potential p1, whose pinned entries position_t rewrites */
+    position_impl(P, N, x1, M, tri, phi, psi, kx[0], NULL);
     bo_pot_t_multi(P, N, x1, M, tri, kx[0], t0, kp[0]);
     dt = fixdt ? dp : get_dt(P, N, dp, kx[0], kp[0]);
     const double ts[4] = {t0, P->frozen_stage_time ? t0 : t0 + 0.5 * dt,
@@ -797,7 +869,7 @@ int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uin
     for (int s = 1; s < 4; ++s) {   /* LinLinSim.cpp:218-234 */
         for (size_t e = 0; e < n3; ++e) xs[e] = cs[s - 1] * kx[s - 1][e] + x1[e];
         for (int64_t i = 0; i < N; ++i) ps[i] = cs[s - 1] * kp[s - 1][i] + phi[i];
-        bo_position_t(P, N, xs, M, tri, ps, kx[s], NULL);
+        position_impl(P, N, xs, M, tri, ps, psi, kx[s], NULL);
         bo_pot_t_multi(P, N, xs, M, tri, kx[s], ts[s], kp[s]);
     }
     /* average = (k1 + 2 k2 + 2 k3 + k4) / 6 (LinLinSim.cpp:236) */
@@ -841,11 +913,11 @@ int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uin
 }
 
 int bo_euler_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uint32_t* tri,
-                  double* phi, double* time, double dp, int fixdt, double* dt_out) {
+                  double* phi, const double* psi, double* time, double dp, int fixdt, double* dt_out) {
     const size_t n3 = 3 * (size_t)N;
     double* vel = (double*)malloc(sizeof(double) * n3);
     double* dpot = (double*)malloc(sizeof(double) * N);
-    bo_position_t(P, N, xyz, M, tri, phi, vel, NULL);
+    position_impl(P, N, xyz, M, tri, phi, psi, vel, NULL);
     bo_pot_t_multi(P, N, xyz, M, tri, vel, *time, dpot);
     const double dt = fixdt ? dp : get_dt(P, N, dp, vel, dpot);
     for (size_t e = 0; e < n3; ++e) xyz[e] = vel[e] * dt + xyz[e];

@@ -84,7 +84,16 @@ typedef struct {
     int frozen_stage_time;                       /* 1 = reference (all stages at step start) */
     int solver;                                  /* 0 = PartialPivLU, 1 = BiCGSTAB */
     int nthreads;
+    int64_t n_pin;                               /* ColocSimPin: the last n_pin vertices are pinned */
 } bo_params;
+
+/* ColocSimPin::set_x_boundary (ColocSimPin.cpp:222-287): vertices with
+ * |x| < 1e-5 get x = 0 and move to the end (swap with the last unvisited
+ * vertex), triangles are relabelled; phi / psi (may be NULL) move with their
+ * vertices.  Returns the number of pinned vertices. */
+int64_t bo_pin_boundary(int64_t N, double* xyz, int64_t M, uint32_t* tri, double* phi, double* psi);
+/* the contact-line drift law (ColocSimPin.cpp:126-136) */
+double bo_drift_velocity(double costheta);
 
 /* LinLinSim::position_t (LinLinSim.cpp:26-83): vertex velocities (N x 3); psi
  * (N) receives the solved normal derivative. */
@@ -98,9 +107,10 @@ void bo_pot_t_multi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
  * (the reference throws std::out_of_range). *dt_out receives dt. */
 int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uint32_t* tri,
                 double* phi, double* psi, double* time, double dp, int fixdt, double* dt_out);
-/* LinLinSim::evolve_system (Euler, LinLinSim.cpp:260-284) */
+/* LinLinSim::evolve_system (Euler, LinLinSim.cpp:260-284); psi (the state's
+ * psi, read at pinned vertices only) may be NULL when n_pin = 0 */
 int bo_euler_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uint32_t* tri,
-                  double* phi, double* time, double dp, int fixdt, double* dt_out);
+                  double* phi, const double* psi, double* time, double dp, int fixdt, double* dt_out);
 /* Simulation::compute_psi (Simulation.hpp:73-77) */
 void bo_compute_psi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
                     const uint32_t* tri, const double* phi, double* psi);
