@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <exception>
 #include <mutex>
 #include <thread>
@@ -865,6 +866,55 @@ static void gemm_nn(rocblas_handle h, int m, int n, int k, const float* A, int l
                           C, ldc));
 }
 
+// U12 = W A12 in FP32 (W = L11^-1, k x k; A12 k x n with the matrix's leading
+// dimension).  For this thin shape (1024 x n x 1024) rocBLAS's default kernel
+// choice reaches 90-127 TFLOP/s on MI355X; one kernel of its own list for the
+// problem runs within 2 % of the fastest listed one at every n from 1024 to
+// 39000 (about 145 TFLOP/s).  It is used where rocBLAS lists it for the exact
+// problem (looked up once per shape), else rocBLAS chooses; the choice is
+// fixed (U12_SOLUTION_DEFAULT), so results do not depend on timings.
+
+static void gemm_u12(Solver* ctx, rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* B,
+                     int ldb, float* C, int ldc) {
+    const float one = 1.0f, zero = 0.0f;
+    const int sol = ctx->u12_solution;
+    if (sol != 0 && m > 0 && n > 0 && k > 0) {
+        const std::array<int, 6> key{m, n, k, lda, ldb, ldc};
+        auto it = ctx->u12_listed.find(key);
+        if (it == ctx->u12_listed.end()) {
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+            rocblas_int cnt = 0;
+            BH_BLAS(rocblas_gemm_ex_get_solutions(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
+                                                  rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
+                                                  rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
+                                                  rocblas_datatype_f32_r, rocblas_gemm_algo_solution_index, 0, nullptr,
+                                                  &cnt));
+            std::vector<rocblas_int> ids(std::max<rocblas_int>(cnt, 1));
+            BH_BLAS(rocblas_gemm_ex_get_solutions(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
+                                                  rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
+                                                  rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
+                                                  rocblas_datatype_f32_r, rocblas_gemm_algo_solution_index, 0,
+                                                  ids.data(), &cnt));
+#pragma clang diagnostic pop
+            const bool listed = std::find(ids.begin(), ids.begin() + cnt, sol) != ids.begin() + cnt;
+            it = ctx->u12_listed.emplace(key, listed).first;
+        }
+        if (it->second) {
+            BH_BLAS(rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
+                                    rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
+                                    rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc, rocblas_datatype_f32_r,
+                                    rocblas_gemm_algo_solution_index, sol, 0));
+            return;
+        }
+    }
+    gemm_nn(h, m, n, k, A, lda, B, ldb, C, ldc);
+}
+static void gemm_u12(Solver*, rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B,
+                     int ldb, double* C, int ldc) {
+    gemm_nn(h, m, n, k, A, lda, B, ldb, C, ldc);
+}
+
 // the triangle not referenced by an inverse set to zero (and a unit diagonal
 // written out), so the inverse can enter a full GEMM
 template <class T>
@@ -993,6 +1043,41 @@ __global__ __launch_bounds__(256) void k_split_b(int K, int cols, const float* _
     }
 }
 
+// Bcat as k_split_b, with the factorisation's fixed exponent eb; the block's
+// max |B| (float bits) goes to *log for the check after the factorisation
+__global__ __launch_bounds__(256) void k_split_b_fixed(int K, int cols, const float* __restrict__ B, int ldb, int eb,
+                                                       __half* __restrict__ out, unsigned int* __restrict__ log) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    float m = 0.0f;
+    if (r < K) {
+        const size_t K3 = 3 * (size_t)K;
+        const int c0 = blockIdx.y * 64, c1 = min(cols, c0 + 64);
+        for (int c = c0; c < c1; ++c) {
+            const float b = B[(size_t)c * ldb + r];
+            m = fmaxf(m, fabsf(b));
+            const float v = ldexpf(b, eb);
+            const __half h = __float2half_rn(v);
+            const __half l = __float2half_rn(v - __half2float(h));
+            out[(size_t)c * K3 + r] = h;
+            out[(size_t)c * K3 + K + r] = l;
+            out[(size_t)c * K3 + 2 * K + r] = h;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(log, __float_as_uint(m));   // NaN bits order above every number
+}
+
+// max |A(i, i)| over the diagonal (float bits)
+__global__ __launch_bounds__(256) void k_diag_absmax(int N, const float* __restrict__ A, unsigned int* __restrict__ out) {
+    float m = 0.0f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+        m = fmaxf(m, fabsf(A[(size_t)i * N + i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
 // fp16 splits of a panel's L21 (rows x k) into the step's A buffer (parity p)
 static void split_a(Solver* ctx, hipStream_t st, int rows, int k, const float* A, int lda, int p,
                     unsigned long long* maxl) {
@@ -1007,8 +1092,15 @@ static void split_a(Solver* ctx, hipStream_t st, int rows, int k, const float* A
 // word and the alpha pair, whose scale word the caller has zeroed.
 static void split_b(Solver* ctx, hipStream_t st, int k, int n, const float* B, int ldb, int col0, int slot) {
     if (n <= 0) return;
-    unsigned int* mx = ctx->split_mx.as<unsigned int>();
     __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
+    if (ctx->split_fixed_on) {
+        BH_CHECK(ctx->split_calls < ctx->split_nlog, BEMHIP_ERR_HIP, "split log overflow");
+        unsigned int* log = ctx->split_log.as<unsigned int>() + ctx->split_calls++;
+        k_split_b_fixed<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb, bcat, log);
+        BH_HIP(hipGetLastError());
+        return;
+    }
+    unsigned int* mx = ctx->split_mx.as<unsigned int>();
     float* ab = reinterpret_cast<float*>(mx + 4) + 2 * slot;
     k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot);
     k_split_b<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot, bcat, ab);
@@ -1020,6 +1112,16 @@ static void split_gemm(Solver* ctx, rocblas_handle h, int m, int n, int k, int c
                        int slot) {
     const __half* acat = ctx->split_a[p].as<__half>();
     const __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
+    if (ctx->split_fixed_on) {
+        // alpha known on the host: no device-pointer scalars (which make
+        // rocBLAS copy them back and wait for the stream)
+        const float alpha = -std::ldexp(1.0f, -(SPLIT_EA + ctx->split_eb)), beta = 1.0f;
+        BH_BLAS(rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, &alpha, acat,
+                                rocblas_datatype_f16_r, m, bcat, rocblas_datatype_f16_r, 3 * k, &beta, C,
+                                rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc, rocblas_datatype_f32_r,
+                                rocblas_gemm_algo_standard, 0, 0));
+        return;
+    }
     float* ab = reinterpret_cast<float*>(ctx->split_mx.as<unsigned int>() + 4) + 2 * slot;
     BH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_device));
     rocblas_status s = rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, 3 * k, ab,
@@ -1339,7 +1441,7 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
                 BH_HIP(hipGetLastError());
             }
             // the next panel's columns first, then hand them over
-            gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+            gemm_u12(ctx, main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
             if constexpr (std::is_same<T, float>::value)
                 if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
             trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
@@ -1347,7 +1449,7 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
             post(Job{k, jb, p ^ 1, false});
             if (rest > jb2) {
                 T* Uw = U + (size_t)jb2 * jb;
-                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+                gemm_u12(ctx, main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
                 if constexpr (std::is_same<T, float>::value)
                     if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
                 trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
@@ -1416,6 +1518,38 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         ctx->split_b.ensure(sizeof(__half) * 3 * nb * (size_t)N);
         ctx->split_mx.ensure(64);
     }
+    ctx->split_fixed_on = false;
+    ctx->split_redo = false;
+    ctx->split_calls = 0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (split16 && ctx->split_fixed) {
+            // one B scale for the whole factorisation, from max |diag A|: its
+            // max maps to [2^9, 2^10); a U12 block is covered while its max
+            // scales into [2^-8, 2^15].  The top is the fp16 range with a
+            // margin.  Below it the low halves lose bits relative to the
+            // block's own max, but their absolute error (2^-24 after scaling,
+            // ~2^-34 of max |diag A|) stays far under the FP32 rounding of the
+            // entries the update lands on; the floor only catches a matrix
+            // whose blocks all shrink by orders of magnitude.
+            const int nlog = 2 * (N / LU_NB + 2);
+            unsigned int* log = static_cast<unsigned int*>(ctx->split_log.ensure(sizeof(unsigned int) * (nlog + 1)));
+            k_fill_u32<<<1, 128, 0, ctx->stream>>>(log, nlog + 1, 0u);
+            k_diag_absmax<<<64, 256, 0, ctx->stream>>>(N, A, log + nlog);
+            BH_HIP(hipGetLastError());
+            unsigned int mb = 0;
+            BH_HIP(hipMemcpyAsync(&mb, log + nlog, sizeof(mb), hipMemcpyDeviceToHost, ctx->stream));
+            BH_HIP(hipStreamSynchronize(ctx->stream));
+            float dm;
+            std::memcpy(&dm, &mb, sizeof(dm));
+            int e = 0;
+            if (dm > 0.0f && std::isfinite(dm)) std::frexp(dm, &e);
+            if (dm > 0.0f && std::isfinite(dm) && e > -90 && e < 90) {
+                ctx->split_eb = 10 - e;
+                ctx->split_nlog = nlog;
+                ctx->split_fixed_on = true;
+            }
+        }
+    }
     const Lane main{ctx->stream, ctx->blas};
     const Lane side = ctx->lookahead ? Lane{ctx->stream2, ctx->blas2} : main;
     hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1], ev_copied = ctx->ev_la[3];
@@ -1455,7 +1589,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
             BH_HIP(hipGetLastError());
         }
         // the next panel's columns first, then hand them to the side stream
-        gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+        gemm_u12(ctx, main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
         if constexpr (std::is_same<T, float>::value)
             if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
         trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
@@ -1469,7 +1603,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         // the host had issued them all, and the main stream would idle meanwhile
         if (rest > jb2) {
             T* Uw = U + (size_t)jb2 * jb;
-            gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+            gemm_u12(ctx, main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
             if constexpr (std::is_same<T, float>::value)
                 if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
             trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
@@ -1504,8 +1638,19 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     }
     unsigned int flags[3] = {0, 0, 0};
     BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, main.s));
+    std::vector<unsigned int> blocks(ctx->split_fixed_on ? ctx->split_calls : 0);
+    if (!blocks.empty())
+        BH_HIP(hipMemcpyAsync(blocks.data(), ctx->split_log.p, sizeof(unsigned int) * blocks.size(),
+                              hipMemcpyDeviceToHost, main.s));
     BH_HIP(hipStreamSynchronize(main.s));
     collect_gemm_timing(ctx);
+    for (unsigned int b : blocks) {
+        float m;
+        std::memcpy(&m, &b, sizeof(m));
+        const float sc = std::ldexp(m, ctx->split_eb);
+        if (!(m == 0.0f || (sc >= 0x1p-8f && sc <= 32768.0f))) ctx->split_redo = true;   // NaN fails too
+    }
+    ctx->split_fixed_on = false;
     BH_CHECK(!(flags[1] & 1u), BEMHIP_ERR_HIP, "panel grid barrier timed out");
     if (flags[1] & 2u) return (int)flags[2];  // 1-based first zero pivot
     return 0;
@@ -1994,6 +2139,12 @@ static float elapsed(Solver* ctx, hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// FP32 copy of an FP64 matrix (n entries)
+__global__ void k_d2f_mat(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, double* x, int solver,
                   double tol, int maxit, bemhip_solve_stats* st, bool report_residual) {
     hipStream_t sm = ctx->stream;
@@ -2017,7 +2168,18 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         lu_solve<double>(ctx, N, G64, perm, x);
     } else if (solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64) {
         BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "mixed solve needs the FP32 copy of G");
-        const int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
+        int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
+        if (ctx->split_redo) {
+            // a U12 block fell outside the fixed scale's window: the FP32 copy
+            // again from G, and the factorisation with a scale per block
+            k_d2f_mat<<<4096, 256, 0, sm>>>((size_t)N * N, G64, G32);
+            BH_HIP(hipGetLastError());
+            const bool keep = ctx->split_fixed;
+            ctx->split_fixed = false;
+            info = lu_factor<float>(ctx, N, G32, ipiv, true);
+            ctx->split_fixed = keep;
+            ctx->split_redos += 1;
+        }
         BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
         int* perm = make_perm(ctx, N, ipiv);
         if (ctx->trsv_persistent) lu_prepare_solve<float>(ctx, N, G32);
@@ -2154,6 +2316,8 @@ void Solver::init(const Solver* cfg) {
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;
+        u12_solution = cfg->u12_solution;
+        split_fixed = cfg->split_fixed;
     }
     BH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BH_BLAS(rocblas_create_handle(&blas));

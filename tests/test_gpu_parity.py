@@ -104,6 +104,36 @@ def test_speculative_panels_equal_gepp(oracle_lib, monkeypatch):
     c3.close()
 
 
+def test_split_fixed_scale_and_redo(oracle_lib, monkeypatch):
+    """The split-fp16 updates use one B scale per factorisation (taken from
+    max |diag G|) while every U12 block fits its window.  On a BEM matrix the
+    result agrees with per-block scales to the refinement tolerance; on a matrix
+    whose U12 blocks are ~1000x its diagonal the check fails and the
+    factorisation is redone with per-block scales, which then gives exactly
+    the per-block result."""
+    say("split-fp16 fixed scale vs per-block scales")
+    v, t = meshgen.icosphere(12)
+    G, H = oracle_lib.assemble(v, t)
+    b = H @ (1.0 + 0.05 * meshgen.random_sh_field(v, 1, 4, 5))
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(2600, 2600))
+    A[np.diag_indices(2600)] *= 1e-3
+    bA = rng.normal(size=2600)
+    c1 = bemhip.Context(0)
+    xg1, _ = c1.solve(G, b, "lu16x3_ir64")
+    xa1, sa1 = c1.solve(A, bA, "lu16x3_ir64")
+    c1.close()
+    monkeypatch.setenv("BEMHIP_SPLIT_FIXED", "0")
+    c2 = bemhip.Context(0)
+    xg2, _ = c2.solve(G, b, "lu16x3_ir64")
+    xa2, sa2 = c2.solve(A, bA, "lu16x3_ir64")
+    c2.close()
+    ref = oracle_lib.solve(G, b)
+    assert rel(xg1, ref) < 1e-8 and rel(xg2, ref) < 1e-8
+    assert np.array_equal(xa1, xa2) and sa1["iterations"] == sa2["iterations"]
+    assert rel(xa1, np.linalg.solve(A, bA)) < 1e-8
+
+
 @pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu32_ir64", 1e-8), ("lu16x3_ir64", 1e-8),
                                         ("bicgstab64", 1e-10)])
 def test_solvers_on_assembled_g(ctx, oracle_lib, solver, tol):
