@@ -380,8 +380,6 @@ int bemhip_create(int device, bemhip_ctx** out) {
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
-        ctx->u12_solution = U12_SOLUTION_DEFAULT;
-        if (const char* v = std::getenv("BEMHIP_U12_SOLUTION")) ctx->u12_solution = std::atoi(v);
         const char* sf = std::getenv("BEMHIP_SPLIT_FIXED");
         if (sf && sf[0] == '0') ctx->split_fixed = false;
         ctx->init(nullptr);

@@ -866,55 +866,6 @@ static void gemm_nn(rocblas_handle h, int m, int n, int k, const float* A, int l
                           C, ldc));
 }
 
-// U12 = W A12 in FP32 (W = L11^-1, k x k; A12 k x n with the matrix's leading
-// dimension).  For this thin shape (1024 x n x 1024) rocBLAS's default kernel
-// choice reaches 90-127 TFLOP/s on MI355X; one kernel of its own list for the
-// problem runs within 2 % of the fastest listed one at every n from 1024 to
-// 39000 (about 145 TFLOP/s).  It is used where rocBLAS lists it for the exact
-// problem (looked up once per shape), else rocBLAS chooses; the choice is
-// fixed (U12_SOLUTION_DEFAULT), so results do not depend on timings.
-
-static void gemm_u12(Solver* ctx, rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* B,
-                     int ldb, float* C, int ldc) {
-    const float one = 1.0f, zero = 0.0f;
-    const int sol = ctx->u12_solution;
-    if (sol != 0 && m > 0 && n > 0 && k > 0) {
-        const std::array<int, 6> key{m, n, k, lda, ldb, ldc};
-        auto it = ctx->u12_listed.find(key);
-        if (it == ctx->u12_listed.end()) {
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Wdeprecated-declarations"
-            rocblas_int cnt = 0;
-            BH_BLAS(rocblas_gemm_ex_get_solutions(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                                                  rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
-                                                  rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
-                                                  rocblas_datatype_f32_r, rocblas_gemm_algo_solution_index, 0, nullptr,
-                                                  &cnt));
-            std::vector<rocblas_int> ids(std::max<rocblas_int>(cnt, 1));
-            BH_BLAS(rocblas_gemm_ex_get_solutions(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                                                  rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
-                                                  rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc,
-                                                  rocblas_datatype_f32_r, rocblas_gemm_algo_solution_index, 0,
-                                                  ids.data(), &cnt));
-#pragma clang diagnostic pop
-            const bool listed = std::find(ids.begin(), ids.begin() + cnt, sol) != ids.begin() + cnt;
-            it = ctx->u12_listed.emplace(key, listed).first;
-        }
-        if (it->second) {
-            BH_BLAS(rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                                    rocblas_datatype_f32_r, lda, B, rocblas_datatype_f32_r, ldb, &zero, C,
-                                    rocblas_datatype_f32_r, ldc, C, rocblas_datatype_f32_r, ldc, rocblas_datatype_f32_r,
-                                    rocblas_gemm_algo_solution_index, sol, 0));
-            return;
-        }
-    }
-    gemm_nn(h, m, n, k, A, lda, B, ldb, C, ldc);
-}
-static void gemm_u12(Solver*, rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B,
-                     int ldb, double* C, int ldc) {
-    gemm_nn(h, m, n, k, A, lda, B, ldb, C, ldc);
-}
-
 // the triangle not referenced by an inverse set to zero (and a unit diagonal
 // written out), so the inverse can enter a full GEMM
 template <class T>
@@ -1441,7 +1392,7 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
                 BH_HIP(hipGetLastError());
             }
             // the next panel's columns first, then hand them over
-            gemm_u12(ctx, main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+            gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
             if constexpr (std::is_same<T, float>::value)
                 if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
             trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
@@ -1449,7 +1400,7 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
             post(Job{k, jb, p ^ 1, false});
             if (rest > jb2) {
                 T* Uw = U + (size_t)jb2 * jb;
-                gemm_u12(ctx, main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
                 if constexpr (std::is_same<T, float>::value)
                     if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
                 trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
@@ -1589,7 +1540,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
             BH_HIP(hipGetLastError());
         }
         // the next panel's columns first, then hand them to the side stream
-        gemm_u12(ctx, main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+        gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
         if constexpr (std::is_same<T, float>::value)
             if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
         trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
@@ -1603,7 +1554,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         // the host had issued them all, and the main stream would idle meanwhile
         if (rest > jb2) {
             T* Uw = U + (size_t)jb2 * jb;
-            gemm_u12(ctx, main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+            gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
             if constexpr (std::is_same<T, float>::value)
                 if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
             trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
@@ -2316,7 +2267,6 @@ void Solver::init(const Solver* cfg) {
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;
-        u12_solution = cfg->u12_solution;
         split_fixed = cfg->split_fixed;
     }
     BH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

@@ -2,13 +2,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
-#define ROCBLAS_BETA_FEATURES_API   // rocblas_gemm_ex_get_solutions
 #include <rocblas/rocblas.h>
 
-#include <array>
 #include <cstdint>
 #include <cstdio>
-#include <map>
 #include <stdexcept>
 #include <memory>
 #include <string>
@@ -145,10 +142,6 @@ struct LuWork {
 // pinned host words and instrumentation.  The context is one solver (lane 0);
 // a batch of independent problems adds more, each driven by its own host
 // thread so the problems' latency-bound kernels overlap on the device.
-// a kernel from rocBLAS's list for the 1024 x n x 1024 FP32 U12 product
-// (ROCm 7.2 on MI355X), used only where rocBLAS lists it for the problem
-constexpr int U12_SOLUTION_DEFAULT = -624953033;
-
 struct Solver {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
@@ -167,11 +160,6 @@ struct Solver {
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
     unsigned int trsv_epoch = 0;
-    // rocBLAS solution used for the FP32 U12 block-row product (0 = rocBLAS's
-    // own choice; U12_SOLUTION_DEFAULT unless BEMHIP_U12_SOLUTION says
-    // otherwise) and, per problem shape, whether rocBLAS lists it
-    int u12_solution = 0;
-    std::map<std::array<int, 6>, bool> u12_listed;
     // split-fp16 updates with one scale for every U12 block of a factorisation
     // (a host-side GEMM alpha); each block's max is logged and checked after
     // the factorisation, which is redone with per-block scales if one fell
