@@ -56,6 +56,10 @@ typedef enum {
     BEMHIP_SOLVE_LU16X3_IR64 = 3  /* FP32 GEPP whose trailing updates run as split-fp16 x3 MFMA
                                      products with FP32 accumulation, + FP64 refinement          */
 } bemhip_solver;
+/* Refinement of the two mixed solvers: x = LU \ b, then sweeps
+ * x += LU \ (b - G x) with the FP64 residual.  The loop stops before a sweep
+ * once |b - G x|_inf <= 1e-10 |b|_inf, or after a sweep once the correction
+ * |d|_inf <= tol |x|_inf (tol <= 0: 1e-9); `iterations` counts the sweeps. */
 
 typedef struct bemhip_ctx bemhip_ctx;
 typedef struct bemhip_sim bemhip_sim;
@@ -64,7 +68,7 @@ typedef struct {
     int32_t iterations;      /* refinement sweeps or BiCGSTAB iterations           */
     int32_t converged;       /* 1 when the stopping test was met                   */
     double residual;         /* |G psi - H phi|_2 / |H phi|_2 in FP64              */
-    double correction;       /* last refinement |d|_inf / |psi|_inf                */
+    double correction;       /* last refinement sweep's |d|_inf / |psi|_inf        */
     double ms_assemble;      /* device time of the assembly kernels                */
     double ms_factor;        /* device time of the factorisation                   */
     double ms_solve;         /* device time of solves / refinement / iterations    */
