@@ -111,7 +111,36 @@ void merge_stats(bemhip_solve_stats& sum, const bemhip_solve_stats& one) {
 // stream) and the context's stream waits for all of them.
 void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G32, const double* d_hphi,
                  double* d_psi, int solver, double tol, int maxit, bemhip_solve_stats* st, bool report_residual) {
-    const int np = (int)topo.probs.size();
+    // small problems: one launch on their own stream (FP64 GEPP in place, no
+    // refinement needed); the rest go to the solver lanes
+    std::vector<int> big;
+    ctx->small_jobs_h.clear();
+    for (int q = 0; q < (int)topo.probs.size(); ++q) {
+        const AsmProb& pr = topo.probs[q];
+        if (solver != BEMHIP_SOLVE_BICGSTAB64 && pr.n <= std::min(ctx->small_n, SMALL_MAX_N))
+            ctx->small_jobs_h.push_back(SmallJob{pr.gofs, pr.n, pr.v0});
+        else
+            big.push_back(q);
+    }
+    const int nsmall = (int)ctx->small_jobs_h.size();
+    if (nsmall > 0) {
+        if (!ctx->small_stream) {
+            BH_HIP(hipStreamCreateWithFlags(&ctx->small_stream, hipStreamNonBlocking));
+            for (auto& e : ctx->small_ev) BH_HIP(hipEventCreate(&e));
+        }
+        hipStream_t ss = ctx->small_stream;
+        SmallJob* dj = static_cast<SmallJob*>(ctx->small_jobs.ensure(sizeof(SmallJob) * nsmall));
+        int* dinfo = static_cast<int*>(ctx->small_info.ensure(sizeof(int) * nsmall));
+        // the host table stays alive in the context until the next call, which
+        // comes after this one's results were read back
+        BH_HIP(hipMemcpyAsync(dj, ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall, hipMemcpyHostToDevice, ss));
+        BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));   // assembly done
+        BH_HIP(hipStreamWaitEvent(ss, ctx->ev[0], 0));
+        BH_HIP(hipEventRecord(ctx->small_ev[0], ss));
+        small_solve_launch(ss, dj, nsmall, G64, d_hphi, d_psi, dinfo);
+        BH_HIP(hipEventRecord(ctx->small_ev[1], ss));
+    }
+    const int np = (int)big.size();
     const int L = std::max(1, std::min(ctx->max_lanes, np));
     while ((int)ctx->lanes.size() < L - 1) {
         ctx->lanes.emplace_back(new Solver());
@@ -120,8 +149,7 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
     std::vector<Solver*> lane(L);
     lane[0] = ctx;
     for (int l = 1; l < L; ++l) lane[l] = ctx->lanes[l - 1].get();
-    std::vector<int> order(np);
-    std::iota(order.begin(), order.end(), 0);
+    std::vector<int> order(big);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return topo.probs[a].n > topo.probs[b].n; });
     std::vector<std::vector<int>> work(L);
     std::vector<double> load(L, 0.0);
@@ -161,9 +189,24 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
         BH_HIP(hipEventRecord(done, lane[l]->stream));
         BH_HIP(hipStreamWaitEvent(ctx->stream, done, 0));
     }
-    for (auto& f : fail) if (f) std::rethrow_exception(f);
     bemhip_solve_stats sum{};
     sum.converged = 1;
+    if (nsmall > 0) {
+        hipStream_t ss = ctx->small_stream;
+        std::vector<int> info(nsmall);
+        BH_HIP(hipMemcpyAsync(info.data(), ctx->small_info.p, sizeof(int) * nsmall, hipMemcpyDeviceToHost, ss));
+        BH_HIP(hipStreamSynchronize(ss));
+        BH_HIP(hipEventRecord(ctx->small_ev[2], ss));
+        BH_HIP(hipStreamWaitEvent(ctx->stream, ctx->small_ev[2], 0));
+        float ms = 0.0f;
+        BH_HIP(hipEventElapsedTime(&ms, ctx->small_ev[0], ctx->small_ev[1]));
+        sum.ms_factor += ms;
+        for (int q = 0; q < nsmall; ++q)
+            BH_CHECK(info[q] == 0, BEMHIP_ERR_SINGULAR,
+                     "zero pivot in column " + std::to_string(info[q]) + " of a batch problem with " +
+                         std::to_string(ctx->small_jobs_h[q].n) + " unknowns");
+    }
+    for (auto& f : fail) if (f) std::rethrow_exception(f);
     for (const auto& p : part) merge_stats(sum, p);
     *st = sum;
 }
@@ -380,6 +423,7 @@ int bemhip_create(int device, bemhip_ctx** out) {
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
+        if (const char* v = std::getenv("BEMHIP_SMALL_N")) ctx->small_n = std::max(0, std::atoi(v));
         const char* sf = std::getenv("BEMHIP_SPLIT_FIXED");
         if (sf && sf[0] == '0') ctx->split_fixed = false;
         ctx->init(nullptr);
@@ -401,6 +445,13 @@ int bemhip_destroy(bemhip_ctx* ctx) {
     ctx->lanes.clear();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    if (ctx->small_stream) {
+        (void)hipStreamSynchronize(ctx->small_stream);
+        for (auto& e : ctx->small_ev) if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->small_stream);
+    }
+    ctx->small_jobs.release();
+    ctx->small_info.release();
     ctx->release();
     delete ctx;
     return BEMHIP_OK;

@@ -130,6 +130,13 @@ struct DevTopology {
 };
 
 // ------------------------------------------------------------- context --
+// small problems of a batch (see small_solve_launch)
+constexpr int SMALL_MAX_N = 1024;
+struct SmallJob {
+    int64_t gofs;     // element offset of the problem's G block
+    int32_t n, v0;    // unknowns, first vertex (offset into b and x)
+};
+
 struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
     // per outer step: inverse of the panel's unit-lower diagonal block and its
@@ -203,6 +210,13 @@ struct bemhip_ctx : bemhip::Solver {
     // extra solvers for batches of independent problems (created on first use)
     std::vector<std::unique_ptr<bemhip::Solver>> lanes;
     int max_lanes = 8;
+    // batches: problems of at most small_n unknowns (and LU solvers) go to the
+    // one-launch small-problem solver on its own stream (0: never)
+    int small_n = bemhip::SMALL_MAX_N;
+    hipStream_t small_stream = nullptr;
+    hipEvent_t small_ev[3] = {};   // start, end, joined
+    std::vector<bemhip::SmallJob> small_jobs_h;
+    bemhip::DevBuf small_jobs, small_info;
 };
 
 namespace bemhip {
@@ -238,6 +252,10 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
 // solvers, whose transposed solve is refined against it)
 void solve_transposed_device(Solver* ctx, int N, const double* G64, const float* G32, int solver,
                              const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st);
+// small problems of a batch: one work-group each, FP64 GEPP + solve in place
+// on the problem's G block (destroyed), all in one launch; info[q] = LAPACK info
+void small_solve_launch(hipStream_t st, const SmallJob* d_jobs, int count, double* G64, const double* b, double* x,
+                        int* d_info);
 // device reductions
 double dev_absmax(Solver* ctx, int n, const double* x);
 double dev_dot(Solver* ctx, int n, const double* x, const double* y);

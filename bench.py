@@ -25,6 +25,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bem-bubbles_amd"))
+# 16 hardware queues for the runtime (read when HIP initialises, before the
+# library is loaded): the batch's 8 solver lanes drive 16 streams, which 4
+# queues (the runtime default) serialise
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import bemhip  # noqa: E402
 import meshgen  # noqa: E402
@@ -108,6 +112,7 @@ def batch_bench(args):
         "config": {"workload": f"{len(cases)} independent spheres, nu in {{4,8,16,32}}, R~U[0.5,2], "
                                f"forcing amplitude~U[0.1,0.5] at 0.5, seed 0, RK4 dt=1e-2",
                    "N_total": N, "M_total": M, "solver": args.solver, "stages_timed": stages,
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                    "sum_n2": float((sizes ** 2).sum()), "lu_flops_per_stage": float((2.0 / 3.0 * sizes ** 3).sum())},
         "unknowns_per_s": round(N * stages / wall, 1),
         "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
@@ -197,7 +202,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"one sphere, nu={args.nu} icosphere (N={N}, M={M}), phi=-1+0.02*SH(l=1..8) seed 0, "
                                f"RK4 dt={case.dt}, one bench step = one RK4 step = 4 stages",
-                   "N": N, "M": M, "solver": args.solver, "stages_timed": stages},
+                   "N": N, "M": M, "solver": args.solver, "stages_timed": stages,
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
         "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
         "refinement_sweeps_per_stage": acc["iterations"] / stages,
         "lu_panels": acc["panels"], "lu_panels_pivoted": acc["panels_pivoted"],

@@ -8,6 +8,7 @@ single-part compute_psi of the same library (1e-12: only the summation order
 of the assembly differs).
 """
 
+import os
 import sys
 
 import numpy as np
@@ -87,6 +88,49 @@ def test_batched_rk4_equals_separate_simulations(ctx, oracle_lib, solver):
         else:
             assert rel(xb, ref.x) < 1e-10 and rel(ph[o:o + n], ref.phi) < 1e-8 and rel(psi[o:o + n], ref.psi) < 1e-8
     assert abs(tm - 0.32) < 1e-15
+
+
+@pytest.fixture(scope="module")
+def ctx_lanes():
+    """A context whose batches solve every problem on the solver lanes
+    (the one-launch small-problem solver switched off)."""
+    old = os.environ.get("BEMHIP_SMALL_N")
+    os.environ["BEMHIP_SMALL_N"] = "0"
+    try:
+        c = bemhip.Context(0)
+    finally:
+        if old is None:
+            del os.environ["BEMHIP_SMALL_N"]
+        else:
+            os.environ["BEMHIP_SMALL_N"] = old
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("solver", ["lu64", "lu16x3_ir64"])
+def test_small_problem_solver_matches_lanes_and_oracle(ctx, ctx_lanes, oracle_lib, solver):
+    """Bubbles of at most 1024 unknowns are solved in one launch (FP64 GEPP
+    and triangular solves, one work-group per bubble); a bubble of N = 1442
+    stays on the solver lanes in the same batch.  Both paths agree part by
+    part, and the small bubbles match the oracle's FP64 LU for either
+    solver (the small path has no FP32 factor)."""
+    say(f"batch small-problem solver {solver}")
+    cases = _cases(seed=5)
+    v12, t12 = meshgen.icosphere(12, 1.1)
+    v12 = v12 + 1e-3 * np.random.default_rng(9).standard_normal(v12.shape)
+    cases.append(meshgen.Case("b12", np.ascontiguousarray(v12), t12,
+                              -0.2 + 0.05 * meshgen.random_sh_field(v12, 1, 3, 19),
+                              meshgen.Physics(1.0, 1.0, 0.0, 1.0), dt=1e-2))
+    v, t, phi, parts, _, _ = meshgen.join_batch(cases)
+    psi, st = ctx.compute_psi(v, t, phi, solver, independent_parts=True)
+    psi_l, st_l = ctx_lanes.compute_psi(v, t, phi, solver, independent_parts=True)
+    assert st["converged"] and st_l["converged"]
+    for c, (o, n) in zip(cases, parts):
+        small = n <= 1024
+        assert rel(psi[o:o + n], psi_l[o:o + n]) < (1e-12 if solver == "lu64" else 1e-8)
+        p = oracle_lib.make_params(c.physics, verts=c.verts, tris=c.tris)
+        ref = oracle_lib.compute_psi(p, c.verts, c.tris, c.phi)
+        assert rel(psi[o:o + n], ref) < (1e-10 if (small or solver == "lu64") else 1e-8)
 
 
 def test_batch_part_physics_reset_and_errors(ctx):
