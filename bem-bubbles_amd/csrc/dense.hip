@@ -1029,12 +1029,87 @@ __global__ __launch_bounds__(256) void k_diag_absmax(int N, const float* __restr
     if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
 }
 
+// k_split_a with two rows per thread (rows and lda even): 4-byte stores of
+// half pairs instead of 2-byte stores; same values
+__global__ __launch_bounds__(256) void k_split_a2(int rows, int K, const float* __restrict__ A, int lda,
+                                                  __half* __restrict__ out, unsigned long long* __restrict__ maxl) {
+    const int r = 2 * (blockIdx.x * 256 + threadIdx.x);
+    const int c0 = blockIdx.y * 32, c1 = min(K, c0 + 32);
+    unsigned int mb = 0;
+    if (r < rows)
+        for (int c = c0; c < c1; ++c) {
+            const float2 a = *reinterpret_cast<const float2*>(A + (size_t)c * lda + r);
+            const float av[2] = {a.x, a.y};
+            unsigned int hb = 0, lb = 0;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const unsigned int ab = __float_as_uint(fabsf(av[u]));
+                mb = ab > mb ? ab : mb;
+                const float v = ldexpf(av[u], SPLIT_EA);
+                const __half h = __float2half_rn(v);
+                const __half l = __float2half_rn(v - __half2float(h));
+                hb |= (unsigned int)__half_as_ushort(h) << (16 * u);
+                lb |= (unsigned int)__half_as_ushort(l) << (16 * u);
+            }
+            *reinterpret_cast<unsigned int*>(out + (size_t)c * rows + r) = hb;
+            *reinterpret_cast<unsigned int*>(out + (size_t)(c + K) * rows + r) = hb;
+            *reinterpret_cast<unsigned int*>(out + (size_t)(c + 2 * K) * rows + r) = lb;
+        }
+    if (maxl == nullptr) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned int o = __shfl_xor(mb, off);
+        mb = o > mb ? o : mb;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const unsigned long long bits = mb > 0x7F800000u ? 0x7FF8000000000000ull
+                                                         : (unsigned long long)__double_as_longlong((double)__uint_as_float(mb));
+        atomicMax(maxl, bits);
+    }
+}
+
+// k_split_b_fixed with four rows per thread (K and ldb multiples of 4): 16-byte
+// loads and 8-byte stores of half quadruples; same values
+__global__ __launch_bounds__(256) void k_split_b_fixed4(int K, int cols, const float* __restrict__ B, int ldb, int eb,
+                                                        __half* __restrict__ out, unsigned int* __restrict__ log) {
+    const int r = 4 * (blockIdx.x * 256 + threadIdx.x);
+    float m = 0.0f;
+    if (r < K) {
+        const size_t K3 = 3 * (size_t)K;
+        const int c0 = blockIdx.y * 16, c1 = min(cols, c0 + 16);
+        for (int c = c0; c < c1; ++c) {
+            const float4 b = *reinterpret_cast<const float4*>(B + (size_t)c * ldb + r);
+            const float bv[4] = {b.x, b.y, b.z, b.w};
+            unsigned int h2[2] = {0u, 0u}, l2[2] = {0u, 0u};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                m = fmaxf(m, fabsf(bv[u]));
+                const float v = ldexpf(bv[u], eb);
+                const __half h = __float2half_rn(v);
+                const __half l = __float2half_rn(v - __half2float(h));
+                h2[u >> 1] |= (unsigned int)__half_as_ushort(h) << (16 * (u & 1));
+                l2[u >> 1] |= (unsigned int)__half_as_ushort(l) << (16 * (u & 1));
+            }
+            const uint2 hv = make_uint2(h2[0], h2[1]), lv = make_uint2(l2[0], l2[1]);
+            *reinterpret_cast<uint2*>(out + (size_t)c * K3 + r) = hv;
+            *reinterpret_cast<uint2*>(out + (size_t)c * K3 + K + r) = lv;
+            *reinterpret_cast<uint2*>(out + (size_t)c * K3 + 2 * K + r) = hv;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(log, __float_as_uint(m));
+}
+
 // fp16 splits of a panel's L21 (rows x k) into the step's A buffer (parity p)
 static void split_a(Solver* ctx, hipStream_t st, int rows, int k, const float* A, int lda, int p,
                     unsigned long long* maxl) {
     if (rows <= 0) return;
     __half* acat = ctx->split_a[p].as<__half>();
-    k_split_a<<<dim3((rows + 255) / 256, (k + 63) / 64), 256, 0, st>>>(rows, k, A, lda, acat, maxl);
+    if (rows % 2 == 0 && lda % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 8 == 0)
+        k_split_a2<<<dim3((rows / 2 + 255) / 256, (k + 31) / 32), 256, 0, st>>>(rows, k, A, lda, acat, maxl);
+    else
+        k_split_a<<<dim3((rows + 255) / 256, (k + 63) / 64), 256, 0, st>>>(rows, k, A, lda, acat, maxl);
     BH_HIP(hipGetLastError());
 }
 
@@ -1047,7 +1122,13 @@ static void split_b(Solver* ctx, hipStream_t st, int k, int n, const float* B, i
     if (ctx->split_fixed_on) {
         BH_CHECK(ctx->split_calls < ctx->split_nlog, BEMHIP_ERR_HIP, "split log overflow");
         unsigned int* log = ctx->split_log.as<unsigned int>() + ctx->split_calls++;
-        k_split_b_fixed<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb, bcat, log);
+        if (k % 4 == 0 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0 &&
+            reinterpret_cast<uintptr_t>(bcat) % 8 == 0)
+            k_split_b_fixed4<<<dim3((k / 4 + 255) / 256, (n + 15) / 16), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb,
+                                                                                      bcat, log);
+        else
+            k_split_b_fixed<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb, bcat,
+                                                                                 log);
         BH_HIP(hipGetLastError());
         return;
     }

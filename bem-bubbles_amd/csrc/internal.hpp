@@ -211,10 +211,10 @@ struct bemhip_ctx : bemhip::Solver {
     std::vector<std::unique_ptr<bemhip::Solver>> lanes;
     int max_lanes = 8;
     // batches: problems of at most small_n unknowns (and LU solvers) go to the
-    // one-launch small-problem solver on its own stream (0: never)
+    // one-launch small-problem solver, enqueued last on the least loaded
+    // solver lane (0: never)
     int small_n = bemhip::SMALL_MAX_N;
-    hipStream_t small_stream = nullptr;
-    hipEvent_t small_ev[3] = {};   // start, end, joined
+    hipEvent_t small_ev[2] = {};   // start, end of the launch
     std::vector<bemhip::SmallJob> small_jobs_h;
     bemhip::DevBuf small_jobs, small_info;
 };

@@ -79,6 +79,9 @@ def batch_bench(args):
     amplitude ~ U[0.1, 0.5] at frequency 0.5, seed 0) advanced together, one
     dense problem per bubble, RK4 dt = 1e-2."""
     cases = meshgen.config_batch(seed=0, count=args.batch)
+    if args.batch_nu:
+        keep = {int(x) for x in args.batch_nu.split(",")}
+        cases = [c for c in cases if int(c.name.rsplit("_nu", 1)[1]) in keep]
     v, t, phi, parts, amp, freq = meshgen.join_batch(cases)
     sizes = np.array([n for _, n in parts], dtype=np.float64)
     N, M = len(v), len(t)
@@ -103,13 +106,15 @@ def batch_bench(args):
     log(f"[bench] batch {ms_stage:.2f} ms/stage {acc}")
     out = {
         "metric": f"ms/RK4 stage (assembly+solve), batch of {len(cases)} independent bubbles",
+        "small_problem_solver_max_n": int(os.environ.get("BEMHIP_SMALL_N", "1024")),
         "value": round(ms_stage, 3), "unit": "ms/stage", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-lu(split-f16 mfma)+f64-ir" if args.solver == "lu16x3_ir64" else
                  ("f32-lu+f64-ir" if args.solver == "lu32_ir64" else "f64"),
         "data": "synthetic",
-        "config": {"workload": f"{len(cases)} independent spheres, nu in {{4,8,16,32}}, R~U[0.5,2], "
+        "config": {"workload": f"{len(cases)} independent spheres, nu in "
+                               f"{{{args.batch_nu or '4,8,16,32'}}} (of {args.batch} drawn from {{4,8,16,32}}), R~U[0.5,2], "
                                f"forcing amplitude~U[0.1,0.5] at 0.5, seed 0, RK4 dt=1e-2",
                    "N_total": N, "M_total": M, "solver": args.solver, "stages_timed": stages,
                    "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
@@ -148,6 +153,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the single-sphere workload, time a batch of this many independent bubbles")
+    ap.add_argument("--batch-nu", default="",
+                    help="keep only the batch's bubbles of these icosphere frequencies (comma list, e.g. 4,8)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (reference for HBM byte counters)")
     args = ap.parse_args()
