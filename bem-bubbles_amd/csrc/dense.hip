@@ -1070,8 +1070,11 @@ __global__ __launch_bounds__(256) void k_split_a2(int rows, int K, const float* 
 
 // k_split_b_fixed with four rows per thread (K and ldb multiples of 4): 16-byte
 // loads and 8-byte stores of half quadruples; same values
+// When dst is given, B's values also go to dst (ld ldd): the U12 block into
+// the factor, in the same pass.
 __global__ __launch_bounds__(256) void k_split_b_fixed4(int K, int cols, const float* __restrict__ B, int ldb, int eb,
-                                                        __half* __restrict__ out, unsigned int* __restrict__ log) {
+                                                        __half* __restrict__ out, unsigned int* __restrict__ log,
+                                                        float* __restrict__ dst, int ldd) {
     const int r = 4 * (blockIdx.x * 256 + threadIdx.x);
     float m = 0.0f;
     if (r < K) {
@@ -1079,6 +1082,10 @@ __global__ __launch_bounds__(256) void k_split_b_fixed4(int K, int cols, const f
         const int c0 = blockIdx.y * 16, c1 = min(cols, c0 + 16);
         for (int c = c0; c < c1; ++c) {
             const float4 b = *reinterpret_cast<const float4*>(B + (size_t)c * ldb + r);
+            if (dst) {
+                float* d = dst + (size_t)c * ldd + r;   // any ldd: four 4-byte stores
+                d[0] = b.x; d[1] = b.y; d[2] = b.z; d[3] = b.w;
+            }
             const float bv[4] = {b.x, b.y, b.z, b.w};
             unsigned int h2[2] = {0u, 0u}, l2[2] = {0u, 0u};
 #pragma unroll
@@ -1116,27 +1123,33 @@ static void split_a(Solver* ctx, hipStream_t st, int rows, int k, const float* A
 // fp16 splits of U12 columns [col0, col0 + n) (k x n, ld ldb) with their own
 // scale; `slot` (0: the next panel's columns, 1: the rest) selects the scale
 // word and the alpha pair, whose scale word the caller has zeroed.
-static void split_b(Solver* ctx, hipStream_t st, int k, int n, const float* B, int ldb, int col0, int slot) {
-    if (n <= 0) return;
+// With dst (ld ldd) the FP32 block B is also copied there when the vectorised
+// kernel runs; returns whether it was.
+static bool split_b(Solver* ctx, hipStream_t st, int k, int n, const float* B, int ldb, int col0, int slot,
+                    float* dst = nullptr, int ldd = 0) {
+    if (n <= 0) return true;
     __half* bcat = ctx->split_b.as<__half>() + (size_t)col0 * 3 * k;
     if (ctx->split_fixed_on) {
         BH_CHECK(ctx->split_calls < ctx->split_nlog, BEMHIP_ERR_HIP, "split log overflow");
         unsigned int* log = ctx->split_log.as<unsigned int>() + ctx->split_calls++;
         if (k % 4 == 0 && ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0 &&
-            reinterpret_cast<uintptr_t>(bcat) % 8 == 0)
-            k_split_b_fixed4<<<dim3((k / 4 + 255) / 256, (n + 15) / 16), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb,
-                                                                                      bcat, log);
-        else
-            k_split_b_fixed<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb, bcat,
-                                                                                 log);
+            reinterpret_cast<uintptr_t>(bcat) % 8 == 0) {
+            k_split_b_fixed4<<<dim3((k / 4 + 255) / 256, (n + 15) / 16), 256, 0, st>>>(
+                k, n, B, ldb, ctx->split_eb, bcat, log, dst, ldd);
+            BH_HIP(hipGetLastError());
+            return dst != nullptr;
+        }
+        k_split_b_fixed<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, ctx->split_eb, bcat,
+                                                                             log);
         BH_HIP(hipGetLastError());
-        return;
+        return false;
     }
     unsigned int* mx = ctx->split_mx.as<unsigned int>();
     float* ab = reinterpret_cast<float*>(mx + 4) + 2 * slot;
     k_block_absmax<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot);
     k_split_b<<<dim3((k + 255) / 256, (n + 63) / 64), 256, 0, st>>>(k, n, B, ldb, mx + slot, bcat, ab);
     BH_HIP(hipGetLastError());
+    return false;
 }
 
 // C[:, col0:col0+n] -= L21 U12[:, col0:col0+n] from the prepared splits
@@ -1472,22 +1485,26 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
                 k_fill_u32<<<1, 64, 0, main.s>>>(ctx->split_mx.as<unsigned int>(), 2, 0u);
                 BH_HIP(hipGetLastError());
             }
-            // the next panel's columns first, then hand them over
+            // the next panel's columns first, then hand them over.  The split
+            // kernel also writes U12 into the factor (A12 is read only by the
+            // U12 product before it); otherwise one copy at the end.
+            bool in_factor = false;
             gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
             if constexpr (std::is_same<T, float>::value)
-                if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
+                if (split16) in_factor = split_b(ctx, main.s, jb, jb2, U, jb, 0, 0, A12, lda);
             trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
             BH_HIP(hipEventRecord(ev_ready, main.s));
             post(Job{k, jb, p ^ 1, false});
             if (rest > jb2) {
                 T* Uw = U + (size_t)jb2 * jb;
-                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+                T* A12w = A12 + (size_t)jb2 * lda;
+                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12w, lda, Uw, jb);
                 if constexpr (std::is_same<T, float>::value)
-                    if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
-                trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
-                                   A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
+                    if (split16) in_factor = split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1, A12w, lda) && in_factor;
+                trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb, A12w + jb, lda, split16,
+                                   jb2, p, 1);
             }
-            copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
+            if (!in_factor) copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
             Done d;
             {
                 std::unique_lock<std::mutex> lk(mu);
