@@ -142,6 +142,111 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def two_bubbles_bench(args):
+    """Two coupled spheres (icosphere nu = 32 each, N = 20484 unknowns), R = 1.0
+    and 0.8 three apart on x, phi = -R * Rdot with Rdot = +0.5 / -0.5, 1e-3
+    seeded vertex jitter; 10 RK4 steps of 5e-4 from the same initial state
+    with the mixed-precision and the plain FP64 solver, the final states
+    compared."""
+    c = meshgen.config_two_bubbles(seed=0, nu=32)
+    ph = c.physics
+    N, M = len(c.verts), len(c.tris)
+    ctx = bemhip.Context(0)
+    res = {}
+    for solver in ("lu16x3_ir64", "lu64"):
+        sim = bemhip.ColocSim(ctx, c.verts, c.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, solver=solver)
+        for w in range(args.warmup):   # warmup steps from the initial state, then back to it
+            sim.set_phi(c.phi)
+            sim.evolve_system_RK4(c.dt, True)
+        sim.set_mesh(c.verts, c.tris)
+        sim.set_phi(c.phi)
+        sim.set_time(0.0)
+        sim.accum_stats(reset=True)
+        t0 = time.perf_counter()
+        for k in range(c.steps):
+            sim.evolve_system_RK4(c.dt, True)
+        wall = time.perf_counter() - t0
+        acc = sim.accum_stats(reset=True)
+        x, phi, psi, tm = sim.state()
+        res[solver] = (wall * 1e3 / (4 * c.steps), x, phi, psi, acc)
+        log(f"[bench] two bubbles {solver}: {res[solver][0]:.2f} ms/stage")
+        sim.close()
+    ctx.close()
+    ms, x, phi, psi, acc = res["lu16x3_ir64"]
+    ms64, x64, phi64, psi64, _ = res["lu64"]
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    out = {
+        "metric": "ms/RK4 stage (assembly+solve), two coupled bubbles N=20484, mixed vs FP64 over 10 steps",
+        "value": round(ms, 3), "unit": "ms/stage", "n_gpus": 1, "steps": c.steps, "warmup": args.warmup,
+        "ms_per_step": round(4 * ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-lu(split-f16 mfma)+f64-ir", "data": "synthetic",
+        "config": {"workload": "two spheres nu=32 (N=20484, M=40960), R=1.0 at x=-1.5 and 0.8 at x=+1.5, "
+                               "Rdot=+0.5/-0.5, jitter 1e-3 seed 0, RK4 dt=5e-4, 10 steps, coupled (one problem)",
+                   "N": N, "M": M, "solver": "lu16x3_ir64",
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+        "fp64_ms_per_stage": round(ms64, 3), "speedup_vs_fp64": round(ms64 / ms, 3),
+        "after_10_steps_rel_diff_vs_fp64": {"x": rel(x, x64), "phi": rel(phi, phi64), "psi": rel(psi, psi64)},
+        "refinement_sweeps_per_stage": acc["iterations"] / (4 * c.steps),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_bench(args):
+    """Gradient of the radius-trajectory loss: sphere nu = 16 (N = 2562) at
+    rest in p(t) = 1 + 0.3 sin(pi t), 50 RK4 steps of 1e-2, checkpoints every
+    10 steps; the target trajectory from the run whose initial phi carries a
+    seeded 1 % spherical-harmonic perturbation.  Times one loss + gradient
+    (w.r.t. x0 and phi0) per solver against the 50 forward RK4 steps alone."""
+    c = meshgen.config_adjoint(seed=0)
+    ph = c.physics
+    ctx = bemhip.Context(0)
+    out_s = {}
+    fwd = bemhip.ColocSim(ctx, c.verts, c.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, field_amp=ph.field_amp,
+                          field_freq=ph.field_freq, solver="lu64")
+    V0 = fwd.get_volume()
+    fwd.set_phi(meshgen.adjoint_target_phi(c, seed=1))
+    target = []
+    for _ in range(c.steps):
+        fwd.evolve_system_RK4(c.dt, True)
+        target.append((3.0 * fwd.get_volume() / (4.0 * np.pi)) ** (1.0 / 3.0))
+    target = np.array(target)
+    fwd.close()
+    for solver in ("lu16x3_ir64", "lu64"):
+        sim = bemhip.ColocSim(ctx, c.verts, c.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, field_amp=ph.field_amp,
+                              field_freq=ph.field_freq, solver=solver, V_0=V0)
+        sim.set_phi(c.phi)
+        sim.adjoint_radius(c.steps, c.dt, target, every=10)   # warmup
+        t0 = time.perf_counter()
+        L, R, gx, gp = sim.adjoint_radius(c.steps, c.dt, target, every=10)
+        t_grad = time.perf_counter() - t0
+        # the forward alone: the same 50 RK4 steps from the same state
+        sim.set_time(0.0)
+        t0 = time.perf_counter()
+        for _ in range(c.steps):
+            sim.evolve_system_RK4(c.dt, True)
+        t_fwd = time.perf_counter() - t0
+        out_s[solver] = {"grad_ms": round(t_grad * 1e3, 1), "forward_ms": round(t_fwd * 1e3, 1),
+                         "grad_over_forward": round(t_grad / t_fwd, 2), "loss": L}
+        log(f"[bench] adjoint {solver}: {out_s[solver]}")
+        sim.close()
+    ctx.close()
+    m = out_s["lu16x3_ir64"]
+    out = {
+        "metric": "ms per loss+gradient (50 RK4 steps, checkpoints every 10), sphere N=2562",
+        "value": m["grad_ms"], "unit": "ms/gradient", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": m["grad_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 state, f32-lu(split-f16 mfma)+f64-ir solves", "data": "synthetic",
+        "config": {"workload": "sphere nu=16 (N=2562) at rest, p_inf(t)=1+0.3 sin(pi t), 50 RK4 steps dt=1e-2, "
+                               "checkpoint every 10, target from 1% perturbed phi (seed 1)",
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+        "per_solver": out_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +256,9 @@ def main():
     ap.add_argument("--solver", default="lu16x3_ir64", choices=sorted(bemhip.SOLVERS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "two-bubbles", "adjoint"],
+                    help="sphere: the N=40962 RK4 stage (default); two-bubbles: N=20484 coupled pair, "
+                         "mixed vs FP64 over 10 steps; adjoint: loss + gradient over 50 steps at N=2562")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the single-sphere workload, time a batch of this many independent bubbles")
     ap.add_argument("--batch-nu", default="",
@@ -174,6 +282,12 @@ def main():
         sys.exit(2)
     if args.batch:
         batch_bench(args)
+        return
+    if args.workload == "two-bubbles":
+        two_bubbles_bench(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_bench(args)
         return
 
     case = meshgen.config_large(seed=0, nu=args.nu)
