@@ -38,7 +38,7 @@ PEAK = {"lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense
 # HBM bytes per trailing-update GEMM launch at nu = 64, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain 1 GiB copy):
 # profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md
-TRAFFIC_NU64 = {"lu16x3_ir64": 6.50e9}
+TRAFFIC_NU64 = {"lu16x3_ir64": 5.99e9}
 
 
 def log(*a):
