@@ -425,6 +425,8 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
         if (const char* v = std::getenv("BEMHIP_SMALL_N")) ctx->small_n = std::max(0, std::atoi(v));
+        const char* s1 = std::getenv("BEMHIP_SMALL_SINGLE");
+        if (s1 && s1[0] == '1') ctx->small_single = true;
         const char* sf = std::getenv("BEMHIP_SPLIT_FIXED");
         if (sf && sf[0] == '0') ctx->split_fixed = false;
         ctx->init(nullptr);

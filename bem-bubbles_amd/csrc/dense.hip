@@ -2211,7 +2211,19 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
     BH_HIP(hipEventCreate(&e0)); BH_HIP(hipEventCreate(&e1)); BH_HIP(hipEventCreate(&e2));
     BH_HIP(hipEventRecord(e0, sm));
     st->iterations = 0; st->converged = 1; st->correction = 0.0; st->residual = 0.0;
-    if (solver == BEMHIP_SOLVE_BICGSTAB64) {
+    const bool small1 = ctx->small_single && solver != BEMHIP_SOLVE_BICGSTAB64 && N <= SMALL_MAX_N;
+    if (small1) {
+        // FP64 GEPP and both solves in one work-group, in place on G64
+        char* w = static_cast<char*>(ctx->small1.ensure(256));
+        const SmallJob job{0, N, 0};
+        BH_HIP(hipMemcpyAsync(w, &job, sizeof(job), hipMemcpyHostToDevice, sm));
+        small_solve_launch(sm, reinterpret_cast<const SmallJob*>(w), 1, G64, b, x, reinterpret_cast<int*>(w + 64));
+        BH_HIP(hipEventRecord(e1, sm));
+        int info = 0;
+        BH_HIP(hipMemcpyAsync(&info, w + 64, sizeof(int), hipMemcpyDeviceToHost, sm));
+        BH_HIP(hipStreamSynchronize(sm));
+        BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
+    } else if (solver == BEMHIP_SOLVE_BICGSTAB64) {
         BH_HIP(hipEventRecord(e1, sm));
         bicgstab(ctx, N, G64, b, x, tol, maxit, st);
     } else if (solver == BEMHIP_SOLVE_LU64) {
@@ -2279,7 +2291,7 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
     st->panels_pivoted = ctx->panels_pivoted;
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     // final FP64 residual |b - G x| / |b| (not for LU64, whose G was overwritten)
-    if (report_residual && solver != BEMHIP_SOLVE_LU64) {
+    if (report_residual && solver != BEMHIP_SOLVE_LU64 && !small1) {
         double* r = static_cast<double*>(ctx->vec[5].ensure(sizeof(double) * N));
         gemv64(ctx, N, G64, x, b, r);
         const double rn = dev_dot(ctx, N, r, r), bn = dev_dot(ctx, N, b, b);
@@ -2330,6 +2342,8 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
                              const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st) {
     const int* perm = ctx->perm.as<int>();
     BH_CHECK(perm != nullptr, BEMHIP_ERR_ARG, "transposed solve needs the factors of a previous LU solve");
+    BH_CHECK(!(ctx->small_single && N <= SMALL_MAX_N), BEMHIP_ERR_ARG,
+             "transposed solves need the blocked LU's factors (BEMHIP_SMALL_SINGLE routes this size elsewhere)");
     st->iterations = 0; st->converged = 1; st->correction = 0.0; st->residual = 0.0;
     if (solver == BEMHIP_SOLVE_LU64) {
         lu_solve_t64(ctx, N, G64, perm, b, x);
@@ -2377,6 +2391,7 @@ void Solver::init(const Solver* cfg) {
         trsv_persistent = cfg->trsv_persistent;
         timing = cfg->timing;
         split_fixed = cfg->split_fixed;
+        small_single = cfg->small_single;
     }
     BH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BH_BLAS(rocblas_create_handle(&blas));

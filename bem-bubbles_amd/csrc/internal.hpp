@@ -166,6 +166,10 @@ struct Solver {
     int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
+    // single problems of at most SMALL_MAX_N unknowns through the one-work-group
+    // FP64 solver (BEMHIP_SMALL_SINGLE=1; leaves no factors for a transposed solve)
+    bool small_single = false;
+    DevBuf small1;   // its job record and info word
     unsigned int trsv_epoch = 0;
     // split-fp16 updates with one scale for every U12 block of a factorisation
     // (a host-side GEMM alpha); each block's max is logged and checked after

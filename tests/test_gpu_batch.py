@@ -133,6 +133,47 @@ def test_small_problem_solver_matches_lanes_and_oracle(ctx, ctx_lanes, oracle_li
         assert rel(psi[o:o + n], ref) < (1e-10 if (small or solver == "lu64") else 1e-8)
 
 
+@pytest.fixture(scope="module")
+def ctx_small1():
+    """A context whose single-problem LU solves of at most 1024 unknowns run
+    on the one-work-group solver of the batches (BEMHIP_SMALL_SINGLE=1)."""
+    old = os.environ.get("BEMHIP_SMALL_SINGLE")
+    os.environ["BEMHIP_SMALL_SINGLE"] = "1"
+    try:
+        c = bemhip.Context(0)
+    finally:
+        if old is None:
+            del os.environ["BEMHIP_SMALL_SINGLE"]
+        else:
+            os.environ["BEMHIP_SMALL_SINGLE"] = old
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n", [13, 300, 1024])
+def test_small_solver_row_interchanges(ctx_small1, n):
+    """The one-work-group solver on matrices that need row interchanges
+    (random normal entries; BEM matrices never pivot), against LAPACK, for
+    both an FP64 and a mixed solver id (the small path is FP64 for both)."""
+    say(f"small solver interchanges n={n}")
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    ref = np.linalg.solve(A, b)
+    for solver in ("lu64", "lu16x3_ir64"):
+        x, st = ctx_small1.solve(A, b, solver)
+        assert rel(x, ref) < 1e-10 and st["panels"] == 0
+
+
+def test_small_solver_zero_pivot(ctx_small1):
+    say("small solver zero pivot")
+    A = np.eye(20)
+    A[:, 7] = 0.0   # a zero column: GEPP finds a zero pivot there
+    with pytest.raises(bemhip.BemHipError) as e:
+        ctx_small1.solve(A, np.ones(20), "lu64")
+    assert e.value.code == bemhip.ERR_SINGULAR and "column 8" in str(e.value)
+
+
 def test_batch_part_physics_reset_and_errors(ctx):
     say("batch part physics")
     cases = _cases()
