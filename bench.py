@@ -12,6 +12,11 @@ MFMA LU with FP32 accumulation + FP64 iterative refinement); the plain FP64
 path (FP64 LU) is timed on the same state for the speed ratio, and the psi of
 both paths is compared (target <= 1e-8 relative).
 
+Other workloads (one JSON line each): --workload two-bubbles (two coupled
+spheres, N = 20484, mixed vs FP64 over 10 RK4 steps), --workload adjoint (loss
+and gradient over 50 RK4 steps at N = 2562), --batch 256 (256 independent
+bubbles of uneven size; --batch-nu keeps some size classes).
+
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 
@@ -39,6 +44,10 @@ PEAK = {"lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense
 # runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain 1 GiB copy):
 # profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md
 TRAFFIC_NU64 = {"lu16x3_ir64": 5.99e9}
+# MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
+# GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
+# profiles/lu_update_gemm_nu64_mfma_utilisation.md
+MFMA_BUSY_NU64 = {"lu16x3_ir64": (0.511, 1.82)}
 
 
 def log(*a):
@@ -337,6 +346,10 @@ def main():
                            "frac": round(achieved / PEAK[args.solver], 4),
                            "traffic": TRAFFIC_NU64.get(args.solver) if args.nu == 64 else None,
                            "launches": acc["gemm_launches"], "avg_launch_ms": round(gms / max(1, acc["gemm_launches"]), 4)}
+        if args.nu == 64 and args.solver in MFMA_BUSY_NU64:
+            busy, clk = MFMA_BUSY_NU64[args.solver]
+            out["roofline"]["mfma_busy_pmc"] = busy
+            out["roofline"]["engine_clock_ghz_pmc"] = clk
 
     # assembly: algorithmic output bytes (G in FP64, plus its FP32 copy for the
     # mixed solvers) and algorithmic flops (35 per quadrature point, 7 points
