@@ -3,6 +3,7 @@
 import ctypes
 import math
 import os
+import sys
 
 import numpy as np
 
@@ -70,3 +71,13 @@ def test_cpp_surface_compiles_and_links(tmp_path):
                         "-o", str(tmp_path / "use"), "-L", os.path.dirname(bemhip.LIB_PATH), "-lbemhip"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so,
+    before any device work (runs on the CPU)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], cwd=root,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
