@@ -40,7 +40,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "bemhip.h")
  ERR_IO) = range(11)
 MIRROR_X0 = 0x1
 INDEPENDENT_PARTS = 0x2
-SOLVERS = {"lu64": 0, "lu32_ir64": 1, "bicgstab64": 2, "lu16x3_ir64": 3}
+SOLVERS = {"lu64": 0, "lu32_ir64": 1, "bicgstab64": 2, "lu16x3_ir64": 3, "lu16_ir64": 4}
 
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -242,11 +242,11 @@ class Context:
             raise BemHipError(rc, self.L.bemhip_last_error(self.h).decode())
 
     # ColocSim::assemble_matrices (ColocSim.cpp:21-110)
-    def assemble(self, verts, tris, mirror=False, want_H=True, phi=None):
+    def assemble(self, verts, tris, mirror=False, want_H=True, phi=None, want_G=True):
         v = np.ascontiguousarray(verts, dtype=np.float64)
         t = np.ascontiguousarray(tris, dtype=np.uint32)
         N, M = v.shape[0], t.shape[0]
-        G = np.zeros((N, N), order="F")
+        G = np.zeros((N, N), order="F") if want_G else None
         H = np.zeros((N, N), order="F") if want_H else None
         ph = np.ascontiguousarray(phi, dtype=np.float64) if phi is not None else None
         hphi = np.zeros(N) if phi is not None else None

@@ -80,7 +80,9 @@ void d2h(void* h, const void* d, size_t bytes, hipStream_t s) {
     BH_HIP(hipStreamSynchronize(s));
 }
 
-bool mixed(int solver) { return solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64; }
+bool mixed(int solver) {
+    return solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64 || solver == BEMHIP_SOLVE_LU16_IR64;
+}
 
 float event_ms(hipEvent_t a, hipEvent_t b) {
     BH_HIP(hipEventSynchronize(b));
@@ -421,6 +423,8 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_NB16")) ctx->nb16 = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_MM_RESERVE")) ctx->mm_reserve = std::max(0, std::atoi(v));
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
@@ -472,8 +476,8 @@ int bemhip_assemble(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, co
                     double* G, double* H, const double* phi, double* Hphi) {
     if (!ctx) return BEMHIP_ERR_ARG;
     return guarded(ctx, [&] {
-        BH_CHECK(xyz && tri && G, BEMHIP_ERR_ARG, "null argument");
-        BH_CHECK(N <= 46340, BEMHIP_ERR_ARG, "host G output limited to N <= 46340");
+        BH_CHECK(xyz && tri && (G || H || (phi && Hphi)), BEMHIP_ERR_ARG, "null argument");
+        BH_CHECK(N <= 46340 || !(G || H), BEMHIP_ERR_ARG, "host G output limited to N <= 46340");
         BH_CHECK(!(flags & ~BEMHIP_MIRROR_X0), BEMHIP_ERR_ARG, "unknown assembly flag");
         check_finite(phi, N, "phi");
         Topology th = build_topology(N, xyz, M, tri);
@@ -492,7 +496,7 @@ int bemhip_assemble(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, co
         double* dG = dev<double>(ctx->G64, nn);
         double* dH = H ? dev<double>(ctx->H, nn) : nullptr;
         assemble_device(ctx, topo, dx, dphi, flags, AsmOut{dG, nullptr, dH, dh});
-        d2h(G, dG, sizeof(double) * nn, ctx->stream);
+        if (G) d2h(G, dG, sizeof(double) * nn, ctx->stream);
         if (H) d2h(H, dH, sizeof(double) * nn, ctx->stream);
         if (dh) d2h(Hphi, dh, sizeof(double) * N, ctx->stream);
     });

@@ -1201,7 +1201,11 @@ static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m
         BH_HIP(hipEventRecord(e0, st));
     }
     if constexpr (std::is_same<T, float>::value) {
-        if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc, p, slot);
+        if (ctx->lu16)
+            mm16_gemm(st, MM_SUB, 0, ctx->lu.t_l21[p].p, k, ctx->lu.t_u12.p, k, col0, m, n, k, C, ldc,
+                      std::ldexp(1.0f, ctx->lu16_e - 22),
+                      ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0);
+        else if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc, p, slot);
         else gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
         gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
@@ -1221,6 +1225,51 @@ static void collect_gemm_timing(Solver* ctx) {
     ctx->gemm_ev_used = 0;
 }
 
+
+// ------------------------------------------------------------- fp16 LU --
+// Every product of the fp16 LU runs on fp16 matrix cores with FP32
+// accumulation (mm16.hip).  Operands are scaled by powers of two (exact) from
+// e, max |diag A| in [2^(e-1), 2^e): blocks of the matrix (A21, A12, U12) by
+// 2^(8-e), U11^-1 by 2^(e-2), L11^-1 by 1 and L21 (|l| <= 1) by 2^14; every
+// pack logs its largest scaled magnitude, and a factorisation whose log
+// exceeds 2^15 (or is not finite) is reported as an fp16 overflow.
+static float lu16_sa(const Solver* ctx) { return std::ldexp(1.0f, 8 - ctx->lu16_e); }
+
+// Step p's products on the panel stream: L21 = A21 U11^-1 (rows below an
+// m x n panel, A21 at A + n) into l21 (ld m - n) with its A tiles and max |l|
+// merged into maxl (when set); with l21 == A + n (a panel factored with row
+// interchanges, L21 already in place) only its tiles.  Then L11^-1 as A tiles.
+static void lu16_panel_products(Solver* ctx, Lane L, int m, int n, const float* A, int lda, float* l21, int ldl,
+                                int p, unsigned long long* maxl) {
+    unsigned int* log = ctx->lu.f16log.as<unsigned int>();
+    if (l21 != A + n) {
+        mm16_pack(L.s, true, n, n, ctx->lu.winv_u.as<float>(), n, std::ldexp(1.0f, ctx->lu16_e - 2), ctx->lu.t_wu.p,
+                  log);
+        mm16_pack(L.s, false, m - n, n, A + n, lda, lu16_sa(ctx), ctx->lu.t_a21.p, log);
+        const Mm16Tiles tl{ctx->lu.t_l21[p].p, false, n, 0, 16384.0f, log, maxl};
+        mm16_gemm(L.s, MM_STORE, 2, ctx->lu.t_a21.p, n, ctx->lu.t_wu.p, n, 0, m - n, n, n, l21, ldl,
+                  std::ldexp(1.0f, -6), 0, &tl);
+    } else {
+        mm16_pack(L.s, false, m - n, n, l21, ldl, 16384.0f, ctx->lu.t_l21[p].p, log, nullptr, 0, maxl);
+    }
+    tri_inverse(L, true, n, A, lda, ctx->lu.winv_l[p].as<float>());
+    mm16_pack(L.s, false, n, n, ctx->lu.winv_l[p].as<float>(), n, 1.0f, ctx->lu.t_wl[p].p, log);
+}
+
+// columns [c0, c0 + nc) of step p's block row (jb rows at A12, ld lda):
+// U12 = L11^-1 A12 into the factor with its B tiles, then the trailing update
+// of those columns (rest rows below the block row)
+static void lu16_cols(Solver* ctx, Lane L, int jb, int rest, int c0, int nc, float* A12, int lda, int p) {
+    if (nc <= 0) return;
+    unsigned int* log = ctx->lu.f16log.as<unsigned int>();
+    const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;   // bytes of c0 / 32 column blocks
+    float* a12 = A12 + (size_t)c0 * lda;
+    mm16_pack(L.s, true, nc, jb, a12, lda, lu16_sa(ctx), ctx->lu.t_a12.as<char>() + off, log);
+    const Mm16Tiles tu{ctx->lu.t_u12.p, true, jb, c0, lu16_sa(ctx), log, nullptr};
+    mm16_gemm(L.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda,
+              std::ldexp(1.0f, ctx->lu16_e - 8), 0, &tu);
+    trailing_update<float>(ctx, L.s, L.h, rest, nc, jb, nullptr, 0, nullptr, 0, a12 + jb, lda, false, c0, p, 0);
+}
 
 // ------------------------------------------------------------ factorise --
 template <class T>
@@ -1289,6 +1338,7 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
     if (m > n) {
         T* W = ctx->lu.winv_u.as<T>();
         tri_inverse(L, false, n, A, lda, W);
+        if (ctx->lu16) return;   // L21 with the step's other products (lu16_panel_products)
         gemm_nn(L.h, m - n, n, n, A + n, lda, W, n, l21, m - n);
     }
 }
@@ -1321,6 +1371,12 @@ template <class T>
 static void panel_products(Solver* ctx, Lane L, int m, int n, const T* A, int lda, const T* l21, int ldl, int p,
                            bool split16, unsigned long long* maxl) {
     if (m <= n) return;   // the last panel: no update follows
+    if constexpr (std::is_same<T, float>::value) {
+        if (ctx->lu16) {
+            lu16_panel_products(ctx, L, m, n, A, lda, const_cast<float*>(l21), ldl, p, maxl);
+            return;
+        }
+    }
     tri_inverse(L, true, n, A, lda, ctx->lu.winv_l[p].as<T>());
     if constexpr (std::is_same<T, float>::value) {
         if (split16) {
@@ -1489,20 +1545,35 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
             // kernel also writes U12 into the factor (A12 is read only by the
             // U12 product before it); otherwise one copy at the end.
             bool in_factor = false;
-            gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
-            if constexpr (std::is_same<T, float>::value)
-                if (split16) in_factor = split_b(ctx, main.s, jb, jb2, U, jb, 0, 0, A12, lda);
-            trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
+            bool f16 = false;
+            if constexpr (std::is_same<T, float>::value) {
+                if (ctx->lu16) {
+                    // U12 goes into the factor with its tiles
+                    lu16_cols(ctx, main, jb, rest, 0, jb2, A12, lda, p);
+                    in_factor = f16 = true;
+                }
+            }
+            if (!f16) {
+                gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+                if constexpr (std::is_same<T, float>::value)
+                    if (split16) in_factor = split_b(ctx, main.s, jb, jb2, U, jb, 0, 0, A12, lda);
+                trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
+            }
             BH_HIP(hipEventRecord(ev_ready, main.s));
             post(Job{k, jb, p ^ 1, false});
             if (rest > jb2) {
                 T* Uw = U + (size_t)jb2 * jb;
                 T* A12w = A12 + (size_t)jb2 * lda;
-                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12w, lda, Uw, jb);
                 if constexpr (std::is_same<T, float>::value)
-                    if (split16) in_factor = split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1, A12w, lda) && in_factor;
-                trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb, A12w + jb, lda, split16,
-                                   jb2, p, 1);
+                    if (f16) lu16_cols(ctx, main, jb, rest, jb2, rest - jb2, A12, lda, p);
+                if (!f16) {
+                    gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12w, lda, Uw, jb);
+                    if constexpr (std::is_same<T, float>::value)
+                        if (split16)
+                            in_factor = split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1, A12w, lda) && in_factor;
+                    trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb, A12w + jb, lda,
+                                       split16, jb2, p, 1);
+                }
             }
             if (!in_factor) copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
             Done d;
@@ -1538,7 +1609,9 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16) {
     // outer block: the split-fp16 update runs at fp16 MFMA speed, so its K must
     // be larger for the C read/write of each update to stay below the compute
-    const int LU_NB = split16 ? ctx->nb_split : (sizeof(T) == 8 ? ctx->nb64 : ctx->nb32);
+    const bool f16 = std::is_same<T, float>::value && ctx->lu16;
+    const int LU_NB = f16 ? ctx->nb16 : split16 ? ctx->nb_split : (sizeof(T) == 8 ? ctx->nb64 : ctx->nb32);
+    BH_CHECK(!f16 || LU_NB % MM_TILE == 0, BEMHIP_ERR_ARG, "fp16 LU block must be a multiple of 256");
     BH_CHECK(LU_NB >= LEAF && LU_NB % LEAF == 0, BEMHIP_ERR_ARG, "LU block must be a multiple of 32");
     ctx->lu.panel_ws.ensure(PanelWs::BYTES);
     char* ws = ctx->lu.panel_ws.as<char>();
@@ -1566,6 +1639,30 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     if (split16) {
         ctx->split_b.ensure(sizeof(__half) * 3 * nb * (size_t)N);
         ctx->split_mx.ensure(64);
+    }
+    if (f16) {
+        ctx->lu.t_a21.ensure(mm_tile_bytes(N, (int)nb));
+        ctx->lu.t_wu.ensure(mm_tile_bytes((int)nb, (int)nb));
+        for (int p = 0; p < 2; ++p) {
+            ctx->lu.t_wl[p].ensure(mm_tile_bytes((int)nb, (int)nb));
+            ctx->lu.t_l21[p].ensure(mm_tile_bytes(N, (int)nb));
+        }
+        ctx->lu.t_a12.ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
+        ctx->lu.t_u12.ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
+        unsigned int* lg = static_cast<unsigned int*>(ctx->lu.f16log.ensure(256));
+        // scale exponent from max |diag A|: its max maps to [2^7, 2^8)
+        k_fill_u32<<<1, 64, 0, ctx->stream>>>(lg, 2, 0u);
+        k_diag_absmax<<<64, 256, 0, ctx->stream>>>(N, reinterpret_cast<const float*>(A), lg + 1);
+        BH_HIP(hipGetLastError());
+        unsigned int mb = 0;
+        BH_HIP(hipMemcpyAsync(&mb, lg + 1, sizeof(mb), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        float dm;
+        std::memcpy(&dm, &mb, sizeof(dm));
+        int e = 0;
+        if (dm > 0.0f && std::isfinite(dm)) std::frexp(dm, &e);
+        ctx->lu16_e = std::max(-100, std::min(100, e));
+        ctx->lu16_overflow = false;
     }
     ctx->split_fixed_on = false;
     ctx->split_redo = false;
@@ -1638,10 +1735,19 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
             BH_HIP(hipGetLastError());
         }
         // the next panel's columns first, then hand them to the side stream
-        gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
-        if constexpr (std::is_same<T, float>::value)
-            if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
-        trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
+        bool f16 = false;
+        if constexpr (std::is_same<T, float>::value) {
+            if (ctx->lu16) {
+                lu16_cols(ctx, main, jb, rest, 0, jb2, A12, lda, p);
+                f16 = true;
+            }
+        }
+        if (!f16) {
+            gemm_nn(main.h, jb, jb2, jb, W, jb, A12, lda, U, jb);
+            if constexpr (std::is_same<T, float>::value)
+                if (split16) split_b(ctx, main.s, jb, jb2, U, jb, 0, 0);
+            trailing_update<T>(ctx, main.s, main.h, rest, jb2, jb, l21, ldl, U, jb, A12 + jb, lda, split16, 0, p, 0);
+        }
         // the panel's small kernels only hide well behind a long update; a
         // short one is cheaper to run first and then factor the panel alone
         const bool overlap = ctx->lookahead && rest - jb2 >= ctx->la_min_rest;
@@ -1652,13 +1758,17 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         // the host had issued them all, and the main stream would idle meanwhile
         if (rest > jb2) {
             T* Uw = U + (size_t)jb2 * jb;
-            gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
             if constexpr (std::is_same<T, float>::value)
-                if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
-            trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
-                               A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
+                if (f16) lu16_cols(ctx, main, jb, rest, jb2, rest - jb2, A12, lda, p);
+            if (!f16) {
+                gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12 + (size_t)jb2 * lda, lda, Uw, jb);
+                if constexpr (std::is_same<T, float>::value)
+                    if (split16) split_b(ctx, main.s, jb, rest - jb2, Uw, jb, jb2, 1);
+                trailing_update<T>(ctx, main.s, main.h, rest, rest - jb2, jb, l21, ldl, Uw, jb,
+                                   A12 + (size_t)jb2 * lda + jb, lda, split16, jb2, p, 1);
+            }
         }
-        copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor
+        if (!f16) copy2d(main.s, jb, rest, U, jb, A12, lda);   // U12 into the factor (lu16_cols wrote it)
         if (overlap) BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
         panel_start(ctx, pl, rest, jb2, A12 + jb, lda, ipiv + k + jb, k + jb, base, p ^ 1, split16);
         const T* l21n = nullptr;
@@ -1700,6 +1810,14 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         if (!(m == 0.0f || (sc >= 0x1p-8f && sc <= 32768.0f))) ctx->split_redo = true;   // NaN fails too
     }
     ctx->split_fixed_on = false;
+    if (f16) {
+        unsigned int mb = 0;
+        BH_HIP(hipMemcpyAsync(&mb, ctx->lu.f16log.p, sizeof(mb), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        float m;
+        std::memcpy(&m, &mb, sizeof(m));
+        ctx->lu16_overflow = !(m <= 32768.0f);
+    }
     BH_CHECK(!(flags[1] & 1u), BEMHIP_ERR_HIP, "panel grid barrier timed out");
     if (flags[1] & 2u) return (int)flags[2];  // 1-based first zero pivot
     return 0;
@@ -2234,9 +2352,20 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         BH_HIP(hipEventRecord(e1, sm));
         BH_HIP(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, sm));
         lu_solve<double>(ctx, N, G64, perm, x);
-    } else if (solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64) {
+    } else if (solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64 ||
+               solver == BEMHIP_SOLVE_LU16_IR64) {
         BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "mixed solve needs the FP32 copy of G");
+        ctx->lu16 = solver == BEMHIP_SOLVE_LU16_IR64;
         int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
+        ctx->lu16 = false;
+        if (solver == BEMHIP_SOLVE_LU16_IR64 && ctx->lu16_overflow) {
+            // an operand left the fp16 range: the FP32 copy again from G and
+            // the factorisation with FP32 products
+            k_d2f_mat<<<4096, 256, 0, sm>>>((size_t)N * N, G64, G32);
+            BH_HIP(hipGetLastError());
+            info = lu_factor<float>(ctx, N, G32, ipiv, false);
+            ctx->lu16_redos += 1;
+        }
         if (ctx->split_redo) {
             // a U12 block fell outside the fixed scale's window: the FP32 copy
             // again from G, and the factorisation with a scale per block
@@ -2349,8 +2478,9 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
         lu_solve_t64(ctx, N, G64, perm, b, x);
         return;
     }
-    BH_CHECK(solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64, BEMHIP_ERR_ARG,
-             "transposed solve needs an LU-based solver");
+    BH_CHECK(solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64 ||
+                 solver == BEMHIP_SOLVE_LU16_IR64,
+             BEMHIP_ERR_ARG, "transposed solve needs an LU-based solver");
     if (tol <= 0.0) tol = IR_TOL;
     if (maxit <= 0) maxit = 30;
     lu_solve_t32(ctx, N, G32, perm, b, x);
@@ -2387,11 +2517,17 @@ void Solver::init(const Solver* cfg) {
         lookahead = cfg->lookahead;
         speculate = cfg->speculate;
         np_top = cfg->np_top;
-        nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split;
+        nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split; nb16 = cfg->nb16;
         trsv_persistent = cfg->trsv_persistent;
+        mm_reserve = cfg->mm_reserve;
         timing = cfg->timing;
         split_fixed = cfg->split_fixed;
         small_single = cfg->small_single;
+    }
+    {
+        int dev = 0;
+        BH_HIP(hipGetDevice(&dev));
+        BH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
     BH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     BH_BLAS(rocblas_create_handle(&blas));

@@ -143,6 +143,10 @@ struct LuWork {
     // L21 (two of each, alternating by step parity), inverse of U11, U12
     DevBuf winv_l[2], l21[2], winv_u, u12;
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
+    // fp16 LU (mm16): fragment tiles of A21, U11^-1, L11^-1 and L21 (the last
+    // two alternating by step parity), of the block row A12 and of U12; the
+    // max |scaled value| of every pack (float bits)
+    DevBuf t_a21, t_wu, t_wl[2], t_l21[2], t_a12, t_u12, f16log;
 };
 
 // Everything one dense solve needs: its streams, BLAS handles, work buffers,
@@ -164,6 +168,11 @@ struct Solver {
     bool np_top = true;                 // a panel's top block in one persistent launch (else 32-column steps)
     unsigned int np_epoch = 0;          // publication stamp of the persistent top-block launches
     int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
+    int nb16 = 1024;                               // fp16 LU (a multiple of 256)
+    // fp16 LU: compute units the trailing update leaves to the panel stream
+    // (its launch is persistent on the others)
+    int mm_reserve = 32;
+    int ncu = 256;   // compute units of the device
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
     // single problems of at most SMALL_MAX_N unknowns through the one-work-group
@@ -176,6 +185,12 @@ struct Solver {
     // the factorisation, which is redone with per-block scales if one fell
     // outside the window the fixed scale covers
     bool split_fixed = true;
+    // fp16 LU: every product of the factorisation on fp16 matrix cores with
+    // FP32 accumulation (mm16); operands scaled by powers of two from max |diag|
+    // (2^lu16_e), an overflow of the fp16 range detected after the
+    // factorisation (lu16_overflow), which the solver then redoes in FP32
+    bool lu16 = false, lu16_overflow = false;
+    int lu16_e = 0, lu16_redos = 0;
     bool split_fixed_on = false, split_redo = false;
     int split_eb = 0, split_calls = 0, split_nlog = 0;
     int split_redos = 0;   // factorisations redone with per-block scales (lifetime count)
@@ -234,6 +249,39 @@ struct AsmOut {
 };
 void assemble_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi,
                      uint32_t flags, const AsmOut& out);
+
+// ------------------------------------------- fp16 matrix-core products --
+// (mm16.hip) fp16 fragment-tiled operands: blocks of 32 rows (or columns) x 16 k
+// in MFMA operand order, padded to MM_TILE rows and MM_BK k
+constexpr int MM_TILE = 256;
+constexpr int MM_BK = 64;
+constexpr int MM_SUB = 0, MM_STORE = 1;
+inline int mm_pad(int n, int q) { return (n + q - 1) / q * q; }
+// bytes of the tiles of R rows (or columns) x K
+inline size_t mm_tile_bytes(int R, int K) { return (size_t)mm_pad(R, MM_TILE) * mm_pad(K, MM_BK) * 2; }
+// fp16 tiles of scale * X; kcontig: X is K x R column-major (B operand), else R x K
+// (A operand); copy (optional, B only, ld ldc) receives X at the same positions;
+// maxlog (optional) the max |scale x| as float bits, amax64 (optional, A only)
+// max |x| as the bits of a non-negative double
+void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long ld, float scale, void* tiles,
+               unsigned int* maxlog, float* copy = nullptr, long ldc = 0, unsigned long long* amax64 = nullptr);
+// C (M x N, ld ldc) -= alpha A B (MM_SUB) or = alpha A B (MM_STORE) over the first
+// K of the k extent; A: tiles of M x Ka, B: tiles of Kb x (c0 + N) read from
+// column c0 (a multiple of MM_TILE); tri 1: A lower, 2: B upper triangular;
+// max_wg > 0: a persistent launch of at most max_wg work-groups (one per CU);
+// tiles (MM_STORE only): the stored C also as fp16 tiles (as mm16_pack would
+// make them from C)
+struct Mm16Tiles {
+    void* out;                   // tile buffer
+    bool b_operand;              // B tiles (k = row of C) or A tiles (k = column of C)
+    int k;                       // k extent of the buffer (rows of C for B, columns for A)
+    int c0;                      // B: column of C's first column within the buffer (multiple of 32)
+    float scale;
+    unsigned int* maxlog;        // max |scale x| (float bits)
+    unsigned long long* amax64;  // A only: max |x| (double bits)
+};
+void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
+               int K, float* C, long ldc, float alpha, int max_wg = 0, const Mm16Tiles* tiles = nullptr);
 
 // ---------------------------------------------------------- dense LA --
 // LU with partial pivoting (column-major, lda = N), pivots as absolute rows.

@@ -1,0 +1,464 @@
+// fp16 matrix-core products of the mixed-precision LU on gfx950
+// (v_mfma_f32_32x32x16_f16: fp16 operands, FP32 accumulation).
+//
+// Replaces, for the fp16 LU of Simulation::solve_system (Simulation.cpp:57-61,
+// Eigen::PartialPivLU), every GEMM-shaped product of the blocked factorisation:
+// the trailing update A22 -= L21 U12, the block row U12 = L11^-1 A12 and the
+// rows below a panel L21 = A21 U11^-1.
+//
+// Operands live in HBM as fp16 "fragment tiles": the matrix is cut into blocks
+// of 32 rows (or columns) x 16 k, and each block is stored as the 64 lanes x
+// 8 halves (16 B per lane) that one MFMA operand register quad holds: lane l
+// carries row (l & 31) and k = 8 (l >> 5) .. +7.  A block is 1 KB contiguous,
+// so the GEMM moves it HBM -> LDS with one direct-to-LDS dwordx4 load per wave
+// (no VGPR staging) and LDS -> registers with one conflict-free ds_read_b128
+// per lane.  Blocks are ordered row-block major: block (rb, kb) at
+// (rb * KB + kb) KB, KB = padded K / 16.  The pack kernels write these tiles
+// from FP32 column-major matrices (scaled by a power of two, zero padded to
+// 256 rows and 64 k) and log max |scaled value| so that the host can detect an
+// fp16 overflow after the factorisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace bemhip {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int MM_BKB = MM_BK / 16;          // 16-k blocks per k-step (4)
+
+// One direct-to-LDS load: each lane's 16 bytes at g land at the wave-uniform
+// LDS address s + 16 lane.  Written as inline asm (not the builtin) so that
+// the compiler does not order later LDS reads of the other stage behind it;
+// the kernel waits for it explicitly (vmcnt) before a barrier.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_dma16(const h8* g, h8* s) {
+    const unsigned m = (unsigned)(size_t)(__attribute__((address_space(3))) h8*)s;
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(m), "v"(g) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+constexpr int MM_RB = MM_TILE / 32;         // 32-row blocks per macro tile (8)
+
+// ------------------------------------------------------------------ pack --
+// Both pack kernels go through LDS so that the FP32 source is read along its
+// contiguous dimension and every wave writes whole 1 KB fragment blocks.
+// maxlog: max |scale x| merged as float bits (NaN and Inf log as +Inf);
+// amax64: max |x| (unscaled) merged as the bits of a non-negative double (NaN
+// orders above every number).
+__device__ inline void pack_logs(float m, double am, unsigned int* maxlog, unsigned long long* amax64) {
+    if (maxlog) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(maxlog, __float_as_uint(m));
+    }
+    if (amax64) {
+        unsigned long long b = (unsigned long long)__double_as_longlong(am);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(b, off);
+            b = o > b ? o : b;
+        }
+        if ((threadIdx.x & 63) == 0 && b > 0ull) atomicMax(amax64, b);
+    }
+}
+
+// A operand: X is R x K column-major (element (r, k) = X[r + k ld]).  A
+// work-group packs 256 rows x 16 k (8 fragment blocks): grid (RP / 256, KB).
+__global__ __launch_bounds__(256) void k_pack_a16(int R, int K, int KB, const float* __restrict__ X, long ld,
+                                                  float scale, h8* __restrict__ out, unsigned int* __restrict__ maxlog,
+                                                  unsigned long long* __restrict__ amax64) {
+    __shared__ float s[16][256];
+    const int r0 = blockIdx.x * 256, kb = blockIdx.y, k0 = kb * 16, t = threadIdx.x;
+    const int r = r0 + t;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s[k][t] = (r < R && k0 + k < K) ? X[(size_t)(k0 + k) * ld + r] : 0.0f;
+    __syncthreads();
+    const int w = t >> 6, l = t & 63;
+    float m = 0.0f;
+    double am = 0.0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int bl = 2 * w + u;   // local row block
+        const int rl = bl * 32 + (l & 31), kk = (l >> 5) * 8;
+        h8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = s[kk + e][rl];
+            const float y = x * scale;
+            const float a = fabsf(y);
+            m = (a <= 3.0e38f) ? fmaxf(m, a) : __builtin_inff();
+            am = fmax(am, fabs((double)x));
+            if (x != x) am = __longlong_as_double(0x7ff8000000000000ll);
+            v[e] = (_Float16)y;
+        }
+        out[((size_t)(blockIdx.x * MM_RB + bl) * KB + kb) * 64 + l] = v;
+    }
+    pack_logs(m, am, maxlog, amax64);
+}
+
+// B operand: X is K x C column-major (element (k, c) = X[k + c ld]).  A
+// work-group packs 32 columns x 128 k (8 fragment blocks): grid (KP / 128,
+// CP / 32), the k chunks of a column block in consecutive work-groups (one
+// DRAM run per column).  copy (optional, ld ldc) receives the unscaled values.
+constexpr int PB_K = 128;
+__global__ __launch_bounds__(256) void k_pack_b16(int C, int K, int KB, const float* __restrict__ X, long ld,
+                                                  float scale, h8* __restrict__ out, unsigned int* __restrict__ maxlog,
+                                                  float* __restrict__ copy, long ldc) {
+    __shared__ float s[32][PB_K + 4];
+    const int c0 = blockIdx.y * 32, k0 = blockIdx.x * PB_K, t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int idx = t + 256 * j, c = idx >> 5, q = idx & 31;
+        const int col = c0 + c, k = k0 + 4 * q;
+        float4 v = {0.f, 0.f, 0.f, 0.f};
+        if (col < C) {
+            const float* src = X + (size_t)col * ld + k;
+            if (k + 3 < K) {
+                __builtin_memcpy(&v, src, 16);
+                if (copy) __builtin_memcpy(copy + (size_t)col * ldc + k, &v, 16);
+            } else {
+                float e4[4] = {0.f, 0.f, 0.f, 0.f};
+                for (int e = 0; e < 4; ++e)
+                    if (k + e < K) {
+                        e4[e] = src[e];
+                        if (copy) copy[(size_t)col * ldc + k + e] = e4[e];
+                    }
+                v = float4{e4[0], e4[1], e4[2], e4[3]};
+            }
+        }
+        *reinterpret_cast<float4*>(&s[c][4 * q]) = v;
+    }
+    __syncthreads();
+    const int w = t >> 6, l = t & 63;
+    float m = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int kl = (2 * w + u) * 16 + (l >> 5) * 8;   // local k of this lane's 8
+        const float4 p0 = *reinterpret_cast<const float4*>(&s[l & 31][kl]);
+        const float4 p1 = *reinterpret_cast<const float4*>(&s[l & 31][kl + 4]);
+        const float x[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+        h8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float y = x[e] * scale;
+            const float a = fabsf(y);
+            m = (a <= 3.0e38f) ? fmaxf(m, a) : __builtin_inff();
+            v[e] = (_Float16)y;
+        }
+        const int kb = blockIdx.x * (PB_K / 16) + 2 * w + u;
+        if (kb < KB) out[((size_t)blockIdx.y * KB + kb) * 64 + l] = v;
+    }
+    pack_logs(m, 0.0, maxlog, nullptr);
+}
+
+void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long ld, float scale, void* tiles,
+               unsigned int* maxlog, float* copy, long ldc, unsigned long long* amax64) {
+    if (R <= 0 || K <= 0) return;
+    const int RP = mm_pad(R, MM_TILE), KB = mm_pad(K, MM_BK) / 16;
+    if (kcontig) {
+        BH_CHECK(amax64 == nullptr, BEMHIP_ERR_ARG, "mm16_pack: no max |x| for B operands");
+        k_pack_b16<<<dim3((KB * 16 + PB_K - 1) / PB_K, RP / 32), 256, 0, st>>>(R, K, KB, X, ld, scale,
+                                                                              static_cast<h8*>(tiles), maxlog, copy,
+                                                                              ldc);
+    } else {
+        BH_CHECK(copy == nullptr, BEMHIP_ERR_ARG, "mm16_pack: no copy for A operands");
+        k_pack_a16<<<dim3(RP / 256, KB), 256, 0, st>>>(R, K, KB, X, ld, scale, static_cast<h8*>(tiles), maxlog,
+                                                        amax64);
+    }
+    BH_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ gemm --
+// C (M x N, column-major, ld ldc) -= alpha A B   (EPI = MM_SUB), or
+// C = alpha A B                                  (EPI = MM_STORE),
+// A: fragment tiles of M rows x K (KBa 16-k blocks per row block), B: fragment
+// tiles of K x (columns from 32-column block cb0 on; KBb blocks per column
+// block).  nks k-steps of 64 (the tiles are zero padded to them).  TRI = 1: A
+// is lower triangular (tile row tm only needs k < 256 (tm + 1)); TRI = 2: B is
+// upper triangular (tile column tn only needs k < 256 (tn + 1)).
+//
+// A work-group (4 waves) owns a 256 x 256 tile of C; wave (wm, wn) a 128 x 128
+// quarter as 4 x 4 MFMA 32x32 accumulators (256 FP32 per lane).  Per k-step of
+// 64 the work-group stages 32 KB of A and 32 KB of B in LDS by direct-to-LDS
+// loads, double-buffered.  The launch covers
+// the tiles in groups of 8 tile rows, and consecutive work-group ids (which
+// the hardware deals round-robin to the 8 XCDs) are remapped so that every XCD
+// works through a contiguous range of groups: the work-groups sharing an L2
+// share A row blocks and B column blocks.
+struct MmArgs {
+    const h8* At;        // A tiles, KBa 16-k blocks per row block
+    const h8* Bt;        // B tiles, KBb blocks per column block, read from column block cb0
+    float* C;
+    long ldc;
+    float alpha;
+    int KBa, KBb, cb0, M, N, nks, tiles_m, tiles_n, per_xcd, wg_xcd;
+    // MM_STORE only: the stored result also as fp16 tiles (tkind 1: A tiles,
+    // k = column of C; 2: B tiles, k = row of C, column blocks from tcb0),
+    // tKB blocks per row (column) block, scaled by tscale; logs as the packs
+    h8* tout;
+    int tkind, tKB, tcb0;
+    float tscale;
+    unsigned int* maxlog;
+    unsigned long long* amax64;
+};
+
+template <int EPI, int TRI>
+__global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
+    const h8* __restrict__ At = g.At;
+    const h8* __restrict__ Bt = g.Bt;
+    float* __restrict__ C = g.C;
+    const long ldc = g.ldc;
+    const float alpha = g.alpha;
+    const int KBa = g.KBa, KBb = g.KBb, cb0 = g.cb0, M = g.M, N = g.N, nks = g.nks;
+    const int tiles_m = g.tiles_m, tiles_n = g.tiles_n, per_xcd = g.per_xcd, wg_xcd = g.wg_xcd;
+    // two stages of [A 32 KB | B 32 KB]
+    constexpr int STAGE = 2 * MM_RB * MM_BKB * 64;   // h8 per stage
+    __shared__ h8 lds[2 * STAGE];
+    const int xcd = blockIdx.x & 7;
+    float tmax = 0.0f;    // max |tscale x| of the fused tiles
+    double tamax = 0.0;   // max |x|
+    for (int t = blockIdx.x >> 3; t < per_xcd; t += wg_xcd) {
+    const int L = xcd * per_xcd + t;
+    if (L >= tiles_m * tiles_n) break;
+    constexpr int GM = 8;
+    const int fm = (L / (GM * tiles_n)) * GM;
+    const int gs = min(tiles_m - fm, GM);
+    const int q = L - fm * tiles_n;
+    const int tm = fm + q % gs, tn = q / gs;
+    int kend = nks;
+    if (TRI == 1) kend = min(nks, (tm + 1) * (MM_TILE / MM_BK));
+    if (TRI == 2) kend = min(nks, (tn + 1) * (MM_TILE / MM_BK));
+    // the wave index as a scalar: every staging address is then a scalar base
+    // plus the lane's 16 bytes
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int wm = w & 1, wn = w >> 1;
+    const h8* abase = At + (size_t)tm * MM_RB * KBa * 64;
+    const h8* bbase = Bt + (size_t)(cb0 + tn * MM_RB) * KBb * 64;
+
+    // wave w stages blocks 8w .. 8w+7 of A and of B (block a: local row block
+    // a >> 2, k block a & 3; LDS slot a)
+    auto issue = [&](int ks) {
+        h8* sA = lds + (ks & 1) * STAGE;
+        h8* sB = sA + STAGE / 2;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int a = w * 8 + u;
+            const h8* src = abase + ((size_t)(a >> 2) * KBa + ks * MM_BKB + (a & 3)) * 64;
+            lds_dma16(src + l, sA + a * 64);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = w * 8 + u;
+            const h8* src = bbase + ((size_t)(b >> 2) * KBb + ks * MM_BKB + (b & 3)) * 64;
+            lds_dma16(src + l, sB + b * 64);
+        }
+    };
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x16{};
+    // Per k-step: wait for the stage and for every wave to be done with the
+    // previous one, start the next stage's loads into the other half of LDS,
+    // then read this stage's fragments and run its 64 MFMAs while those loads
+    // are in flight.
+    if (kend > 0) issue(0);
+    for (int ks = 0; ks < kend; ++ks) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (ks + 1 < kend) issue(ks + 1);
+        const h8* sA = lds + (ks & 1) * STAGE;
+        const h8* sB = sA + STAGE / 2;
+#pragma unroll
+        for (int s = 0; s < MM_BKB; ++s) {
+            h8 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = sA[((wm * 4 + i) * MM_BKB + s) * 64 + l];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = sB[((wn * 4 + j) * MM_BKB + s) * 64 + l];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    // Epilogue through LDS: the accumulators hold a column per lane, C is
+    // column-major, so each wave writes 32 columns x 128 rows of its quarter at
+    // a time into its own 17 KB of LDS (columns padded to 132 floats: the
+    // 16-byte writes of 16 lanes hit distinct banks) and reads them back with
+    // the lanes along the rows: every global access of a wave is two
+    // contiguous 512-byte column segments.
+    // accumulator layout of v_mfma_f32_32x32x16_f16: lane l holds column
+    // (l & 31) and rows 8 g + 4 (l >> 5) + e of register 4 g + e
+    constexpr int CP = 132;
+    __syncthreads();   // every wave is done with the staging buffers
+    float* ep = reinterpret_cast<float*>(lds) + w * (32 * CP);
+    const int row0 = tm * MM_TILE + wm * 128, col0 = tn * MM_TILE + wn * 128;
+    const int rr = 4 * (l & 31);   // this lane's 4 rows (of 128) when reading back
+    const bool rows_ok = row0 + rr + 3 < M;
+    // C of the next 32-column chunk is loaded while the current one is
+    // transposed and written: 16 loads of 16 bytes per lane in flight
+    float4 cv[2][16];
+    auto load_chunk = [&](int j, float4* c) {
+        if (EPI != MM_SUB) return;
+#pragma unroll
+        for (int c2 = 0; c2 < 16; ++c2) {
+            const int col = col0 + j * 32 + 2 * c2 + (l >> 5);
+            const float* cc = C + (size_t)col * ldc + row0 + rr;
+            if (col < N && rows_ok) __builtin_memcpy(&c[c2], cc, 16);
+            else c[c2] = float4{0.f, 0.f, 0.f, 0.f};
+        }
+    };
+    load_chunk(0, cv[0]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j + 1 < 4) load_chunk(j + 1, cv[(j + 1) & 1]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float4 v;
+                v.x = acc[i][j][4 * g];
+                v.y = acc[i][j][4 * g + 1];
+                v.z = acc[i][j][4 * g + 2];
+                v.w = acc[i][j][4 * g + 3];
+                *reinterpret_cast<float4*>(ep + (l & 31) * CP + i * 32 + g * 8 + (l >> 5) * 4) = v;
+            }
+        if (EPI == MM_STORE && g.tkind == 1) {
+            // A tiles of C: this chunk is 4 row blocks x 2 k blocks
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+                    const int kb = (col0 + j * 32) / 16 + kk;
+                    if (kb >= g.tKB) continue;
+                    const int rl = ib * 32 + (l & 31), kl = kk * 16 + (l >> 5) * 8;
+                    h8 v;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float x = alpha * ep[(kl + e) * CP + rl];
+                        const float y = g.tscale * x;
+                        const float a = fabsf(y);
+                        tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
+                        tamax = fmax(tamax, fabs((double)x));
+                        if (x != x) tamax = __longlong_as_double(0x7ff8000000000000ll);
+                        v[e] = (_Float16)y;
+                    }
+                    g.tout[((size_t)((row0 + ib * 32) / 32) * g.tKB + kb) * 64 + l] = v;
+                }
+        } else if (EPI == MM_STORE && g.tkind == 2) {
+            // B tiles of C: this chunk is one column block x 8 k blocks
+            const int cb = g.tcb0 + (col0 + j * 32) / 32;
+#pragma unroll
+            for (int kbl = 0; kbl < 8; ++kbl) {
+                const int kb = row0 / 16 + kbl;
+                if (kb >= g.tKB) continue;
+                const int kl = kbl * 16 + (l >> 5) * 8;
+                const float4 p0 = *reinterpret_cast<const float4*>(ep + (l & 31) * CP + kl);
+                const float4 p1 = *reinterpret_cast<const float4*>(ep + (l & 31) * CP + kl + 4);
+                const float xs[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                h8 v;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float y = g.tscale * (alpha * xs[e]);
+                    const float a = fabsf(y);
+                    tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
+                    v[e] = (_Float16)y;
+                }
+                g.tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
+            }
+        }
+#pragma unroll
+        for (int c2 = 0; c2 < 16; ++c2) {
+            const int cl = 2 * c2 + (l >> 5);          // column within the 32
+            const int col = col0 + j * 32 + cl;
+            const float4 a = *reinterpret_cast<const float4*>(ep + cl * CP + rr);
+            if (col >= N) continue;
+            float* cc = C + (size_t)col * ldc + row0 + rr;
+            if (rows_ok) {
+                float4 v;
+                if (EPI == MM_SUB) {
+                    v = cv[j & 1][c2];
+                    v.x -= alpha * a.x;
+                    v.y -= alpha * a.y;
+                    v.z -= alpha * a.z;
+                    v.w -= alpha * a.w;
+                } else {
+                    v.x = alpha * a.x;
+                    v.y = alpha * a.y;
+                    v.z = alpha * a.z;
+                    v.w = alpha * a.w;
+                }
+                __builtin_memcpy(cc, &v, 16);
+            } else {
+                const float av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (row0 + rr + e < M) {
+                        if (EPI == MM_SUB) cc[e] -= alpha * av[e];
+                        else cc[e] = alpha * av[e];
+                    }
+            }
+        }
+    }
+    __syncthreads();   // the epilogue's LDS is the next tile's staging buffer
+    }
+    if (EPI == MM_STORE && g.tkind) pack_logs(tmax, tamax, g.maxlog, g.tkind == 1 ? g.amax64 : nullptr);
+}
+
+// max_wg > 0: at most that many work-groups (one per compute unit), each
+// looping over tiles, so that the remaining compute units stay free for
+// kernels of another stream
+void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
+               int K, float* C, long ldc, float alpha, int max_wg, const Mm16Tiles* tiles) {
+    if (M <= 0 || N <= 0) return;
+    BH_CHECK(c0 % MM_TILE == 0, BEMHIP_ERR_ARG, "mm16: column offset must be a multiple of 256");
+    BH_CHECK(K <= Ka && K <= Kb, BEMHIP_ERR_ARG, "mm16: K exceeds the operand tiles");
+    BH_CHECK(!(tiles && epi != MM_STORE), BEMHIP_ERR_ARG, "mm16: tiles of the result only for stores");
+    MmArgs g{};
+    g.At = static_cast<const h8*>(At);
+    g.Bt = static_cast<const h8*>(Bt);
+    g.C = C;
+    g.ldc = ldc;
+    g.alpha = alpha;
+    g.KBa = mm_pad(Ka, MM_BK) / 16;
+    g.KBb = mm_pad(Kb, MM_BK) / 16;
+    g.cb0 = c0 / 32;
+    g.M = M;
+    g.N = N;
+    g.nks = mm_pad(K, MM_BK) / MM_BK;
+    g.tiles_m = (M + MM_TILE - 1) / MM_TILE;
+    g.tiles_n = (N + MM_TILE - 1) / MM_TILE;
+    g.per_xcd = (g.tiles_m * g.tiles_n + 7) / 8;
+    g.wg_xcd = max_wg > 0 ? std::max(1, std::min(g.per_xcd, max_wg / 8)) : g.per_xcd;
+    if (tiles) {
+        g.tout = static_cast<h8*>(tiles->out);
+        g.tkind = tiles->b_operand ? 2 : 1;
+        g.tKB = mm_pad(tiles->k, MM_BK) / 16;
+        g.tcb0 = tiles->c0 / 32;
+        g.tscale = tiles->scale;
+        g.maxlog = tiles->maxlog;
+        g.amax64 = tiles->amax64;
+        BH_CHECK(tiles->c0 % 32 == 0, BEMHIP_ERR_ARG, "mm16: tile column offset must be a multiple of 32");
+    }
+    const dim3 grid(8 * g.wg_xcd);
+    if (epi == MM_SUB) {
+        BH_CHECK(tri == 0, BEMHIP_ERR_ARG, "mm16: triangular operands only for stores");
+        k_mm16<MM_SUB, 0><<<grid, 256, 0, st>>>(g);
+    } else if (tri == 1) {
+        k_mm16<MM_STORE, 1><<<grid, 256, 0, st>>>(g);
+    } else if (tri == 2) {
+        k_mm16<MM_STORE, 2><<<grid, 256, 0, st>>>(g);
+    } else {
+        k_mm16<MM_STORE, 0><<<grid, 256, 0, st>>>(g);
+    }
+    BH_HIP(hipGetLastError());
+}
+
+}  // namespace bemhip

@@ -39,7 +39,7 @@ import bemhip  # noqa: E402
 import meshgen  # noqa: E402
 
 METRIC = "ms/RK4 stage (assembly+solve) at N=40962; mixed-prec vs FP64 GPU @1e-8 rel"
-PEAK = {"lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
+PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
 # HBM bytes per trailing-update GEMM launch at nu = 64, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain 1 GiB copy):
 # profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md

@@ -53,10 +53,14 @@ typedef enum {
     BEMHIP_SOLVE_LU64 = 0,        /* FP64 blocked GEPP (Eigen::PartialPivLU semantics)          */
     BEMHIP_SOLVE_LU32_IR64 = 1,   /* FP32 GEPP factor + FP64-residual iterative refinement       */
     BEMHIP_SOLVE_BICGSTAB64 = 2,  /* FP64 BiCGSTAB, diagonal preconditioner (Eigen::BiCGSTAB)    */
-    BEMHIP_SOLVE_LU16X3_IR64 = 3  /* FP32 GEPP whose trailing updates run as split-fp16 x3 MFMA
+    BEMHIP_SOLVE_LU16X3_IR64 = 3, /* FP32 GEPP whose trailing updates run as split-fp16 x3 MFMA
                                      products with FP32 accumulation, + FP64 refinement          */
+    BEMHIP_SOLVE_LU16_IR64 = 4    /* GEPP with every block product (trailing updates, L11^-1 A12,
+                                     A21 U11^-1) on fp16 MFMA with FP32 accumulation and an FP32
+                                     factor, + FP64 refinement (2-3 sweeps); falls back to FP32
+                                     products if an operand leaves the fp16 range                */
 } bemhip_solver;
-/* Refinement of the two mixed solvers: x = LU \ b, then sweeps
+/* Refinement of the mixed solvers: x = LU \ b, then sweeps
  * x += LU \ (b - G x) with the FP64 residual.  The loop stops before a sweep
  * once |b - G x|_inf <= 1e-10 |b|_inf, or after a sweep once the correction
  * |d|_inf <= tol |x|_inf (tol <= 0: 1e-9); `iterations` counts the sweeps. */
@@ -107,9 +111,10 @@ const char* bemhip_last_error(const bemhip_ctx* ctx);
 void bemhip_quadrature(double* out35);
 
 /* ---- ColocSim::assemble_matrices (ColocSim.cpp:21-110) ------------------
- * G (and H when non-null) receive the column-major N x N matrices of the
+ * G and H, when non-null, receive the column-major N x N matrices of the
  * reference, H including its 4-pi diagonal.  When phi and Hphi are both
- * non-null, Hphi = H phi is formed inside the assembly without storing H. */
+ * non-null, Hphi = H phi is formed inside the assembly without storing H
+ * (then G and H may both be null). */
 int bemhip_assemble(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
                     uint32_t flags, double* G, double* H, const double* phi, double* Hphi);
 

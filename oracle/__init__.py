@@ -43,42 +43,51 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
-        L.bo_quadrature.argtypes = [f64p] * 5
-        L.bo_panel.argtypes = [f64p, f64p, f64p, f64p, C.c_int, C.c_int, f64p, f64p]
-        L.bo_assemble.argtypes = [C.c_int64, f64p, C.c_int64, u32p, C.c_int, f64p, f64p, C.c_int]
-        L.bo_gemv.argtypes = [C.c_int64, f64p, f64p, f64p, C.c_int]
-        L.bo_lu_factor.argtypes = [C.c_int64, f64p, i64p, C.c_int]
-        L.bo_lu_factor.restype = C.c_int64
-        L.bo_lu_solve.argtypes = [C.c_int64, f64p, i64p, f64p]
-        L.bo_bicgstab.argtypes = [C.c_int64, f64p, f64p, f64p, C.c_double, C.c_int64, C.POINTER(C.c_double), C.c_int]
-        L.bo_bicgstab.restype = C.c_int64
-        for nm in ("bo_triangle_normals", "bo_vertex_normals"):
-            getattr(L, nm).argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p]
-        L.bo_tangent_gradients.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
-        L.bo_curvatures.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
-        L.bo_kappa.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p]
-        L.bo_components.argtypes = [C.c_int64, C.c_int64, u32p, i64p]
-        L.bo_components.restype = C.c_int64
-        L.bo_part_volumes.argtypes = [C.c_int64, f64p, C.c_int64, u32p, i64p, C.c_int64, f64p]
-        L.bo_position_t.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, f64p]
-        L.bo_pot_t_multi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, C.c_double, f64p]
-        L.bo_rk4_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
-                                  C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
-        L.bo_rk4_step.restype = C.c_int
-        L.bo_euler_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
-                                    C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
-        L.bo_pin_boundary.argtypes = [C.c_int64, f64p, C.c_int64, u32p, C.c_void_p, C.c_void_p]
-        L.bo_pin_boundary.restype = C.c_int64
-        L.bo_drift_velocity.argtypes = [C.c_double]
-        L.bo_drift_velocity.restype = C.c_double
-        L.bo_euler_step.restype = C.c_int
-        L.bo_compute_psi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
-        L.bo_quadrature19.argtypes = [f64p] * 3
-        L.bo_exterior_panel.argtypes = [f64p] * 6
-        L.bo_exterior_pot.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p, C.c_int]
-        _lib = L
+        _lib = bind(_LIB)
     return _lib
+
+
+def bind(path):
+    """Loads an oracle build at `path` with its argument types (the mutation
+    tests bind builds with planted mistakes this way)."""
+    L = C.CDLL(path)
+    L.bo_quadrature.argtypes = [f64p] * 5
+    L.bo_panel.argtypes = [f64p, f64p, f64p, f64p, C.c_int, C.c_int, f64p, f64p]
+    L.bo_assemble.argtypes = [C.c_int64, f64p, C.c_int64, u32p, C.c_int, f64p, f64p, C.c_int]
+    L.bo_gemv.argtypes = [C.c_int64, f64p, f64p, f64p, C.c_int]
+    L.bo_lu_factor.argtypes = [C.c_int64, f64p, i64p, C.c_int]
+    L.bo_lu_factor.restype = C.c_int64
+    L.bo_lu_solve.argtypes = [C.c_int64, f64p, i64p, f64p]
+    L.bo_bicgstab.argtypes = [C.c_int64, f64p, f64p, f64p, C.c_double, C.c_int64, C.POINTER(C.c_double), C.c_int]
+    L.bo_bicgstab.restype = C.c_int64
+    for nm in ("bo_triangle_normals", "bo_vertex_normals"):
+        getattr(L, nm).argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p]
+    L.bo_tangent_gradients.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
+    L.bo_curvatures.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
+    L.bo_kappa.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p]
+    L.bo_components.argtypes = [C.c_int64, C.c_int64, u32p, i64p]
+    L.bo_components.restype = C.c_int64
+    L.bo_part_volumes.argtypes = [C.c_int64, f64p, C.c_int64, u32p, i64p, C.c_int64, f64p]
+    L.bo_position_t.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, f64p]
+    L.bo_pot_t_multi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, C.c_double, f64p]
+    L.bo_rk4_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
+                              C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.bo_rk4_step.restype = C.c_int
+    L.bo_euler_step.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p,
+                                C.POINTER(C.c_double), C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.bo_pin_boundary.argtypes = [C.c_int64, f64p, C.c_int64, u32p, C.c_void_p, C.c_void_p]
+    L.bo_pin_boundary.restype = C.c_int64
+    L.bo_drift_velocity.argtypes = [C.c_double]
+    L.bo_drift_velocity.restype = C.c_double
+    L.bo_euler_step.restype = C.c_int
+    L.bo_compute_psi.argtypes = [C.POINTER(Params), C.c_int64, f64p, C.c_int64, u32p, f64p, f64p]
+    L.bo_quadrature19.argtypes = [f64p] * 3
+    L.bo_exterior_panel.argtypes = [f64p] * 6
+    L.bo_exterior_pot.argtypes = [C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p, C.c_int]
+    L.bo_get_dt.argtypes = [C.POINTER(Params), C.c_int64, C.c_double, f64p, f64p]
+    L.bo_get_dt.restype = C.c_double
+    L.bo_nopenetration.argtypes = [C.c_int64, C.c_double, C.c_double, f64p, f64p, f64p]
+    return L
 
 
 def _c(a, dt):
@@ -277,10 +286,25 @@ def pot_t_multi(params, verts, tris, vel, t):
     return out
 
 
+def get_dt(params, dp, vel, dpot):
+    v = _c(vel, np.float64)
+    return lib().bo_get_dt(C.byref(params), v.shape[0], float(dp), v.reshape(-1), _c(dpot, np.float64))
+
+
+def nopenetration(eps, dt, x0, c):
+    """(xf, c') of LinLinSim::nopenetration; c' is c with the rewritten c.x"""
+    x = _c(x0, np.float64)
+    cc = np.array(c, dtype=np.float64, copy=True)
+    xf = np.zeros_like(x)
+    lib().bo_nopenetration(x.shape[0], float(eps), float(dt), x.reshape(-1), cc.reshape(-1), xf.reshape(-1))
+    return xf, cc
+
+
 class Sim:
     """Oracle state stepper mirroring LinLinSim (x, phi, psi, time)."""
 
-    def __init__(self, params, verts, tris, phi):
+    def __init__(self, params, verts, tris, phi, library=None):
+        self.lib = library if library is not None else lib()
         self.p = params
         self.x = np.array(verts, dtype=np.float64, copy=True)
         self.t = _c(tris, np.uint32)
@@ -291,7 +315,7 @@ class Sim:
     def rk4(self, dp, fixdt=True):
         tm = C.c_double(self.time)
         dt = C.c_double(0.0)
-        rc = lib().bo_rk4_step(C.byref(self.p), self.x.shape[0], self.x.reshape(-1), self.t.shape[0],
+        rc = self.lib.bo_rk4_step(C.byref(self.p), self.x.shape[0], self.x.reshape(-1), self.t.shape[0],
                                self.t.reshape(-1), self.phi, self.psi, C.byref(tm), float(dp), int(fixdt), C.byref(dt))
         if rc != 0:
             raise IndexError("negative values")   # LinLinSim.cpp:157 throws out_of_range
@@ -301,7 +325,7 @@ class Sim:
     def euler(self, dp, fixdt=True):
         tm = C.c_double(self.time)
         dt = C.c_double(0.0)
-        lib().bo_euler_step(C.byref(self.p), self.x.shape[0], self.x.reshape(-1), self.t.shape[0],
+        self.lib.bo_euler_step(C.byref(self.p), self.x.shape[0], self.x.reshape(-1), self.t.shape[0],
                             self.t.reshape(-1), self.phi, self.psi, C.byref(tm), float(dp), int(fixdt), C.byref(dt))
         self.time = tm.value
         return dt.value

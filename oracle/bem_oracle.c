@@ -11,6 +11,11 @@
  */
 #include "bem_oracle.h"
 
+/* BO_MUTANT selects a planted mistake for the mutation tests (0: none). */
+#ifndef BO_MUTANT
+#define BO_MUTANT 0
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -824,17 +829,48 @@ void bo_pot_t_multi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
 }
 
 /* Simulation::get_dt (Simulation.cpp:77-92) */
-static double get_dt(const bo_params* P, int64_t N, double dp, const double* vel, const double* dpot) {
+double bo_get_dt(const bo_params* P, int64_t N, double dp, const double* vel, const double* dpot) {
     double max_pot = 0.0, max_vel = 0.0;
     for (int64_t i = 0; i < N; ++i) {
         max_pot = fmax(fabs(dpot[i]), max_pot);
         const v3 v = V(vel + 3 * i);
         max_vel = fmax(vdot(v, v), max_vel);
     }
+#if BO_MUTANT != 4
     max_vel = sqrt(max_vel);
+#endif
     double dt = dp / (max_pot + P->dp_balance * max_vel);
     if (P->min_dt > 0.0) dt = fmin(dt, P->min_dt);
     return dt;
+}
+
+/* LinLinSim::nopenetration (LinLinSim.cpp:162-190): xf = x0 + dt c, except
+ * that a vertex moving towards the wall x = 0 (c.x dt < 0) inside the band
+ * x < eps, or entering it within the step, follows an exponential approach
+ * instead; its c.x is rewritten to the effective velocity (xf.x - x0.x)/dt. */
+void bo_nopenetration(int64_t N, double eps, double dt, const double* x0, double* c, double* xf) {
+    for (int64_t i = 0; i < N; ++i) {
+        for (int d = 0; d < 3; ++d) xf[3 * i + d] = dt * c[3 * i + d] + x0[3 * i + d];
+        const double cx = c[3 * i];
+        if (!(cx * dt < 0.0)) continue;
+        int flag = 0;
+        const double xpos = x0[3 * i];
+        if (xpos < eps) {
+            xf[3 * i] = xpos * exp(cx / xpos * dt);
+            flag = 1;
+        } else {
+            const double te = (eps - xpos) / cx;
+            if (fabs(te) < fabs(dt)) {
+#if BO_MUTANT == 5
+                xf[3 * i] = eps * exp(cx / eps * dt);
+#else
+                xf[3 * i] = eps * exp(cx / eps * (dt - te));
+#endif
+                flag = 1;
+            }
+        }
+        if (flag) c[3 * i] = (xf[3 * i] - xpos) / dt;
+    }
 }
 
 static int any_negative_x(int64_t N, const double* xyz) {  /* test_negative, LinLinSim.cpp:148-159 */
@@ -862,7 +898,7 @@ int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uin
     /* phi is the stage potential p1, whose pinned entries position_t rewrites */
     position_impl(P, N, x1, M, tri, phi, psi, kx[0], NULL);
     bo_pot_t_multi(P, N, x1, M, tri, kx[0], t0, kp[0]);
-    dt = fixdt ? dp : get_dt(P, N, dp, kx[0], kp[0]);
+    dt = fixdt ? dp : bo_get_dt(P, N, dp, kx[0], kp[0]);
     const double ts[4] = {t0, P->frozen_stage_time ? t0 : t0 + 0.5 * dt,
                           P->frozen_stage_time ? t0 : t0 + 0.5 * dt, P->frozen_stage_time ? t0 : t0 + dt};
     const double cs[3] = {0.5 * dt, 0.5 * dt, dt};
@@ -874,31 +910,27 @@ int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uin
     }
     /* average = (k1 + 2 k2 + 2 k3 + k4) / 6 (LinLinSim.cpp:236) */
     double* avg = xs;
+#if BO_MUTANT >= 1 && BO_MUTANT <= 3
+    /* planted mistakes for the mutation tests: 1 forward Euler (k1 only),
+     * 2 midpoint RK2 (k2 only), 3 equal stage weights (k1+k2+k3+k4)/4 */
+    static const double wm[3][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0.25, 0.25, 0.25, 0.25}};
+    const double* w = wm[BO_MUTANT - 1];
+    for (size_t e = 0; e < n3; ++e)
+        avg[e] = w[0] * kx[0][e] + w[1] * kx[1][e] + w[2] * kx[2][e] + w[3] * kx[3][e];
+    for (int64_t i = 0; i < N; ++i)
+        phi[i] = dt * (w[0] * kp[0][i] + w[1] * kp[1][i] + w[2] * kp[2][i] + w[3] * kp[3][i]) + phi[i];
+#else
     for (size_t e = 0; e < n3; ++e)
         avg[e] = (((kx[0][e] + 2.0 * kx[1][e]) + 2.0 * kx[2][e]) + kx[3][e]) * (1.0 / 6.0);
     /* pf (LinLinSim.cpp:238) */
     for (int64_t i = 0; i < N; ++i)
         phi[i] = (dt / 6.0) * (((kp[0][i] + 2.0 * kp[1][i]) + 2.0 * kp[2][i]) + kp[3][i]) + phi[i];
-    /* nopenetration (LinLinSim.cpp:162-190); the plain update without guards */
-    for (int64_t i = 0; i < N; ++i) {
-        for (int d = 0; d < 3; ++d) xyz[3 * i + d] = dt * avg[3 * i + d] + x1[3 * i + d];
-        if (!P->wall_guards) continue;
-        const double cx = avg[3 * i];
-        if (cx * dt < 0.0) {
-            int flag = 0;
-            const double xpos = x1[3 * i];
-            if (xpos < P->wall_eps) {
-                xyz[3 * i] = xpos * exp(cx / xpos * dt);
-                flag = 1;
-            } else {
-                const double te = (P->wall_eps - xpos) / cx;
-                if (fabs(te) < fabs(dt)) {
-                    xyz[3 * i] = P->wall_eps * exp(cx / P->wall_eps * (dt - te));
-                    flag = 1;
-                }
-            }
-            if (flag) avg[3 * i] = (xyz[3 * i] - xpos) / dt;
-        }
+#endif
+    /* nopenetration (LinLinSim.cpp:240), applied with the wall guards only */
+    if (P->wall_guards) {
+        bo_nopenetration(N, P->wall_eps, dt, x1, avg, xyz);
+    } else {
+        for (size_t e = 0; e < n3; ++e) xyz[e] = dt * avg[e] + x1[e];
     }
     /* psi = average . vertex normal of the new mesh (LinLinSim.cpp:243-249) */
     double* vn = kx[0];
@@ -919,7 +951,7 @@ int bo_euler_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const u
     double* dpot = (double*)malloc(sizeof(double) * N);
     position_impl(P, N, xyz, M, tri, phi, psi, vel, NULL);
     bo_pot_t_multi(P, N, xyz, M, tri, vel, *time, dpot);
-    const double dt = fixdt ? dp : get_dt(P, N, dp, vel, dpot);
+    const double dt = fixdt ? dp : bo_get_dt(P, N, dp, vel, dpot);
     for (size_t e = 0; e < n3; ++e) xyz[e] = vel[e] * dt + xyz[e];
     for (int64_t i = 0; i < N; ++i) phi[i] = dt * dpot[i] + phi[i];
     *time += dt;

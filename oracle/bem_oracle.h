@@ -105,6 +105,11 @@ void bo_pot_t_multi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
 /* LinLinSim::evolve_system_RK4 (LinLinSim.cpp:194-257).  xyz, phi, psi, time
  * are updated in place; returns 0, or -1 when a wall-mode vertex has x < 0
  * (the reference throws std::out_of_range). *dt_out receives dt. */
+/* Simulation::get_dt (Simulation.cpp:77-92): dp / (max|dpot| + dp_balance max|vel|), capped by min_dt > 0 */
+double bo_get_dt(const bo_params* P, int64_t N, double dp, const double* vel, const double* dpot);
+/* LinLinSim::nopenetration (LinLinSim.cpp:162-190): xf from x0 and the velocities c (N x 3); rewrites c.x
+ * of the vertices it slows down */
+void bo_nopenetration(int64_t N, double eps, double dt, const double* x0, double* c, double* xf);
 int bo_rk4_step(const bo_params* P, int64_t N, double* xyz, int64_t M, const uint32_t* tri,
                 double* phi, double* psi, double* time, double dp, int fixdt, double* dt_out);
 /* LinLinSim::evolve_system (Euler, LinLinSim.cpp:260-284); psi (the state's
