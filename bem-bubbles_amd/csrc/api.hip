@@ -420,6 +420,8 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (sp && sp[0] == '0') ctx->speculate = false;
         const char* nt = std::getenv("BEMHIP_NP_TOP");
         if (nt && nt[0] == '0') ctx->np_top = false;
+        const char* n32 = std::getenv("BEMHIP_NP_TOP32");
+        if (n32 && n32[0] == '0') ctx->np_top32 = false;
         if (const char* v = std::getenv("BEMHIP_NB64")) ctx->nb64 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB32")) ctx->nb32 = std::atoi(v);
         if (const char* v = std::getenv("BEMHIP_NB_SPLIT")) ctx->nb_split = std::atoi(v);
@@ -579,7 +581,7 @@ int bemhip_sim_create(bemhip_ctx* ctx, const bemhip_sim_params* p, int64_t N, co
     int rc = guarded(ctx, [&] {
         BH_CHECK(p && xyz && tri, BEMHIP_ERR_ARG, "null argument");
         BH_CHECK(!(p->flags & ~(BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)), BEMHIP_ERR_ARG, "unknown flag");
-        BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
+        BH_CHECK(p->solver >= 0 && p->solver <= BEMHIP_SOLVE_LU16_IR64, BEMHIP_ERR_ARG, "unknown solver");
         s->ctx = ctx;
         s->P = *p;
         s->N = N;
@@ -738,7 +740,7 @@ int bemhip_sim_set_time(bemhip_sim* s, double t) {
 
 int bemhip_sim_set_solver(bemhip_sim* s, int solver, double tol, int32_t maxit) {
     if (!s) return BEMHIP_ERR_ARG;
-    if (solver < 0 || solver > 3) { s->ctx->err = "unknown solver"; return BEMHIP_ERR_ARG; }
+    if (solver < 0 || solver > BEMHIP_SOLVE_LU16_IR64) { s->ctx->err = "unknown solver"; return BEMHIP_ERR_ARG; }
     s->P.solver = solver;
     s->P.tol = tol;
     s->P.maxit = maxit;
@@ -751,7 +753,7 @@ int bemhip_sim_set_params(bemhip_sim* s, const bemhip_sim_params* p) {
         BH_CHECK(!(p->flags & ~(BEMHIP_MIRROR_X0 | BEMHIP_INDEPENDENT_PARTS)), BEMHIP_ERR_ARG, "unknown flag");
         BH_CHECK(!((p->flags ^ s->P.flags) & BEMHIP_INDEPENDENT_PARTS), BEMHIP_ERR_ARG,
                  "BEMHIP_INDEPENDENT_PARTS is fixed when the simulation is created");
-        BH_CHECK(p->solver >= 0 && p->solver <= 3, BEMHIP_ERR_ARG, "unknown solver");
+        BH_CHECK(p->solver >= 0 && p->solver <= BEMHIP_SOLVE_LU16_IR64, BEMHIP_ERR_ARG, "unknown solver");
         const double v0 = s->P.V_0;
         s->P = *p;
         if (std::isnan(p->V_0)) s->P.V_0 = v0;

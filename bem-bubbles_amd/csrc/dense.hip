@@ -715,6 +715,223 @@ __global__ __launch_bounds__(top_max_threads<T>()) void k_np_top(int n, T* __res
     }
 }
 
+// k order of a 32x32x2 product whose row operand comes straight from an
+// accumulator (lane half kh holds columns 8 g + 4 kh + e, g, e < 4): step kk
+// covers k = 8 (kk / 4) + 4 kh + kk % 4, the same k for both operands
+__device__ inline int kperm(int kk, int kh) { return 8 * (kk >> 2) + 4 * kh + (kk & 3); }
+__device__ inline floatx16 mfma_rows32p(const float* __restrict__ W, const floatx16& xv, int lane) {
+    floatx16 acc = {};
+    const int c = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(W[kperm(kk, kh) * NPB + c], xv[kk], acc, 0, 0, 0);
+    return acc;
+}
+
+// FP32 top block, the same stripe algorithm as k_np_top, with every work-group's
+// stripe held in registers for the whole launch (its row blocks in the
+// accumulator layout of the 32x32 products: lane row rb * 32 + lane % 32,
+// register q column 8 (q / 4) + 4 (lane / 32) + q % 4): a step j only reads
+// stripe j's multipliers and L^-1 (published by its work-group) and updates
+// registers, where k_np_top read and wrote the whole stripe in memory every
+// step.  A row block that another wave needs (the step's own row block, the
+// diagonal block) goes through LDS.  Only final values are stored: the U rows
+// of every step, the factored diagonal block and the multipliers below it.
+// backup (optional, ld n) receives the block's original values.
+__global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, int lda,
+                                                  unsigned int* __restrict__ sync, unsigned int epoch,
+                                                  float* __restrict__ linv, float* __restrict__ backup) {
+    constexpr int MAXW = 8;
+    __shared__ float Us[NPB * NPB];      // U rows of step j: Us[i * NPB + c]
+    __shared__ float Wt[NPB * NPB];      // published L^-1 of stripe j (transposed) / own U^-1
+    __shared__ float Xs[NPB][NPB + 1];   // a row block of this stripe handed between waves: Xs[i][c]
+    __shared__ float Ld[NPB][NPB + 1];   // own factored diagonal block
+    __shared__ float invd[NPB];
+    __shared__ unsigned long long red[MAXW];
+    __shared__ int s_stripe, s_go;
+    unsigned int* ticket = sync + 2;
+    unsigned int* err = sync + 3;
+    unsigned int* flags = sync + 16;
+    const int tid = threadIdx.x, lane = tid & 63, nt = blockDim.x;
+    const int wv = tid >> 6, nw = nt >> 6;
+    const int li = lane & 31, kh = lane >> 5;
+    if (tid == 0) s_stripe = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int s = s_stripe;
+    const int c0 = s * NPB, w = min(NPB, n - c0);
+    float* S = A + (size_t)c0 * lda;
+    floatx16 sv[TOP_RBW];
+#pragma unroll
+    for (int h = 0; h < TOP_RBW; ++h) {
+        const int rb = top_rb(wv, h, nw, n), i = rb * NPB + li;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = 8 * (q >> 2) + 4 * kh + (q & 3);
+            float v = 0.0f;
+            if (rb >= 0 && i < n && c < w) {
+                v = S[(size_t)c * lda + i];
+                if (backup) backup[(size_t)(c0 + c) * n + i] = v;
+            }
+            sv[h][q] = v;
+        }
+    }
+    // the wave holding row block rb writes it to Xs
+    auto hand_over = [&](int rb) {
+        if (wv == rb % nw) {
+            const int h = rb / nw;
+#pragma unroll
+            for (int hh = 0; hh < TOP_RBW; ++hh)
+                if (hh == h)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) Xs[li][8 * (q >> 2) + 4 * kh + (q & 3)] = sv[hh][q];
+        }
+    };
+    unsigned long long lbits = 0;
+    for (int j = 0; j < s; ++j) {
+        const int r0 = j * NPB;
+        if (tid == 0) {
+            int go = 1;
+            unsigned int spins = 0;
+            while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > (1u << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    go = 0;
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;   // error raised: results invalid, every work-group leaves
+        const float* Lj = A + (size_t)r0 * lda;   // stripe j (published)
+        float xb[TOP_RBW][16];
+#pragma unroll
+        for (int h = 0; h < TOP_RBW; ++h) {
+            const int rb = top_rb(wv, h, nw, n), i = min(max(rb, 0) * NPB + li, n - 1);
+            const bool act = rb > j;
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk)
+                xb[h][kk] = act ? ld_agent(Lj + (size_t)(2 * kk + kh) * lda + i) : 0.0f;
+        }
+        for (int e = tid; e < NPB * NPB; e += nt) Wt[e] = ld_agent(linv + (size_t)j * NPB * NPB + e);
+        hand_over(j);
+        __syncthreads();
+        // U rows r0.. of this stripe = L_jj^-1 A[r0.., stripe] (final values)
+        if (wv == 0) {
+            float xv[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) xv[kk] = Xs[2 * kk + kh][li];
+            const floatx16 acc = mfma_rows32(Wt, xv, lane);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int i = 8 * (q >> 2) + 4 * kh + (q & 3);
+                Us[i * NPB + li] = acc[q];
+                if (li < w) S[(size_t)li * lda + r0 + i] = acc[q];
+            }
+        }
+        __syncthreads();
+        // rank-32 update of this stripe's row blocks below, in registers
+#pragma unroll
+        for (int h = 0; h < TOP_RBW; ++h) {
+            const int rb = top_rb(wv, h, nw, n);
+            if (rb > j) {
+                const floatx16 acc = mfma_rows32(Us, xb[h], lane);
+                sv[h] -= acc;
+            }
+        }
+        __syncthreads();   // Us / Wt / Xs are refilled by the next step
+    }
+    // own diagonal block: its row block from the holding wave, factored by
+    // wave 0 in registers (pivot row by v_readlane)
+    hand_over(s);
+    __syncthreads();
+    if (tid < 64) {
+        const bool row = lane < w;
+        float rw[NPB];
+        unsigned int fbits = 0;
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) rw[k] = (row && k < w) ? Xs[lane & 31][k] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < NPB; ++j) {
+            if (j < w) {
+                const float piv = lane_value(rw[j], j);
+                const float inv = __builtin_amdgcn_rcpf(piv);
+                const bool below = row && lane > j;
+                const float l = below ? rw[j] * inv : 0.0f;
+                if (below) rw[j] = l;
+                const unsigned int lb = __float_as_uint(fabsf(l));
+                fbits = lb > fbits ? lb : fbits;
+                float pk[NPB];
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
+#pragma unroll
+                for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
+                if (lane == j) invd[j] = inv;
+            } else if (lane == j) {
+                invd[j] = 0.0f;
+            }
+        }
+        if (lane < NPB) {
+#pragma unroll
+            for (int k = 0; k < NPB; ++k) {
+                Ld[lane][k] = (row && k < w) ? rw[k] : 0.0f;
+                if (row && k < w) st_agent(&S[(size_t)k * lda + c0 + lane], rw[k]);
+            }
+        }
+        const unsigned long long fd = fbits > 0x7F800000u ? 0x7FF8000000000000ull
+                                                          : (unsigned long long)__double_as_longlong((double)__uint_as_float(fbits));
+        lbits = fd > lbits ? fd : lbits;
+    }
+    __syncthreads();
+    const bool below = c0 + NPB < n;
+    {
+        const float* Lb = &Ld[0][0] + opaque_zero();
+        if (wv == (1 % nw) && lane < NPB && w == NPB)
+            unit_lower_inverse(Lb, lane, [&](int e, float v) { st_agent(linv + (size_t)s * NPB * NPB + e, v); });
+        if (wv == (2 % nw) && lane < NPB && below) upper_inverse(Lb, invd, lane, Wt);
+    }
+    __syncthreads();
+    // multipliers of the row blocks below: L = A U^-1, the row operand from registers
+    unsigned int fb = 0;
+#pragma unroll
+    for (int h = 0; h < TOP_RBW; ++h) {
+        const int rb = top_rb(wv, h, nw, n);
+        if (below && rb > s) {
+            const int i = rb * NPB + li;
+            const floatx16 acc = mfma_rows32p(Wt, sv[h], lane);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int c = 8 * (q >> 2) + 4 * kh + (q & 3);
+                if (i < n) {
+                    const float lq = acc[q];
+                    st_agent(S + (size_t)c * lda + i, lq);
+                    const unsigned int lb = __float_as_uint(fabsf(lq));
+                    fb = lb > fb ? lb : fb;
+                }
+            }
+        }
+    }
+    const unsigned long long fbd = fb > 0x7F800000u ? 0x7FF8000000000000ull
+                                                    : (unsigned long long)__double_as_longlong((double)__uint_as_float(fb));
+    lbits = fbd > lbits ? fbd : lbits;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(lbits, off);
+        lbits = o > lbits ? o : lbits;
+    }
+    if (lane == 0) red[wv] = lbits;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long bits = red[0];
+        for (int q = 1; q < nw; ++q) bits = red[q] > bits ? red[q] : bits;
+        atomicMax(reinterpret_cast<unsigned long long*>(sync), bits);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // the factored diagonal block from k_np_step's scratch into A
 template <class T>
 __device__ inline void np_diag_store(int bs, const T* __restrict__ dscr, T* __restrict__ Adiag, int lda) {
@@ -1323,11 +1540,22 @@ static void panel_rec(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipi
 template <class T>
 static void np_steps(Lane L, int n, T* A, int lda, unsigned long long* maxl, T* dscr);
 
+// the FP32 top block in one launch with the stripes in registers (it also
+// writes the backup of the block)
+template <class T> static bool top32(const Solver* ctx, int n) {
+    return std::is_same<T, float>::value && ctx->np_top && ctx->np_top32 && (n + NPB - 1) / NPB <= TOP_STRIPES &&
+           n <= top_max_rows<T>();
+}
+
 template <class T>
 static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned int* sync, T* l21) {
     unsigned long long* maxl = reinterpret_cast<unsigned long long*>(sync);
     const int stripes = (n + NPB - 1) / NPB;
-    if (ctx->np_top && stripes <= TOP_STRIPES && n <= top_max_rows<T>()) {
+    if (top32<T>(ctx, n)) {
+        if constexpr (std::is_same<T, float>::value)
+            k_np_top32<<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, ctx->lu.diag.as<T>(),
+                                                               ctx->lu.backup.as<T>());
+    } else if (ctx->np_top && stripes <= TOP_STRIPES && n <= top_max_rows<T>()) {
         T* linv = ctx->lu.diag.as<T>();
         k_np_top<T><<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, linv);
     } else {
@@ -1404,7 +1632,7 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
         panel_products(ctx, L, m, n, A, lda, A + n, lda, p, split16, nullptr);
         return;
     }
-    copy2d(L.s, n, n, A, lda, ctx->lu.backup.as<T>(), n);
+    if (!top32<T>(ctx, n)) copy2d(L.s, n, n, A, lda, ctx->lu.backup.as<T>(), n);
     // sync words: max |l| (bits of a non-negative double), ticket, error, flags
     unsigned int* sync = ctx->lu.maxl.as<unsigned int>();
     k_fill_u32<<<1, 64, 0, L.s>>>(sync, 4, 0u);
@@ -2517,6 +2745,7 @@ void Solver::init(const Solver* cfg) {
         lookahead = cfg->lookahead;
         speculate = cfg->speculate;
         np_top = cfg->np_top;
+        np_top32 = cfg->np_top32;
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split; nb16 = cfg->nb16;
         trsv_persistent = cfg->trsv_persistent;
         mm_reserve = cfg->mm_reserve;

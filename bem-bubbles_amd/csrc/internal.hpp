@@ -166,6 +166,7 @@ struct Solver {
     bool lookahead = true;
     bool speculate = true;              // panels without interchanges, verified against GEPP's choice
     bool np_top = true;                 // a panel's top block in one persistent launch (else 32-column steps)
+    bool np_top32 = true;               // FP32: that launch with the stripes held in registers
     unsigned int np_epoch = 0;          // publication stamp of the persistent top-block launches
     int nb64 = 768, nb32 = 512, nb_split = 1024;   // LU outer block per precision (measured at N = 40962)
     int nb16 = 1024;                               // fp16 LU (a multiple of 256)
