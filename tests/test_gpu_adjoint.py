@@ -103,7 +103,7 @@ def test_adjoint_workload_central_differences(ctx):
     V0 = fwd.get_volume()
     target = _radii_gpu(fwd, case.tris, x0, meshgen.adjoint_target_phi(case, seed=1), steps, dt)
     grads = {}
-    for solver in ("lu64", "lu16x3_ir64"):
+    for solver in ("lu64", "lu16x3_ir64", "lu16_ir64"):
         sim = bemhip.ColocSim(ctx, x0, case.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, field_amp=ph.field_amp,
                               field_freq=ph.field_freq, solver=solver, V_0=V0)
         sim.set_phi(phi0)
@@ -122,7 +122,7 @@ def test_adjoint_workload_central_differences(ctx):
             ad = float(np.sum(gx * dx) + np.sum(gp * dp))
             worst[solver] = max(worst[solver], abs(fd - ad) / abs(fd))
     say(f"  worst relative FD mismatch over 20 directions: {worst}")
-    assert worst["lu64"] <= 1e-5 and worst["lu16x3_ir64"] <= 1e-5
+    assert worst["lu64"] <= 1e-5 and worst["lu16x3_ir64"] <= 1e-5 and worst["lu16_ir64"] <= 1e-5
 
 
 def test_adjoint_torch_function(ctx):

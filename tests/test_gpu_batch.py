@@ -42,7 +42,7 @@ def _cases(seed=0):
     return out
 
 
-@pytest.mark.parametrize("solver,tol", [("lu64", 1e-12), ("lu16x3_ir64", 1e-8)])
+@pytest.mark.parametrize("solver,tol", [("lu64", 1e-12), ("lu16x3_ir64", 1e-8), ("lu16_ir64", 1e-8)])
 def test_batched_compute_psi_equals_single(ctx, oracle_lib, solver, tol):
     say(f"batch compute_psi {solver}")
     cases = _cases()
@@ -60,7 +60,7 @@ def test_batched_compute_psi_equals_single(ctx, oracle_lib, solver, tol):
     assert rel(coupled, psi) > 1e-6
 
 
-@pytest.mark.parametrize("solver", ["lu64", "lu16x3_ir64"])
+@pytest.mark.parametrize("solver", ["lu64", "lu16x3_ir64", "lu16_ir64"])
 def test_batched_rk4_equals_separate_simulations(ctx, oracle_lib, solver):
     say(f"batch rk4 {solver}")
     cases = _cases(seed=3)

@@ -75,7 +75,7 @@ def test_exterior_potential_bad_inputs(ctx):
         ctx.exterior_potential(v, t, one, one, np.array([[np.nan, 0, 0]]))
 
 
-@pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu16x3_ir64", 1e-8)])
+@pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu16x3_ir64", 1e-8), ("lu16_ir64", 1e-8)])
 def test_sim_exterior_pot_matches_oracle(ctx, oracle_lib, solver, tol):
     say(f"sim exterior_pot {solver}")
     case = meshgen.config_small(seed=0)

@@ -98,7 +98,7 @@ def test_speculative_panels_equal_gepp(oracle_lib, monkeypatch):
     monkeypatch.setenv("BEMHIP_LU_SPECULATE", "1")
     monkeypatch.setenv("BEMHIP_NP_TOP", "0")
     c3 = bemhip.Context(0)
-    for solver, tol in (("lu64", 1e-12), ("lu16x3_ir64", 1e-10)):
+    for solver, tol in (("lu64", 1e-12), ("lu16x3_ir64", 1e-10), ("lu16_ir64", 1e-8)):
         x3, st3 = c3.solve(G, b, solver)
         assert st3["panels_pivoted"] == 0 and rel(x3, x1) < tol
     c3.close()
@@ -134,8 +134,39 @@ def test_split_fixed_scale_and_redo(oracle_lib, monkeypatch):
     assert rel(xa1, np.linalg.solve(A, bA)) < 1e-8
 
 
+def test_lu16_with_pivoting(ctx):
+    """fp16 products with GEPP panels: a row-permuted diagonally dominant
+    matrix (well conditioned, but partial pivoting must interchange rows in
+    every panel) is solved to the FP64 refinement's accuracy."""
+    say("lu16 pivoted panels N=2600")
+    rng = np.random.default_rng(21)
+    n = 2600
+    D = rng.standard_normal((n, n)) + 3.0 * np.sqrt(n) * np.eye(n)
+    A = D[rng.permutation(n)]
+    b = rng.standard_normal(n)
+    x, st = ctx.solve(A, b, "lu16_ir64")
+    say(f"  {st}")
+    assert st["converged"] == 1 and st["panels_pivoted"] == st["panels"] > 1
+    assert rel(x, np.linalg.solve(A, b)) < 1e-10
+
+
+def test_lu16_overflow_falls_back_to_fp32(ctx):
+    """Operands scaled from max |diag A| overflow fp16 when the off-diagonal
+    entries are ~1000x the diagonal: the factorisation is redone with FP32
+    products, which then is the lu32_ir64 factorisation exactly."""
+    say("lu16 fp16 overflow -> FP32 redo")
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=(2600, 2600))
+    A[np.diag_indices(2600)] *= 1e-3
+    b = rng.normal(size=2600)
+    x16, s16 = ctx.solve(A, b, "lu16_ir64")
+    x32, s32 = ctx.solve(A, b, "lu32_ir64")
+    assert np.array_equal(x16, x32) and s16["iterations"] == s32["iterations"]
+    assert rel(x16, np.linalg.solve(A, b)) < 1e-8
+
+
 @pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu32_ir64", 1e-8), ("lu16x3_ir64", 1e-8),
-                                        ("bicgstab64", 1e-10)])
+                                        ("lu16_ir64", 1e-8), ("bicgstab64", 1e-10)])
 def test_solvers_on_assembled_g(ctx, oracle_lib, solver, tol):
     say(f"solve {solver}")
     v, t = meshgen.icosphere(12)
@@ -285,7 +316,7 @@ def test_large_sphere_properties(ctx):
     Y = meshgen.real_sph_harm(2, v)[(2, 1)]
     phi = -1.0 + 0.02 * Y
     p64, _ = ctx.compute_psi(v, t, phi, "lu64")
-    for solver in ("lu32_ir64", "lu16x3_ir64"):
+    for solver in ("lu32_ir64", "lu16x3_ir64", "lu16_ir64"):
         p, st = ctx.compute_psi(v, t, phi, solver)
         say(solver, st)
         assert rel(p, p64) < 1e-8
@@ -303,6 +334,7 @@ def test_large_speculative_lu_equals_gepp(monkeypatch):
     c1 = bemhip.Context(0)
     p1, st1 = c1.compute_psi(v, t, phi, "lu64")
     pm, stm = c1.compute_psi(v, t, phi, "lu16x3_ir64")
+    p16, st16 = c1.compute_psi(v, t, phi, "lu16_ir64")
     c1.close()
     monkeypatch.setenv("BEMHIP_LU_SPECULATE", "0")
     c2 = bemhip.Context(0)
@@ -312,3 +344,4 @@ def test_large_speculative_lu_equals_gepp(monkeypatch):
     assert st1["panels_pivoted"] == 0 and st2["panels_pivoted"] == st2["panels"]
     assert rel(p1, p2) < 1e-12
     assert rel(pm, p2) < 1e-8
+    assert st16["panels_pivoted"] == 0 and rel(p16, p2) < 1e-8

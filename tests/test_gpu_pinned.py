@@ -61,7 +61,7 @@ def test_pinned_compute_psi_hemisphere(ctx, oracle_lib):
     assert rel(got, want) < 1e-10
 
 
-@pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu16x3_ir64", 1e-8)])
+@pytest.mark.parametrize("solver,tol", [("lu64", 1e-10), ("lu16x3_ir64", 1e-8), ("lu16_ir64", 1e-8)])
 def test_pinned_rk4_reference_input(ctx, oracle_lib, solver, tol):
     """two RK4 steps (dt = 1e-3, fixed) of the reference's pinned bubble from
     rest (phi = 0, as pinned-bubble.cpp:123), wall mode, vs the oracle"""
