@@ -23,14 +23,10 @@ import weakref
 
 import numpy as np
 
-# Hardware queues of the HIP runtime (read once, when the process first
-# initialises HIP; its default is 4).  A batch of independent problems is
-# solved on up to 8 solver lanes of two streams each; with 4 queues those 16
-# streams share 4 in-order hardware queues and the lanes serialise behind each
-# other (measured on MI355X, 72 bubbles of N = 10242: 919 ms per RK4 stage
-# with 4 queues, 668 with 16, 956 with 32).  A value already in the
-# environment is kept (an application sets it before loading the library).
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# The library leaves the HIP runtime's hardware-queue count
+# (GPU_MAX_HW_QUEUES, read once when the process first initialises HIP,
+# default 4) to the application.  A batch of independent problems is solved
+# on up to 8 solver lanes of two streams each, which then share those queues.
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbemhip.so")

@@ -7,10 +7,12 @@ degree 1..8, fixed RK4 step dt = 1e-3, free-space (no wall), sigma = 0.
 
 A bench "step" is one RK4 step (four stages, each = assembly of G with the
 fused H phi + dense solve + stage update), all state resident in HBM.  The
-reported value is ms per RK4 stage of the mixed-precision path (split-fp16 x3
-MFMA LU with FP32 accumulation + FP64 iterative refinement); the plain FP64
-path (FP64 LU) is timed on the same state for the speed ratio, and the psi of
-both paths is compared (target <= 1e-8 relative).
+reported value is ms per RK4 stage of the mixed-precision path (LU whose block
+products run on fp16 matrix cores with FP32 accumulation and an FP32 factor,
++ FP64 iterative refinement); the plain FP64 paths (FP64 LU, and FP64
+BiCGSTAB, the reference's default solver, at relative residual 1e-10) are
+timed on the same state for the speed ratio, and the psi of every path is
+compared with the FP64 LU's (target <= 1e-8 relative).
 
 Other workloads (one JSON line each): --workload two-bubbles (two coupled
 spheres, N = 20484, mixed vs FP64 over 10 RK4 steps), --workload adjoint (loss
@@ -30,57 +32,87 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bem-bubbles_amd"))
-# 16 hardware queues for the runtime (read when HIP initialises, before the
-# library is loaded): the batch's 8 solver lanes drive 16 streams, which 4
-# queues (the runtime default) serialise
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
-
 import bemhip  # noqa: E402
 import meshgen  # noqa: E402
 
 METRIC = "ms/RK4 stage (assembly+solve) at N=40962; mixed-prec vs FP64 GPU @1e-8 rel"
 PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
-# HBM bytes per trailing-update GEMM launch at nu = 64, from two rocprofv3 --pmc
-# runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors of a plain 1 GiB copy):
-# profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md
+# HBM bytes per trailing-update GEMM launch at nu = 64 (average over the
+# factorisation's launches), from two rocprofv3 --pmc runs (FETCH_SIZE, then
+# WRITE_SIZE, each scaled by what it reads for a plain copy of known size):
+# profiles/lu16_update_gemm_nu64_hbm_traffic.md (fp16 LU),
+# profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md (split-fp16 LU)
 TRAFFIC_NU64 = {"lu16x3_ir64": 5.99e9}
+# algorithmic bytes of the same launches (C read + written in FP32, the fp16
+# operand tiles read once), average per launch
+ALGO_BYTES_NU64 = {}
 # MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
 # GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
 # profiles/lu_update_gemm_nu64_mfma_utilisation.md
 MFMA_BUSY_NU64 = {"lu16x3_ir64": (0.511, 1.82)}
+DTYPE = {"lu16_ir64": "fp16-mfma lu (fp32 acc, fp32 factor)+f64-ir", "lu16x3_ir64": "f32-lu(split-f16 mfma)+f64-ir",
+         "lu32_ir64": "f32-lu+f64-ir", "lu64": "f64", "bicgstab64": "f64"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(threads):
-    """Oracle (FP64 C/OpenMP restatement of the reference's loop structure)
-    timed on a bounded sample: one compute_psi stage at nu = 32 (N = 10242),
-    assembly scaled by (40962/10242)^2 and the LU by (40962/10242)^3."""
+    """The reference's CPU path on the host cores, on a bounded sample:
+    assembly (+ H phi) by the oracle (FP64 C/OpenMP restatement of
+    ColocSim::assemble_matrices' triangle-parallel loop with its critical
+    scatter and 4-pi pass) at nu = 48 (N = 23042), scaled to nu = 64 by the
+    work ratio N M; the dense LU by LAPACK dgetrf (scipy, the blocked
+    right-looking GEPP that Eigen's PartialPivLU also is) on the leading
+    12000 x 12000 block of that G, scaled by (40962 / 12000)^3.  Without
+    scipy the oracle's own cache-blocked GEPP takes the LU's place."""
     sys.path.insert(0, ROOT)
     import oracle
     oracle.build()
-    v, t = meshgen.icosphere(32)
-    n = len(v)
+    v, t = meshgen.icosphere(48)
+    n, m = len(v), len(t)
     t0 = time.perf_counter()
     G, H = oracle.assemble(v, t, nthreads=threads)
     t1 = time.perf_counter()
     phi = -1.0 + 0.02 * meshgen.random_sh_field(v, 1, 8, 0)
-    b = oracle.gemv(H, phi, nthreads=threads)
+    oracle.gemv(H, phi, nthreads=threads)
     del H
     t2 = time.perf_counter()
-    LU, piv, _ = oracle.lu_factor(G, nthreads=threads)
-    oracle.lu_solve(LU, piv, b)
-    t3 = time.perf_counter()
-    del G, LU
-    s = 40962.0 / n
-    est_ms = ((t1 - t0) + (t2 - t1)) * s * s * 1e3 + (t3 - t2) * s ** 3 * 1e3
-    return {"value": round(est_ms, 1), "unit": "ms/stage (extrapolated to N=40962)", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle compute_psi at nu=32 (N={n}): assembly {t1 - t0:.2f}s, H*phi {t2 - t1:.2f}s, "
-                      f"LU+solve {t3 - t2:.2f}s on {threads} threads; assembly/GEMV scaled by (N/n)^2, "
-                      f"LU by (N/n)^3"}
+    nl = 12000
+    A = np.asfortranarray(G[:nl, :nl])
+    del G
+    try:
+        import scipy.linalg
+        lu_kind = "LAPACK dgetrf (scipy)"
+        t3 = time.perf_counter()
+        scipy.linalg.lu_factor(A, overwrite_a=True, check_finite=False)
+    except ImportError:
+        lu_kind = "oracle blocked GEPP"
+        t3 = time.perf_counter()
+        oracle.lu_factor_blocked(A, nthreads=threads, in_place=True)
+    t4 = time.perf_counter()
+    del A
+    N, M = 40962.0, 81920.0
+    s_asm = N * M / (n * m)
+    s_lu = (N / nl) ** 3
+    est_ms = ((t1 - t0) + (t2 - t1)) * s_asm * 1e3 + (t4 - t3) * s_lu * 1e3
+    return {"value": round(est_ms, 1), "unit": "ms/stage (N=40962, from a bounded sample)", "cores": threads,
+            "kind": "port", "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": f"oracle assembly at nu=48 (N={n}, M={m}) {t1 - t0:.2f}s + H*phi {t2 - t1:.2f}s, scaled by "
+                      f"N*M ({s_asm:.2f}x); {lu_kind} of a {nl}x{nl} block of that G {t4 - t3:.2f}s "
+                      f"({2.0 / 3.0 * nl ** 3 / (t4 - t3) / 1e9:.0f} GFLOP/s), scaled by (N/{nl})^3 "
+                      f"({s_lu:.1f}x); {threads} threads"}
 
 
 def batch_bench(args):
@@ -119,8 +151,7 @@ def batch_bench(args):
         "value": round(ms_stage, 3), "unit": "ms/stage", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-lu(split-f16 mfma)+f64-ir" if args.solver == "lu16x3_ir64" else
-                 ("f32-lu+f64-ir" if args.solver == "lu32_ir64" else "f64"),
+        "dtype": DTYPE[args.solver],
         "data": "synthetic",
         "config": {"workload": f"{len(cases)} independent spheres, nu in "
                                f"{{{args.batch_nu or '4,8,16,32'}}} (of {args.batch} drawn from {{4,8,16,32}}), R~U[0.5,2], "
@@ -162,7 +193,7 @@ def two_bubbles_bench(args):
     N, M = len(c.verts), len(c.tris)
     ctx = bemhip.Context(0)
     res = {}
-    for solver in ("lu16x3_ir64", "lu64"):
+    for solver in (args.solver, "lu64"):
         sim = bemhip.ColocSim(ctx, c.verts, c.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, solver=solver)
         for w in range(args.warmup):   # warmup steps from the initial state, then back to it
             sim.set_phi(c.phi)
@@ -181,7 +212,7 @@ def two_bubbles_bench(args):
         log(f"[bench] two bubbles {solver}: {res[solver][0]:.2f} ms/stage")
         sim.close()
     ctx.close()
-    ms, x, phi, psi, acc = res["lu16x3_ir64"]
+    ms, x, phi, psi, acc = res[args.solver]
     ms64, x64, phi64, psi64, _ = res["lu64"]
 
     def rel(a, b):
@@ -191,11 +222,11 @@ def two_bubbles_bench(args):
         "metric": "ms/RK4 stage (assembly+solve), two coupled bubbles N=20484, mixed vs FP64 over 10 steps",
         "value": round(ms, 3), "unit": "ms/stage", "n_gpus": 1, "steps": c.steps, "warmup": args.warmup,
         "ms_per_step": round(4 * ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-lu(split-f16 mfma)+f64-ir", "data": "synthetic",
+        "dtype": DTYPE[args.solver], "data": "synthetic",
         "config": {"workload": "two spheres nu=32 (N=20484, M=40960), R=1.0 at x=-1.5 and 0.8 at x=+1.5, "
                                "Rdot=+0.5/-0.5, jitter 1e-3 seed 0, RK4 dt=5e-4, 10 steps, coupled (one problem)",
-                   "N": N, "M": M, "solver": "lu16x3_ir64",
-                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+                   "N": N, "M": M, "solver": args.solver,
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default (4)")},
         "fp64_ms_per_stage": round(ms64, 3), "speedup_vs_fp64": round(ms64 / ms, 3),
         "after_10_steps_rel_diff_vs_fp64": {"x": rel(x, x64), "phi": rel(phi, phi64), "psi": rel(psi, psi64)},
         "refinement_sweeps_per_stage": acc["iterations"] / (4 * c.steps),
@@ -223,7 +254,7 @@ def adjoint_bench(args):
         target.append((3.0 * fwd.get_volume() / (4.0 * np.pi)) ** (1.0 / 3.0))
     target = np.array(target)
     fwd.close()
-    for solver in ("lu16x3_ir64", "lu64"):
+    for solver in (args.solver, "lu64"):
         sim = bemhip.ColocSim(ctx, c.verts, c.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, field_amp=ph.field_amp,
                               field_freq=ph.field_freq, solver=solver, V_0=V0)
         sim.set_phi(c.phi)
@@ -242,15 +273,15 @@ def adjoint_bench(args):
         log(f"[bench] adjoint {solver}: {out_s[solver]}")
         sim.close()
     ctx.close()
-    m = out_s["lu16x3_ir64"]
+    m = out_s[args.solver]
     out = {
         "metric": "ms per loss+gradient (50 RK4 steps, checkpoints every 10), sphere N=2562",
         "value": m["grad_ms"], "unit": "ms/gradient", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": m["grad_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 state, f32-lu(split-f16 mfma)+f64-ir solves", "data": "synthetic",
+        "dtype": "f64 state, " + DTYPE[args.solver] + " solves", "data": "synthetic",
         "config": {"workload": "sphere nu=16 (N=2562) at rest, p_inf(t)=1+0.3 sin(pi t), 50 RK4 steps dt=1e-2, "
                                "checkpoint every 10, target from 1% perturbed phi (seed 1)",
-                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default (4)")},
         "per_solver": out_s,
     }
     print(json.dumps(out), flush=True)
@@ -265,7 +296,7 @@ def exterior_bench(args):
     case = meshgen.config_large(seed=0, nu=64)
     ph = case.physics
     ctx = bemhip.Context(0)
-    sim = bemhip.ColocSim(ctx, case.verts, case.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, solver="lu16x3_ir64")
+    sim = bemhip.ColocSim(ctx, case.verts, case.tris, ph.p_inf, ph.epsilon, ph.sigma, ph.gamma, solver=args.solver)
     sim.set_phi(case.phi)
     sim.compute_psi()
     psi = sim.get_psi()
@@ -308,7 +339,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nu", type=int, default=64)
-    ap.add_argument("--solver", default="lu16x3_ir64", choices=sorted(bemhip.SOLVERS))
+    ap.add_argument("--solver", default="lu16_ir64", choices=sorted(bemhip.SOLVERS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "two-bubbles", "adjoint", "exterior"],
@@ -377,13 +408,12 @@ def main():
         "metric": METRIC, "value": round(ms_stage, 3), "unit": "ms/stage", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-lu(split-f16 mfma)+f64-ir" if args.solver == "lu16x3_ir64" else
-                 ("f32-lu+f64-ir" if args.solver == "lu32_ir64" else "f64"),
+        "dtype": DTYPE[args.solver],
         "data": "synthetic",
         "config": {"workload": f"one sphere, nu={args.nu} icosphere (N={N}, M={M}), phi=-1+0.02*SH(l=1..8) seed 0, "
                                f"RK4 dt={case.dt}, one bench step = one RK4 step = 4 stages",
                    "N": N, "M": M, "solver": args.solver, "stages_timed": stages,
-                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES")},
+                   "hip_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default (4)")},
         "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
         "refinement_sweeps_per_stage": acc["iterations"] / stages,
         "lu_panels": acc["panels"], "lu_panels_pivoted": acc["panels_pivoted"],
@@ -391,15 +421,30 @@ def main():
     gf, gms = acc["gemm_flops"], acc["gemm_ms"]
     if gms > 0:
         achieved = gf / (gms * 1e-3) / 1e12
-        out["roofline"] = {"bound": "mfma", "kernel": "LU trailing-update GEMM", "achieved": round(achieved, 2),
-                           "peak": round(PEAK[args.solver], 1), "unit": "TFLOP/s",
-                           "frac": round(achieved / PEAK[args.solver], 4),
-                           "traffic": TRAFFIC_NU64.get(args.solver) if args.nu == 64 else None,
-                           "launches": acc["gemm_launches"], "avg_launch_ms": round(gms / max(1, acc["gemm_launches"]), 4)}
+        launches = acc["gemm_launches"]
+        rl = {"bound": "mfma", "kernel": "LU trailing update " + (
+                  "bemhip::k_mm16<0,0> (fp16 MFMA, FP32 accumulation)" if args.solver == "lu16_ir64" else
+                  "(rocBLAS GEMM)"),
+              "achieved": round(achieved, 2), "peak": round(PEAK[args.solver], 1), "unit": "TFLOP/s",
+              "frac": round(achieved / PEAK[args.solver], 4),
+              "traffic": TRAFFIC_NU64.get(args.solver) if args.nu == 64 else None,
+              "launches": launches, "avg_launch_ms": round(gms / max(1, launches), 4)}
+        if args.solver == "lu16_ir64":
+            # algorithmic bytes: C read and written in FP32, the fp16 L21 and U12
+            # tiles read once, over the factorisation's update launches
+            nb = int(os.environ.get("BEMHIP_NB16", "1024"))
+            shapes = lu_update_shapes(N, nb)
+            ab = sum(8.0 * m * n + 2.0 * (m + n) * k for m, n, k in shapes)
+            fl = sum(2.0 * m * n * k for m, n, k in shapes)
+            if abs(fl - gf / stages) < 1e-6 * fl:
+                rl["algorithmic_bytes_per_launch"] = round(ab / len(shapes))
+                rl["hbm_achieved_GBps"] = round(ab * stages / (gms * 1e-3) / 1e9, 1)
+                rl["hbm_frac"] = round(ab * stages / (gms * 1e-3) / 8e12, 4)
         if args.nu == 64 and args.solver in MFMA_BUSY_NU64:
             busy, clk = MFMA_BUSY_NU64[args.solver]
-            out["roofline"]["mfma_busy_pmc"] = busy
-            out["roofline"]["engine_clock_ghz_pmc"] = clk
+            rl["mfma_busy_pmc"] = busy
+            rl["engine_clock_ghz_pmc"] = clk
+        out["roofline"] = rl
 
     # assembly: algorithmic output bytes (G in FP64, plus its FP32 copy for the
     # mixed solvers) and algorithmic flops (35 per quadrature point, 7 points
@@ -413,7 +458,9 @@ def main():
                        "achieved_fp64_TFLOPs": round(asm_flops / (asm_ms * 1e-3) / 1e12, 2),
                        "fp64_frac": round(asm_flops / (asm_ms * 1e-3) / 78.6e12, 4), "bound": "fp64 valu"}
 
-    # plain FP64 path on the same state (timing + accuracy of the mixed psi)
+    # the plain FP64 paths on the same state: the FP64 LU (the ratio of the
+    # metric) and BiCGSTAB (the reference's default solver) at relative
+    # residual 1e-10; psi of every path against the FP64 LU's
     if not args.no_fp64:
         sim.compute_psi()
         psi_mixed = sim.get_psi()
@@ -421,29 +468,61 @@ def main():
         sim.compute_psi()
         psi64 = sim.get_psi()
         err = float(np.max(np.abs(psi_mixed - psi64)) / np.max(np.abs(psi64)))
-        x, phi, psi, tm = sim.state()
+        sim.set_solver("bicgstab64", tol=1e-10)
+        sim.compute_psi()
+        errb = float(np.max(np.abs(sim.get_psi() - psi64)) / np.max(np.abs(psi64)))
+        sim.set_solver("lu64")
         sim.accum_stats(reset=True)
         t0 = time.perf_counter()
         sim.evolve_system_RK4(case.dt, True)
         fp64_ms = (time.perf_counter() - t0) * 1e3 / 4
         a64 = sim.accum_stats(reset=True)
-        log(f"[bench] fp64 {fp64_ms:.2f} ms/stage err={err:.3e} {a64}")
+        log(f"[bench] fp64 lu {fp64_ms:.2f} ms/stage err={err:.3e} {a64}")
         out["fp64_ms_per_stage"] = round(fp64_ms, 3)
         out["speedup_vs_fp64"] = round(fp64_ms / ms_stage, 3)
         out["psi_rel_err_vs_fp64"] = err
         if a64["gemm_ms"] > 0:
             out["fp64_gemm_tflops"] = round(a64["gemm_flops"] / (a64["gemm_ms"] * 1e-3) / 1e12, 2)
+        sim.set_solver("bicgstab64", tol=1e-10)
+        sim.accum_stats(reset=True)
+        t0 = time.perf_counter()
+        sim.evolve_system_RK4(case.dt, True)
+        bcg_ms = (time.perf_counter() - t0) * 1e3 / 4
+        ab = sim.accum_stats(reset=True)
+        log(f"[bench] fp64 bicgstab {bcg_ms:.2f} ms/stage err={errb:.3e} {ab}")
+        out["fp64_bicgstab"] = {"ms_per_stage": round(bcg_ms, 3), "tol": 1e-10,
+                                "iterations_per_stage": ab["iterations"] / 4.0,
+                                "psi_rel_err_vs_fp64_lu": errb,
+                                "speedup_of_mixed": round(bcg_ms / ms_stage, 3)}
     sim.close()
     ctx.close()
 
     if not args.no_cpu:
         try:
-            threads = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            threads = min(16, os.cpu_count() or 1)
+            threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        except (AttributeError, ValueError):
+            threads = os.cpu_count() or 1
         log(f"[bench] cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(threads)
     print(json.dumps(out), flush=True)
+
+
+def lu_update_shapes(N, nb):
+    """(m, n, k) of the fp16 LU's trailing-update launches: per step, the next
+    panel's columns first, then the rest (the look-ahead schedule)."""
+    out = []
+    k = 0
+    while k < N:
+        jb = min(nb, N - k)
+        rest = N - k - jb
+        if rest <= 0:
+            break
+        jb2 = min(nb, rest)
+        out.append((rest, jb2, jb))
+        if rest > jb2:
+            out.append((rest, rest - jb2, jb))
+        k += jb
+    return out
 
 
 if __name__ == "__main__":

@@ -87,6 +87,8 @@ def bind(path):
     L.bo_get_dt.argtypes = [C.POINTER(Params), C.c_int64, C.c_double, f64p, f64p]
     L.bo_get_dt.restype = C.c_double
     L.bo_nopenetration.argtypes = [C.c_int64, C.c_double, C.c_double, f64p, f64p, f64p]
+    L.bo_lu_factor_blocked.argtypes = [C.c_int64, C.c_void_p, i64p, C.c_int, C.c_int64, C.c_int64]
+    L.bo_lu_factor_blocked.restype = C.c_int64
     return L
 
 
@@ -150,6 +152,16 @@ def lu_factor(A, nthreads=0):
     LU = np.array(A, dtype=np.float64, order="F", copy=True)
     piv = np.zeros(LU.shape[0], dtype=np.int64)
     info = lib().bo_lu_factor(LU.shape[0], LU.reshape(-1, order="F"), piv, nthreads)
+    return LU, piv, int(info)
+
+
+def lu_factor_blocked(A, nthreads=0, nb=256, max_panels=0, in_place=False):
+    """Cache-blocked GEPP (the CPU baseline's LU); in_place factors a Fortran-
+    ordered float64 A without a copy."""
+    LU = A if in_place else np.array(A, dtype=np.float64, order="F", copy=True)
+    assert LU.flags.f_contiguous and LU.dtype == np.float64
+    piv = np.zeros(LU.shape[0], dtype=np.int64)
+    info = lib().bo_lu_factor_blocked(LU.shape[0], LU.ctypes.data, piv, nthreads, nb, max_panels)
     return LU, piv, int(info)
 
 

@@ -105,6 +105,9 @@ void bo_pot_t_multi(const bo_params* P, int64_t N, const double* xyz, int64_t M,
 /* LinLinSim::evolve_system_RK4 (LinLinSim.cpp:194-257).  xyz, phi, psi, time
  * are updated in place; returns 0, or -1 when a wall-mode vertex has x < 0
  * (the reference throws std::out_of_range). *dt_out receives dt. */
+/* cache-blocked GEPP (PartialPivLU's structure: GEPP panels of NB columns, packed GEMM
+ * trailing updates); max_panels > 0 stops after that many panels (a timed sample) */
+int64_t bo_lu_factor_blocked(int64_t N, double* A, int64_t* piv, int nthreads, int64_t NB, int64_t max_panels);
 /* Simulation::get_dt (Simulation.cpp:77-92): dp / (max|dpot| + dp_balance max|vel|), capped by min_dt > 0 */
 double bo_get_dt(const bo_params* P, int64_t N, double dp, const double* vel, const double* dpot);
 /* LinLinSim::nopenetration (LinLinSim.cpp:162-190): xf from x0 and the velocities c (N x 3); rewrites c.x

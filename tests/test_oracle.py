@@ -349,3 +349,24 @@ def test_exterior_potential_is_sum_of_panels(oracle_lib):
         want += (-g @ psi[[ia, ib, ic]] + h @ phi[[ia, ib, ic]]) / (4 * math.pi)
     got = oracle_lib.exterior_pot(v, t, phi, psi, y, nthreads=1)
     assert abs(got[0] - want) < 1e-14 * max(1.0, abs(want))
+
+
+def test_blocked_lu_matches_gepp(oracle_lib):
+    """The CPU baseline's cache-blocked GEPP chooses the same pivots as the
+    oracle's GEPP (PartialPivLU's first maximum) and reproduces LAPACK; a
+    sample of the first panels leaves the rest of the matrix updated by
+    exactly those panels."""
+    rng = np.random.default_rng(4)
+    n = 900
+    A = rng.standard_normal((n, n))
+    L1, p1, _ = oracle_lib.lu_factor(A)
+    L2, p2, info = oracle_lib.lu_factor_blocked(A, nb=128)
+    assert info == 0 and np.array_equal(p1, p2)
+    assert np.max(np.abs(L1 - L2)) < 1e-10 * np.max(np.abs(L1))
+    b = rng.standard_normal(n)
+    x = oracle_lib.lu_solve(L2, p2, b)
+    assert np.max(np.abs(x - np.linalg.solve(A, b))) < 1e-9 * np.max(np.abs(x))
+    L3, p3, _ = oracle_lib.lu_factor_blocked(A, nb=128, max_panels=2)
+    assert np.array_equal(p3[:256], p2[:256])
+    # rows below 256 are interchanged by the later panels: the leading block only
+    assert np.max(np.abs(L3[:256, :256] - L2[:256, :256])) < 1e-10 * np.max(np.abs(L2))

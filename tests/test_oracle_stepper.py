@@ -219,7 +219,8 @@ def mutant_libs(tmp_path_factory, oracle_lib):
     for k in MUTANTS:
         so = str(d / f"liboracle_m{k}.so")
         subprocess.run([cc, "-O2", "-fopenmp", "-fPIC", "-std=c11", "-shared", f"-DBO_MUTANT={k}", "-o", so,
-                        os.path.join(ORACLE_DIR, "bem_oracle.c"), "-lm"], check=True)
+                        os.path.join(ORACLE_DIR, "bem_oracle.c"), os.path.join(ORACLE_DIR, "bem_oracle_lu.c"), "-lm"],
+                       check=True)
         out[k] = oracle_lib.bind(so)
     return out
 
