@@ -42,14 +42,11 @@ PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "l
 # WRITE_SIZE, each scaled by what it reads for a plain copy of known size):
 # profiles/lu16_update_gemm_nu64_hbm_traffic.md (fp16 LU),
 # profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md (split-fp16 LU)
-TRAFFIC_NU64 = {"lu16x3_ir64": 5.99e9}
-# algorithmic bytes of the same launches (C read + written in FP32, the fp16
-# operand tiles read once), average per launch
-ALGO_BYTES_NU64 = {}
+TRAFFIC_NU64 = {"lu16_ir64": 3.767e9, "lu16x3_ir64": 5.99e9}
 # MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
 # GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
-# profiles/lu_update_gemm_nu64_mfma_utilisation.md
-MFMA_BUSY_NU64 = {"lu16x3_ir64": (0.511, 1.82)}
+# profiles/lu16_update_gemm_nu64_mfma_utilisation.md, profiles/lu_update_gemm_nu64_mfma_utilisation.md
+MFMA_BUSY_NU64 = {"lu16_ir64": (0.252, 2.05), "lu16x3_ir64": (0.511, 1.82)}
 DTYPE = {"lu16_ir64": "fp16-mfma lu (fp32 acc, fp32 factor)+f64-ir", "lu16x3_ir64": "f32-lu(split-f16 mfma)+f64-ir",
          "lu32_ir64": "f32-lu+f64-ir", "lu64": "f64", "bicgstab64": "f64"}
 
