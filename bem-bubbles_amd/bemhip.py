@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_get_part_volumes.argtypes = [vp, f64p, C.c_int32]
     L.bemhip_sim_set_part_physics.argtypes = [vp, C.c_int32, vp, vp, vp, vp]
     L.bemhip_exterior_potential.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p, f64p, f64p, C.c_int64, f64p, f64p]
+    L.bemhip_exterior_potential_noalloc.argtypes = [vp, C.c_int64, vp, C.c_int64, vp, vp, vp, C.c_int64, vp, vp]
     L.bemhip_sim_exterior_pot.argtypes = [vp, C.c_int64, f64p, f64p]
     L.bemhip_sim_get_triangles.argtypes = [vp, u32p]
     L.bemhip_sim_export_ply.argtypes = [vp, C.c_char_p, C.c_int32]
@@ -270,6 +271,13 @@ class Context:
                                              np.ascontiguousarray(phi, dtype=np.float64),
                                              flags, _solver_id(solver), tol, maxit, psi, C.byref(st)))
         return psi, st.as_dict()
+
+    # no-allocation variant (no reference counterpart): device pointers (ints)
+    # of caller-owned buffers; xyz N x 3, tri M x 3 uint32, phi / psi N,
+    # points P x 3, out P
+    def exterior_potential_noalloc(self, N, d_xyz, M, d_tri, d_phi, d_psi, P, d_points, d_out):
+        self.check(self.L.bemhip_exterior_potential_noalloc(self.h, N, d_xyz, M, d_tri, d_phi, d_psi, P, d_points,
+                                                            d_out))
 
     # compute_exterior_pot (LinLinSim.cpp:440-477)
     def exterior_potential(self, verts, tris, phi, psi, points):

@@ -412,6 +412,7 @@ int bemhip_create(int device, bemhip_ctx** out) {
                  std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
         ctx->device = device;
         ctx->cus = prop.multiProcessorCount;
+        exterior_scratch_init(ctx.get());
         for (auto& e : ctx->ev) BH_HIP(hipEventCreate(&e));
         if (const char* v = std::getenv("BEMHIP_LA_MIN_REST")) ctx->la_min_rest = std::atoi(v);
         const char* la = std::getenv("BEMHIP_LU_LOOKAHEAD");
@@ -899,6 +900,19 @@ int bemhip_exterior_potential(bemhip_ctx* ctx, int64_t N, const double* xyz, int
         h2d(dtg, points, sizeof(double) * 3 * P, ctx->stream);
         exterior_device(ctx, M, dx, dtri, dphi, dpsi, P, dtg, dout);
         d2h(out, dout, sizeof(double) * P, ctx->stream);
+    });
+}
+
+int bemhip_exterior_potential_noalloc(bemhip_ctx* ctx, int64_t N, const double* d_xyz, int64_t M,
+                                      const uint32_t* d_tri, const double* d_phi, const double* d_psi, int64_t P,
+                                      const double* d_points, double* d_out) {
+    if (!ctx) return BEMHIP_ERR_ARG;
+    return guarded(ctx, [&] {
+        BH_CHECK(N > 0 && M > 0 && P >= 0, BEMHIP_ERR_ARG, "bad sizes");
+        BH_CHECK(d_xyz && d_tri && d_phi && d_psi && (P == 0 || (d_points && d_out)), BEMHIP_ERR_ARG,
+                 "null argument");
+        if (P == 0) return;
+        exterior_device_noalloc(ctx, N, M, d_xyz, d_tri, d_phi, d_psi, P, d_points, d_out);
     });
 }
 

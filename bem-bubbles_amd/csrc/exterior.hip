@@ -128,4 +128,75 @@ void exterior_device(bemhip_ctx* ctx, int64_t M, const double* xyz, const uint32
     BH_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------ no-allocation variant --
+// The work buffers of exterior_device (triangle records, per-chunk partial
+// sums) and an error word in one fixed device array per GPU, for callers that
+// cannot allocate: records first (EXT_REC doubles per triangle), partial sums
+// after them, the error word in the last slot.  The array's host-side name is
+// not a device address: exterior_scratch_init takes its address on the
+// current device once (at context creation) and every use goes through that
+// pointer.  Calls that overlap on one GPU share the array and race.
+constexpr size_t EXT_SCRATCH = (size_t)8 << 20;    // doubles (64 MiB)
+constexpr size_t EXT_SCRATCH_USABLE = EXT_SCRATCH - 32;
+static __device__ __align__(256) double scratch[EXT_SCRATCH];
+
+void exterior_scratch_init(bemhip_ctx* ctx) {
+    void* p = nullptr;
+    BH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BH_CHECK(p != nullptr, BEMHIP_ERR_HIP, "no device address for the exterior-potential scratch array");
+    ctx->ext_scratch = static_cast<double*>(p);
+}
+
+// flag |= 1: a triangle index >= N; |= 2: a non-finite phi, psi or point
+__global__ __launch_bounds__(256) void k_ext_check(int64_t N, int64_t M, const uint32_t* __restrict__ tri,
+                                                   const double* __restrict__ phi, const double* __restrict__ psi,
+                                                   int64_t P, const double* __restrict__ pts,
+                                                   unsigned int* __restrict__ flag) {
+    unsigned int f = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * M; i += stride)
+        if (tri[i] >= (uint64_t)N) f |= 1u;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride)
+        if (!isfinite(phi[i]) || !isfinite(psi[i])) f |= 2u;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * P; i += stride)
+        if (!isfinite(pts[i])) f |= 2u;
+    if (f) atomicOr(flag, f);
+}
+
+__global__ void k_ext_flag_clear(unsigned int* flag) {
+    if (threadIdx.x == 0) *flag = 0u;
+}
+
+void exterior_device_noalloc(bemhip_ctx* ctx, int64_t N, int64_t M, const double* xyz, const uint32_t* tri,
+                             const double* phi, const double* psi, int64_t P, const double* tgt, double* out) {
+    BH_CHECK(ctx->ext_scratch != nullptr, BEMHIP_ERR_HIP, "exterior-potential scratch array not set up");
+    BH_CHECK((size_t)EXT_REC * M + (size_t)P <= EXT_SCRATCH_USABLE, BEMHIP_ERR_ARG,
+             "104 M + P exceeds the no-allocation scratch (8388576 doubles)");
+    hipStream_t st = ctx->stream;
+    double* rec = ctx->ext_scratch;
+    unsigned int* flag = reinterpret_cast<unsigned int*>(ctx->ext_scratch + EXT_SCRATCH - 1);
+    k_ext_flag_clear<<<1, 64, 0, st>>>(flag);
+    k_ext_check<<<256, 256, 0, st>>>(N, M, tri, phi, psi, P, tgt, flag);
+    BH_HIP(hipGetLastError());
+    unsigned int* hflag = reinterpret_cast<unsigned int*>(ctx->host_scal + 48);
+    BH_HIP(hipMemcpyAsync(hflag, flag, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    BH_CHECK(!(*hflag & 1u), BEMHIP_ERR_BAD_MESH, "vertex index out of range");
+    BH_CHECK(!(*hflag & 2u), BEMHIP_ERR_NAN, "non-finite phi, psi or point");
+    k_ext_prep<<<(unsigned)((M + 255) / 256), 256, 0, st>>>(M, xyz, tri, phi, psi, rec);
+    BH_HIP(hipGetLastError());
+    // as many triangle chunks as fill the chip and fit beside the records
+    const int64_t gx = (P + EXT_TGT - 1) / EXT_TGT;
+    const int64_t room = (int64_t)((EXT_SCRATCH_USABLE - (size_t)EXT_REC * M) / (size_t)P);
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>((8 * ctx->cus + gx - 1) / gx, (M + 63) / 64));
+    chunks = std::min<int64_t>(std::min<int64_t>(chunks, room), 65535);
+    const int tpc = (int)((M + chunks - 1) / chunks);
+    chunks = (M + tpc - 1) / tpc;
+    double* part = rec + (size_t)EXT_REC * M;
+    k_ext<<<dim3((unsigned)gx, (unsigned)chunks), EXT_TGT, 0, st>>>(P, M, tpc, tgt, rec, part);
+    k_ext_sum<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(P, (int)chunks, part, out);
+    BH_HIP(hipGetLastError());
+    BH_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace bemhip

@@ -225,6 +225,7 @@ struct bemhip_ctx : bemhip::Solver {
     // assembly and per-problem buffers
     bemhip::DevBuf triq, part_hphi, part_hrow, hphi, xyz, phi, G64, G32, H;
     bemhip::DevBuf ext_rec, ext_part, ext_tgt, ext_out, ext_tri, ext_psi;   // exterior potential
+    double* ext_scratch = nullptr;   // this device's address of the no-allocation scratch array
     bemhip::DevBuf adj_coef, adj_tc, adj_tpart, adj_rpart, adj_sc, adj_zero;   // adjoint partials
     hipEvent_t ev[4] = {};
     // extra solvers for batches of independent problems (created on first use)
@@ -290,6 +291,12 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
 // all pointers on the device, out[P]
 void exterior_device(bemhip_ctx* ctx, int64_t M, const double* xyz, const uint32_t* tri, const double* phi,
                      const double* psi, int64_t P, const double* tgt, double* out);
+// the same with every work buffer in a fixed device array (no allocation);
+// validates the triangle indices and finiteness on the device; synchronous
+void exterior_device_noalloc(bemhip_ctx* ctx, int64_t N, int64_t M, const double* xyz, const uint32_t* tri,
+                             const double* phi, const double* psi, int64_t P, const double* tgt, double* out);
+// this device's address of that array into ctx (once, at context creation)
+void exterior_scratch_init(bemhip_ctx* ctx);
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
 template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* ipiv, double* x);

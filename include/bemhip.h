@@ -195,6 +195,21 @@ int bemhip_sim_accum_stats(bemhip_sim* sim, bemhip_solve_stats* stats, int reset
 int bemhip_exterior_potential(bemhip_ctx* ctx, int64_t N, const double* xyz, int64_t M, const uint32_t* tri,
                               const double* phi, const double* psi, int64_t P, const double* points,
                               double* out);
+/* ---- No-allocation variant of bemhip_exterior_potential -----------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the same potential, but every array is a device pointer owned
+ * by the caller (d_xyz N x 3, d_tri M x 3, d_phi and d_psi N, d_points P x 3,
+ * d_out P), and the work buffers (triangle records, per-chunk partial sums,
+ * an error word) live in a fixed 64 MiB device array of the library, one per
+ * GPU: 104 M + P <= 8388576 (e.g. M <= 65536 triangles with P up to 1.5
+ * million points), BEMHIP_ERR_ARG beyond.  Indices and finiteness are checked
+ * on the device (BEMHIP_ERR_BAD_MESH, BEMHIP_ERR_NAN).  Runs on the context's
+ * stream, synchronous at return; calls no hipMalloc / hipFree.  Calls that
+ * overlap on one GPU (from any context) share the array and race. */
+int bemhip_exterior_potential_noalloc(bemhip_ctx* ctx, int64_t N, const double* d_xyz, int64_t M,
+                                      const uint32_t* d_tri, const double* d_phi, const double* d_psi, int64_t P,
+                                      const double* d_points, double* d_out);
 /* LinLinSim::exterior_pot (LinLinSim.cpp:478-483): psi for the current mesh
  * and phi (the simulation's solver and mirror flag; the simulation's psi is
  * left unchanged), then the potential at the P points from (phi, psi). */
