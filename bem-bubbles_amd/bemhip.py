@@ -345,7 +345,10 @@ class ColocSim:
         self._check(self.ctx.L.bemhip_sim_set_phi(self.h, phi))
 
     def set_psi(self, psi):
-        self._check(self.ctx.L.bemhip_sim_set_psi(self.h, np.ascontiguousarray(psi, dtype=np.float64)))
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        if psi.shape != (self.N,):
+            raise ValueError(f"psi must have shape ({self.N},), got {psi.shape}")
+        self._check(self.ctx.L.bemhip_sim_set_psi(self.h, psi))
 
     def set_V_0(self, v):
         self._check(self.ctx.L.bemhip_sim_set_V_0(self.h, float(v)))
@@ -434,9 +437,11 @@ class ColocSim:
         return vel, psi
 
     def pot_t_multi(self, vel, t):
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        if vel.size != 3 * self.N:
+            raise ValueError(f"vel must hold {self.N} x 3 values, got shape {vel.shape}")
         out = np.zeros(self.N)
-        self._check(self.ctx.L.bemhip_sim_pot_t_multi(self.h, np.ascontiguousarray(vel, dtype=np.float64).reshape(-1),
-                                                      float(t), out))
+        self._check(self.ctx.L.bemhip_sim_pot_t_multi(self.h, vel.reshape(-1), float(t), out))
         return out
 
     def compute_psi(self):

@@ -430,6 +430,8 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (const char* v = std::getenv("BEMHIP_MM_RESERVE")) ctx->mm_reserve = std::max(0, std::atoi(v));
         const char* tp = std::getenv("BEMHIP_TRSV_PERSISTENT");
         if (tp && tp[0] == '0') ctx->trsv_persistent = false;
+        const char* tg = std::getenv("BEMHIP_TRSV_FORCE_GIVEUP");
+        if (tg && tg[0] == '1') ctx->trsv_force_giveup = true;
         if (const char* v = std::getenv("BEMHIP_LANES")) ctx->max_lanes = std::max(1, std::atoi(v));
         if (const char* v = std::getenv("BEMHIP_SMALL_N")) ctx->small_n = std::max(0, std::atoi(v));
         const char* s1 = std::getenv("BEMHIP_SMALL_SINGLE");

@@ -2162,7 +2162,8 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     __shared__ double part[2][TS];
     __shared__ int s_t, s_ready;
     unsigned int* ticket = sync;
-    unsigned int* err = sync + 1;
+    unsigned int* err = sync + 1;      // this launch: a work-group gave up waiting
+    unsigned int* sticky = sync + 2;   // any launch since the host cleared it
     unsigned int* flags = sync + 16;
     const int nb = (N + TS - 1) / TS;
     const int tid = threadIdx.x;
@@ -2205,19 +2206,27 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
                 const int jb = LOWER ? qq : nb - 1 - qq;
                 ok = __hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
             }
+            // the error word is read with the flags, every round: once one
+            // work-group has given up, the others stop waiting as well
+            const unsigned int e = tid == 0 ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
             const unsigned long long bal = __ballot(ok);
             int nready = (~bal == 0ull) ? 64 : (__ffsll((long long)~bal) - 1);
             nready = min(nready, min(t - q, TRSV_BATCH));
             if (tid == 0) {
-                if (nready == 0) {
+                if (nready == 0 && !e) {
                     __builtin_amdgcn_s_sleep(2);
-                    if (++spins > (1u << 24) ||
-                        __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (++spins > (1u << 24))
                         __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        nready = t - q;   // give up waiting: finish (results invalid, error reported)
-                    }
+                    else
+                        nready = -1;   // poll again
                 }
-                s_ready = nready;
+                if (nready == 0 || e) {
+                    // give up waiting: consume the next blocks as they stand and
+                    // finish (results invalid; the host reads the sticky word)
+                    __hip_atomic_fetch_or(sticky, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    nready = min(t - q, TRSV_BATCH);
+                }
+                s_ready = max(nready, 0);
             }
         }
         __syncthreads();
@@ -2307,10 +2316,26 @@ template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, dou
     const int nb = (N + TS - 1) / TS;
     unsigned int* sync = ctx->lu.flag.as<unsigned int>();
     const unsigned int epoch = ++ctx->trsv_epoch;
-    k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error flag
+    k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error word
+    if (ctx->trsv_force_giveup) k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync + 1, 1, 1u);
     if (lower) k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch);
     else k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), x, sync, epoch);
     BH_HIP(hipGetLastError());
+}
+
+// the sticky give-up word of the persistent triangular solves: cleared before
+// a solve's first launch, read back after its last
+static void trsv_sticky_clear(Solver* ctx) {
+    if (ctx->trsv_persistent && ctx->lu.flag.p)
+        k_fill_u32<<<1, 64, 0, ctx->stream>>>(ctx->lu.flag.as<unsigned int>() + 2, 1, 0u);
+}
+static void trsv_sticky_check(Solver* ctx) {
+    if (!ctx->trsv_persistent || !ctx->lu.flag.p) return;
+    unsigned int* h = reinterpret_cast<unsigned int*>(ctx->host_scal + 40);
+    BH_HIP(hipMemcpyAsync(h, ctx->lu.flag.as<unsigned int>() + 2, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    BH_CHECK(*h == 0u, BEMHIP_ERR_HIP, "persistent triangular solve gave up waiting for a block (result invalid)");
 }
 
 template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* perm, double* x) {
@@ -2553,8 +2578,13 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
     int* ipiv = static_cast<int*>(ctx->lu.ipiv.ensure(sizeof(int) * N));
     ctx->gemm_ms = 0.0; ctx->gemm_flops = 0.0; ctx->gemm_launches = 0;
     ctx->panels = 0; ctx->panels_pivoted = 0;
-    hipEvent_t e0, e1, e2;
-    BH_HIP(hipEventCreate(&e0)); BH_HIP(hipEventCreate(&e1)); BH_HIP(hipEventCreate(&e2));
+    // the timing events, destroyed on every exit (the checks below throw)
+    struct Events {
+        hipEvent_t e[3] = {};
+        ~Events() { for (hipEvent_t v : e) if (v) (void)hipEventDestroy(v); }
+    } ev;
+    for (hipEvent_t& v : ev.e) BH_HIP(hipEventCreate(&v));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     BH_HIP(hipEventRecord(e0, sm));
     st->iterations = 0; st->converged = 1; st->correction = 0.0; st->residual = 0.0;
     const bool small1 = ctx->small_single && solver != BEMHIP_SOLVE_BICGSTAB64 && N <= SMALL_MAX_N;
@@ -2577,9 +2607,11 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
         int* perm = make_perm(ctx, N, ipiv);
         if (ctx->trsv_persistent) lu_prepare_solve<double>(ctx, N, G64);
+        trsv_sticky_clear(ctx);
         BH_HIP(hipEventRecord(e1, sm));
         BH_HIP(hipMemcpyAsync(x, b, sizeof(double) * N, hipMemcpyDeviceToDevice, sm));
         lu_solve<double>(ctx, N, G64, perm, x);
+        trsv_sticky_check(ctx);
     } else if (solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64 ||
                solver == BEMHIP_SOLVE_LU16_IR64) {
         BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "mixed solve needs the FP32 copy of G");
@@ -2608,6 +2640,7 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         BH_CHECK(info == 0, BEMHIP_ERR_SINGULAR, "zero pivot in column " + std::to_string(info));
         int* perm = make_perm(ctx, N, ipiv);
         if (ctx->trsv_persistent) lu_prepare_solve<float>(ctx, N, G32);
+        trsv_sticky_clear(ctx);
         BH_HIP(hipEventRecord(e1, sm));
         if (tol <= 0.0) tol = IR_TOL;
         if (maxit <= 0) maxit = 30;
@@ -2631,6 +2664,7 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
             if (!std::isfinite(cn)) break;
             if (cn <= tol) { conv = true; break; }
         }
+        trsv_sticky_check(ctx);
         st->iterations = it;
         st->converged = conv ? 1 : 0;
         st->correction = cn;
@@ -2646,7 +2680,6 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
     st->gemm_launches = ctx->gemm_launches;
     st->panels = ctx->panels;
     st->panels_pivoted = ctx->panels_pivoted;
-    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     // final FP64 residual |b - G x| / |b| (not for LU64, whose G was overwritten)
     if (report_residual && solver != BEMHIP_SOLVE_LU64 && !small1) {
         double* r = static_cast<double*>(ctx->vec[5].ensure(sizeof(double) * N));
@@ -2748,6 +2781,7 @@ void Solver::init(const Solver* cfg) {
         np_top32 = cfg->np_top32;
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split; nb16 = cfg->nb16;
         trsv_persistent = cfg->trsv_persistent;
+        trsv_force_giveup = cfg->trsv_force_giveup;
         mm_reserve = cfg->mm_reserve;
         timing = cfg->timing;
         split_fixed = cfg->split_fixed;

@@ -176,6 +176,10 @@ struct Solver {
     int ncu = 256;   // compute units of the device
     int panels = 0, panels_pivoted = 0;
     bool trsv_persistent = true;
+    // test hook (BEMHIP_TRSV_FORCE_GIVEUP=1): the host pre-sets the persistent
+    // triangular solve's error word, so every waiting work-group gives up at
+    // once and the solve must report BEMHIP_ERR_HIP
+    bool trsv_force_giveup = false;
     // single problems of at most SMALL_MAX_N unknowns through the one-work-group
     // FP64 solver (BEMHIP_SMALL_SINGLE=1; leaves no factors for a transposed solve)
     bool small_single = false;

@@ -288,6 +288,11 @@ def test_surface_tension_curvature_path(ctx, oracle_lib):
     s = _gpu_sim(ctx, c, "lu64")
     got = s.pot_t_multi(vel, 0.25)
     assert rel(got, ref) < 1e-12
+    # arrays of the wrong length are refused before the library reads them
+    with pytest.raises(ValueError):
+        s.pot_t_multi(vel[:-1], 0.25)
+    with pytest.raises(ValueError):
+        s.set_psi(np.zeros(len(c.verts) - 1))
 
 
 def test_bad_inputs_are_reported(ctx):
@@ -303,6 +308,37 @@ def test_bad_inputs_are_reported(ctx):
     with pytest.raises(bemhip.BemHipError) as e:
         ctx.compute_psi(v, t, phi)
     assert e.value.code == bemhip.ERR_NAN
+
+
+def test_triangular_solve_give_up_is_reported(oracle_lib, monkeypatch):
+    """The persistent triangular solves wait on each other's blocks with a
+    bounded spin; a work-group that gives up finishes with invalid results and
+    raises a sticky error word.  Here the host pre-sets the error word before
+    every launch (BEMHIP_TRSV_FORCE_GIVEUP=1), so every waiting work-group gives
+    up at once and exits normally: each LU solver must then report
+    BEMHIP_ERR_HIP instead of returning x.  A context without the hook solves
+    the same system afterwards."""
+    say("triangular solve give-up")
+    v, t = meshgen.icosphere(12)   # N = 1442: 12 row blocks of 128
+    G, H = oracle_lib.assemble(v, t)
+    b = H @ (1.0 + 0.05 * meshgen.random_sh_field(v, 1, 4, 5))
+    monkeypatch.setenv("BEMHIP_TRSV_FORCE_GIVEUP", "1")
+    c1 = bemhip.Context(0)
+    try:
+        for solver in ("lu64", "lu32_ir64", "lu16_ir64"):
+            with pytest.raises(bemhip.BemHipError) as e:
+                c1.solve(G, b, solver)
+            assert e.value.code == bemhip.ERR_HIP, solver
+            assert "gave up" in str(e.value)
+    finally:
+        c1.close()
+    monkeypatch.delenv("BEMHIP_TRSV_FORCE_GIVEUP")
+    c2 = bemhip.Context(0)
+    try:
+        x, st = c2.solve(G, b, "lu64")
+        assert rel(x, oracle_lib.solve(G, b)) < 1e-10
+    finally:
+        c2.close()
 
 
 def test_large_sphere_properties(ctx):
