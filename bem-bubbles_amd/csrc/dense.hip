@@ -1058,18 +1058,10 @@ struct Lane {
 
 // ------------------------------------------- triangular solves by inverse --
 // Large blocked triangular solves as one GEMM with the explicit inverse of the
-// (at most NB x NB) triangle: rocBLAS's TRSM runs many small GEMMs per call.
-// The inverse of a unit lower triangle with |l| <= 1 (partial pivoting) or of
-// U11 of such a factorisation is well conditioned for these matrices, and the
-// factor only preconditions the FP64 refinement of the mixed solvers.
-static void trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, const double* A, int lda,
-                  double* W) {
-    BH_BLAS(rocblas_dtrtri(h, uplo, diag, n, A, lda, W, n));
-}
-static void trtri(rocblas_handle h, rocblas_fill uplo, rocblas_diagonal diag, int n, const float* A, int lda,
-                  float* W) {
-    BH_BLAS(rocblas_strtri(h, uplo, diag, n, A, lda, W, n));
-}
+// (at most NB x NB) triangle (trinv.hip).  The inverse of a unit lower triangle
+// with |l| <= 1 (partial pivoting) or of U11 of such a factorisation is well
+// conditioned for these matrices, and the factor only preconditions the FP64
+// refinement of the mixed solvers.
 static void gemm_nn(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* B, int ldb,
                     double* C, int ldc) {
     const double one = 1.0, zero = 0.0;
@@ -1083,25 +1075,13 @@ static void gemm_nn(rocblas_handle h, int m, int n, int k, const float* A, int l
                           C, ldc));
 }
 
-// the triangle not referenced by an inverse set to zero (and a unit diagonal
-// written out), so the inverse can enter a full GEMM
+// Wl, Wu (n x n, ld n) = the inverses of A's unit-lower and non-unit upper
+// triangles, each with its other triangle zero, ready for a GEMM (either may
+// be null; both are built by the same launches)
 template <class T>
-__global__ __launch_bounds__(256) void k_tri_clean(int n, T* __restrict__ W, int lower, int unit) {
-    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
-    if (i >= n) return;
-    T& w = W[(size_t)j * n + i];
-    if (lower ? (i < j) : (i > j)) w = T(0);
-    else if (unit && i == j) w = T(1);
-}
-
-// W (n x n) = the inverse of A's unit-lower (lower) or non-unit upper triangle,
-// the other triangle zeroed and a unit diagonal written out, ready for a GEMM
-template <class T>
-static void tri_inverse(Lane L, bool lower, int n, const T* A, int lda, T* W) {
-    trtri(L.h, lower ? rocblas_fill_lower : rocblas_fill_upper,
-          lower ? rocblas_diagonal_unit : rocblas_diagonal_non_unit, n, A, lda, W);
-    k_tri_clean<T><<<dim3((n + 255) / 256, n), 256, 0, L.s>>>(n, W, lower ? 1 : 0, lower ? 1 : 0);
-    BH_HIP(hipGetLastError());
+static void tri_inverse(Solver* ctx, Lane L, int n, const T* A, int lda, T* Wl, T* Wu) {
+    T* t = ctx->lu.tinv.as<T>();
+    tri_inv2<T>(L.s, n, A, lda, Wl, Wu, t, t + tri_inv_tmp_elems(n));
 }
 
 // max |x| over a column-major block, merged into *out as the bits of a
@@ -1455,7 +1435,8 @@ static float lu16_sa(const Solver* ctx) { return std::ldexp(1.0f, 8 - ctx->lu16_
 // Step p's products on the panel stream: L21 = A21 U11^-1 (rows below an
 // m x n panel, A21 at A + n) into l21 (ld m - n) with its A tiles and max |l|
 // merged into maxl (when set); with l21 == A + n (a panel factored with row
-// interchanges, L21 already in place) only its tiles.  Then L11^-1 as A tiles.
+// interchanges, L21 already in place) only its tiles, and L11^-1 (which the
+// interchange-free panel had built with U11^-1).  Then L11^-1 as A tiles.
 static void lu16_panel_products(Solver* ctx, Lane L, int m, int n, const float* A, int lda, float* l21, int ldl,
                                 int p, unsigned long long* maxl) {
     unsigned int* log = ctx->lu.f16log.as<unsigned int>();
@@ -1468,8 +1449,8 @@ static void lu16_panel_products(Solver* ctx, Lane L, int m, int n, const float* 
                   std::ldexp(1.0f, -6), 0, &tl);
     } else {
         mm16_pack(L.s, false, m - n, n, l21, ldl, 16384.0f, ctx->lu.t_l21[p].p, log, nullptr, 0, maxl);
+        tri_inverse<float>(ctx, L, n, A, lda, ctx->lu.winv_l[p].as<float>(), nullptr);
     }
-    tri_inverse(L, true, n, A, lda, ctx->lu.winv_l[p].as<float>());
     mm16_pack(L.s, false, n, n, ctx->lu.winv_l[p].as<float>(), n, 1.0f, ctx->lu.t_wl[p].p, log);
 }
 
@@ -1548,7 +1529,7 @@ template <class T> static bool top32(const Solver* ctx, int n) {
 }
 
 template <class T>
-static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned int* sync, T* l21) {
+static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned int* sync, T* l21, int p) {
     unsigned long long* maxl = reinterpret_cast<unsigned long long*>(sync);
     const int stripes = (n + NPB - 1) / NPB;
     if (top32<T>(ctx, n)) {
@@ -1564,8 +1545,9 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
     }
     BH_HIP(hipGetLastError());
     if (m > n) {
+        // U11^-1 for L21 and L11^-1 for the step's U12, in the same launches
         T* W = ctx->lu.winv_u.as<T>();
-        tri_inverse(L, false, n, A, lda, W);
+        tri_inverse<T>(ctx, L, n, A, lda, ctx->lu.winv_l[p].as<T>(), W);
         if (ctx->lu16) return;   // L21 with the step's other products (lu16_panel_products)
         gemm_nn(L.h, m - n, n, n, A + n, lda, W, n, l21, m - n);
     }
@@ -1605,7 +1587,9 @@ static void panel_products(Solver* ctx, Lane L, int m, int n, const T* A, int ld
             return;
         }
     }
-    tri_inverse(L, true, n, A, lda, ctx->lu.winv_l[p].as<T>());
+    // L11^-1 of a panel factored with interchanges (an interchange-free one
+    // had it built with U11^-1)
+    if (l21 == A + n) tri_inverse<T>(ctx, L, n, A, lda, ctx->lu.winv_l[p].as<T>(), nullptr);
     if constexpr (std::is_same<T, float>::value) {
         if (split16) {
             split_a(ctx, L.s, m - n, n, l21, ldl, p, maxl);
@@ -1637,7 +1621,7 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
     unsigned int* sync = ctx->lu.maxl.as<unsigned int>();
     k_fill_u32<<<1, 64, 0, L.s>>>(sync, 4, 0u);
     T* l21 = ctx->lu.l21[p].as<T>();
-    panel_np(ctx, L, m, n, A, lda, sync, l21);
+    panel_np(ctx, L, m, n, A, lda, sync, l21, p);
     panel_products(ctx, L, m, n, A, lda, l21, m - n, p, split16, reinterpret_cast<unsigned long long*>(sync));
     BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, sync, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
@@ -1852,6 +1836,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
         if (split16) ctx->split_a[p].ensure(sizeof(__half) * 3 * (size_t)N * nb);
     }
     ctx->lu.winv_u.ensure(sizeof(T) * nb * nb);
+    ctx->lu.tinv.ensure(sizeof(T) * 2 * tri_inv_tmp_elems((int)nb));
     ctx->lu.u12.ensure(sizeof(T) * nb * (size_t)N);
     ctx->lu.backup.ensure(sizeof(T) * nb * nb);
     // published diagonal-block inverses (persistent top-block launches) or the

@@ -147,6 +147,7 @@ struct LuWork {
     // two alternating by step parity), of the block row A12 and of U12; the
     // max |scaled value| of every pack (float bits)
     DevBuf t_a21, t_wu, t_wl[2], t_l21[2], t_a12, t_u12, f16log;
+    DevBuf tinv;   // work of the triangular inverses (2 tri_inv_tmp_elems(NB) elements)
 };
 
 // Everything one dense solve needs: its streams, BLAS handles, work buffers,
@@ -301,6 +302,13 @@ void exterior_device_noalloc(bemhip_ctx* ctx, int64_t N, int64_t M, const double
                              const double* phi, const double* psi, int64_t P, const double* tgt, double* out);
 // this device's address of that array into ctx (once, at context creation)
 void exterior_scratch_init(bemhip_ctx* ctx);
+// Wl, Wu (n x n, ld n) = inverses of the unit-lower and the non-unit upper
+// triangle of A (ld lda), each with its other triangle written as zeros, in
+// the same launches (a null W skips its triangle); tmp_l and tmp_u hold
+// tri_inv_tmp_elems(n) elements each (trinv.hip)
+template <class T>
+void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l, T* tmp_u);
+size_t tri_inv_tmp_elems(int n);
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
 template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* ipiv, double* x);
