@@ -10,24 +10,27 @@
 //
 // Device design:
 //   k_entry_prep one thread per patch entry (a triangle as seen from one
-//                column patch): the 7 quadrature points, unit normal, Jacobian
+//                column patch): vertex a, 2 ab, 2 bc, the 7 values
+//                |u_q ab + v_q bc|^2 and w_q phi(y_q), unit normal, Jacobian
 //                (|ab x bc|), the potential at its vertices, vertex ids and
 //                patch slots -- one contiguous 256-byte record.
 //   k_far        work-group = 128 collocation rows (one per lane) x a range of
 //                column patches, one item of a host-built work table: one dense
 //                problem for a coupled mesh, one per loose part for a batch of
-//                independent bubbles (each with its own G block, about 8192
-//                work-groups in all, shared in proportion to n^2).
-//                A patch is PATCH (16) mesh-adjacent vertices; the
+//                independent bubbles (each with its own G block; about
+//                ASM_WG_TARGET work-groups in all, shared in proportion to n^2).
+//                A patch is PATCH (12) mesh-adjacent vertices; the
 //                work-group streams the triangles incident to the patch (wave-
-//                uniform, scalar loads) and accumulates each lane's 16 G (and
+//                uniform, scalar loads) and accumulates each lane's 12 G (and
 //                optionally H) columns in LDS, private per lane, so no atomics
 //                and a fixed summation order.  The LDS per lane sets the
-//                occupancy: 16 columns allow 5 waves per SIMD, 32 only 2.5
-//                (measured 25% slower despite 11% fewer triangle entries).  A flushed patch column is a
-//                coalesced 1 KiB store per wave.  H is never stored unless
-//                asked: H phi is reduced on the fly as
-//                sum_k H_k (phi_k - phi_i), which equals the 4-pi-closed
+//                occupancy (12 columns: 6 waves per SIMD).  A triangle is an
+//                entry of every patch that holds one of its vertices; each
+//                entry sums only its in-patch vertices' G columns, and the
+//                triangle's primary entry alone adds its whole H phi part.
+//                A flushed patch column is a coalesced 1 KiB store per wave.
+//                H is never stored unless asked: H phi is reduced on the fly
+//                as sum_k H_k (phi_k - phi_i), which equals the 4-pi-closed
 //                product (H phi)_i + 4 pi phi_i exactly in real arithmetic.
 //   k_self       one thread per vertex: adds the polar-rule self panels to its
 //                own row, finishes H phi (and H's diagonal when stored).
@@ -46,11 +49,28 @@ constexpr int TQ_STRIDE = 32;  // doubles per triangle record
 __host__ __device__ constexpr double basis_w(int q, int k) {
     return TQ_W[q] * (k == 0 ? (1.0 - (TQ_X[q] + TQ_Y[q])) : (k == 1 ? TQ_X[q] : TQ_Y[q]));
 }
+// the quadrature point y_q = a + u_q ab + v_q bc (Interpolator.hpp:15-21)
+__host__ __device__ constexpr double q_u(int q) { return TQ_X[q] + TQ_Y[q]; }
+__host__ __device__ constexpr double q_v(int q) { return TQ_Y[q]; }
 
-// One record per patch entry (a triangle as seen from one patch), so the
-// assembly streams records contiguously with a single scalar load per
-// triangle: the 7 quadrature points, unit normal, Jacobian, the potential at
-// the 3 vertices, the vertex ids and the vertex slots in this patch.
+// w_q phi_k(q) for a vertex index k known only at run time: a wave-uniform
+// selection between the three constants (no memory access)
+__device__ __forceinline__ double basis_w_sel(int q, int k) {
+    return k == 0 ? basis_w(q, 0) : (k == 1 ? basis_w(q, 1) : basis_w(q, 2));
+}
+
+// The record of one patch entry (a triangle as seen from one patch), 32
+// doubles, streamed with scalar loads:
+//   [0..2] a   [3..5] 2 ab   [6..8] 2 bc   [9..15] c_q = |u_q ab + v_q bc|^2
+//   [16..22] w_q phi(y_q)   [23..25] unit normal   [26] J = |ab x bc|
+//   [27], [30], [31] phi at a, b, c   [28..29] vertex ids a, b, c and the
+//   packed patch slots (bits 0-23) with the primary-entry flag (bit 24)
+// |y_q - x|^2 = |a - x|^2 + c_q + u_q (a - x).(2 ab) + v_q (a - x).(2 bc): the
+// dot products are formed once per (row, entry), so a quadrature point costs
+// one add and two FMAs instead of three differences and three products.
+enum : int { R_A = 0, R_E1 = 3, R_E2 = 6, R_C = 9, R_WPHI = 16, R_N = 23, R_J = 26, R_PHIA = 27, R_PHIB = 30,
+             R_PHIC = 31 };
+
 __global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __restrict__ xyz,
                                                     const double* __restrict__ phi, const uint32_t* __restrict__ tri,
                                                     const uint32_t* __restrict__ ptri,
@@ -69,22 +89,160 @@ __global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __re
     const double J = sqrt(nx * nx + ny * ny + nz * nz);
     const double rj = 1.0 / J;
     nx *= rj; ny *= rj; nz *= rj;
+    const double pa = phi ? phi[ia] : 0.0, pb = phi ? phi[ib] : 0.0, pc = phi ? phi[ic] : 0.0;
     double* o = rec + (size_t)TQ_STRIDE * e;
+    o[R_A] = ax; o[R_A + 1] = ay; o[R_A + 2] = az;
+    o[R_E1] = 2.0 * abx; o[R_E1 + 1] = 2.0 * aby; o[R_E1 + 2] = 2.0 * abz;
+    o[R_E2] = 2.0 * bcx; o[R_E2 + 1] = 2.0 * bcy; o[R_E2 + 2] = 2.0 * bcz;
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-        const double u = TQ_X[q] + TQ_Y[q], v = TQ_Y[q];
-        o[3 * q + 0] = ax + (u * abx + v * bcx);
-        o[3 * q + 1] = ay + (u * aby + v * bcy);
-        o[3 * q + 2] = az + (u * abz + v * bcz);
+        const double u = q_u(q), v = q_v(q);
+        const double wx = u * abx + v * bcx, wy = u * aby + v * bcy, wz = u * abz + v * bcz;
+        o[R_C + q] = wx * wx + wy * wy + wz * wz;
+        o[R_WPHI + q] = basis_w(q, 0) * pa + basis_w(q, 1) * pb + basis_w(q, 2) * pc;
     }
-    o[21] = nx; o[22] = ny; o[23] = nz;
-    o[24] = J;
-    o[25] = phi ? phi[ia] : 0.0;
-    o[26] = phi ? phi[ib] : 0.0;
-    o[27] = phi ? phi[ic] : 0.0;
+    o[R_N] = nx; o[R_N + 1] = ny; o[R_N + 2] = nz;
+    o[R_J] = J;
+    o[R_PHIA] = pa; o[R_PHIB] = pb; o[R_PHIC] = pc;
     uint32_t* w = reinterpret_cast<uint32_t*>(o + 28);
     w[0] = ia; w[1] = ib; w[2] = ic; w[3] = pslots[e];
-    o[30] = 0.0; o[31] = 0.0;
+}
+
+// x's view of one entry: d0 = a - x and the dot products of the expansion
+// (and of the image point R x, R: x -> -x, in wall mode)
+struct FarGeo {
+    double D, P, Q;      // |a - x|^2, (a - x).(2 ab), (a - x).(2 bc)
+    double Dm, Pm, Qm;   // the same for the image triangle R T and x
+};
+
+template <bool MIRROR>
+__device__ __forceinline__ FarGeo far_geo(const double (&r)[28], double xi, double yi, double zi) {
+    FarGeo f;
+    const double dx = r[R_A] - xi, dy = r[R_A + 1] - yi, dz = r[R_A + 2] - zi;
+    const double dyz = dy * dy + dz * dz;
+    const double pyz = dy * r[R_E1 + 1] + dz * r[R_E1 + 2];
+    const double qyz = dy * r[R_E2 + 1] + dz * r[R_E2 + 2];
+    f.D = dx * dx + dyz;
+    f.P = dx * r[R_E1] + pyz;
+    f.Q = dx * r[R_E2] + qyz;
+    if constexpr (MIRROR) {
+        // R T has vertex R a and edges R ab, R bc: (R a - x) = (-a_x - x, ..)
+        const double dxm = -r[R_A] - xi;
+        f.Dm = dxm * dxm + dyz;
+        f.Pm = -dxm * r[R_E1] + pyz;
+        f.Qm = -dxm * r[R_E2] + qyz;
+    }
+    return f;
+}
+
+// One entry's 7-point sums for one collocation point x:
+//   g[j] = sum_q w_q phi_k(q) / r_q for NG in-patch vertices (all three, or
+//          the basis indices ka, kb);
+//   SK:    s[k] = sum_q w_q phi_k(q) / r_q^3 for all three k (stored H);
+//   HA:    A = sum_q w_q phi(y_q) / r_q^3 and B = sum_q w_q / r_q^3, the H phi
+//          row part sum_k s_k (phi_k - phi_i) = A - phi_i B.
+// The *m sums are the image point's in wall mode.
+struct FarSums {
+    double g[3], s[3], A, B;
+    double gm[3], sm[3], Am, Bm;
+};
+
+template <bool MIRROR, int NG, bool SK, bool HA>
+__device__ __forceinline__ void far_sums(const double (&r)[28], const FarGeo& f, int ka, int kb, FarSums& o) {
+#pragma unroll
+    for (int l = 0; l < 3; ++l) o.g[l] = o.s[l] = o.gm[l] = o.sm[l] = 0.0;
+    o.A = o.B = o.Am = o.Bm = 0.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const double c = r[R_C + k];
+        const double inv = rsqrt_nr(fma(q_v(k), f.Q, fma(q_u(k), f.P, f.D + c)));
+        if constexpr (NG == 3) {
+            o.g[0] += basis_w(k, 0) * inv; o.g[1] += basis_w(k, 1) * inv; o.g[2] += basis_w(k, 2) * inv;
+        } else {
+            o.g[0] += basis_w_sel(k, ka) * inv;
+            if constexpr (NG == 2) o.g[1] += basis_w_sel(k, kb) * inv;
+        }
+        if constexpr (SK || HA) {
+            const double inv3 = inv * inv * inv;
+            if constexpr (SK) {
+                o.s[0] += basis_w(k, 0) * inv3; o.s[1] += basis_w(k, 1) * inv3; o.s[2] += basis_w(k, 2) * inv3;
+            }
+            if constexpr (HA) {
+                o.A += r[R_WPHI + k] * inv3;
+                o.B += TQ_W[k] * inv3;
+            }
+        }
+        if constexpr (MIRROR) {
+            const double invm = rsqrt_nr(fma(q_v(k), f.Qm, fma(q_u(k), f.Pm, f.Dm + c)));
+            if constexpr (NG == 3) {
+                o.gm[0] += basis_w(k, 0) * invm; o.gm[1] += basis_w(k, 1) * invm; o.gm[2] += basis_w(k, 2) * invm;
+            } else {
+                o.gm[0] += basis_w_sel(k, ka) * invm;
+                if constexpr (NG == 2) o.gm[1] += basis_w_sel(k, kb) * invm;
+            }
+            if constexpr (SK || HA) {
+                const double invm3 = invm * invm * invm;
+                if constexpr (SK) {
+                    o.sm[0] += basis_w(k, 0) * invm3; o.sm[1] += basis_w(k, 1) * invm3;
+                    o.sm[2] += basis_w(k, 2) * invm3;
+                }
+                if constexpr (HA) {
+                    o.Am += r[R_WPHI + k] * invm3;
+                    o.Bm += TQ_W[k] * invm3;
+                }
+            }
+        }
+    }
+}
+
+// (y_q - x).n for every quadrature point (they lie in the triangle's plane),
+// and (y_q - R x).n for the image (image point R y with normal R n:
+// (R y - x).(R n) = (y - R x).n)
+__device__ __forceinline__ double far_zn(const double (&r)[28], double xi, double yi, double zi) {
+    return (r[R_A] - xi) * r[R_N] + (r[R_A + 1] - yi) * r[R_N + 1] + (r[R_A + 2] - zi) * r[R_N + 2];
+}
+__device__ __forceinline__ double far_znm(const double (&r)[28], double xi, double yi, double zi) {
+    return (r[R_A] + xi) * r[R_N] + (r[R_A + 1] - yi) * r[R_N + 1] + (r[R_A + 2] - zi) * r[R_N + 2];
+}
+
+// One patch entry of k_far without a stored H: G of the in-patch vertices
+// into the lane's LDS columns and, on the triangle's primary entry, its whole
+// H phi row contribution.  i in T: the direct part is the self panel (added
+// by k_self) and H = 0; the quadrature points are strictly inside T
+// (barycentric >= 0.0597), so the sums stay finite there and a zero
+// Jacobian removes them.
+template <bool MIRROR, bool PRIMARY, int NG>
+__device__ __forceinline__ void far_entry(const double (&r)[28], const uint32_t (&v)[3], uint32_t row, double xi,
+                                          double yi, double zi, double phii, const uint32_t (&sl)[3], int ka, int kb,
+                                          double (*accG)[ASM_ROWS], int tid, double& hphi) {
+    const FarGeo f = far_geo<MIRROR>(r, xi, yi, zi);
+    FarSums o;
+    far_sums<MIRROR, NG, false, PRIMARY>(r, f, ka, kb, o);
+    const double J = r[R_J];
+    const bool far = (row != v[0]) & (row != v[1]) & (row != v[2]);
+    const double Jf = far ? J : 0.0;
+    if constexpr (NG == 3) {
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            double Gv = Jf * o.g[l];
+            if constexpr (MIRROR) Gv += J * o.gm[l];
+            accG[sl[l]][tid] += Gv;
+        }
+    } else {
+        double Ga = Jf * o.g[0];
+        if constexpr (MIRROR) Ga += J * o.gm[0];
+        accG[sl[ka]][tid] += Ga;
+        if constexpr (NG == 2) {
+            double Gb = Jf * o.g[1];
+            if constexpr (MIRROR) Gb += J * o.gm[1];
+            accG[sl[kb]][tid] += Gb;
+        }
+    }
+    if constexpr (PRIMARY) {
+        // sum_k H_k (phi_k - phi_i), H_k = -((y - x).n) J s_k
+        hphi += (-far_zn(r, xi, yi, zi) * Jf) * (o.A - phii * o.B);
+        if constexpr (MIRROR) hphi += (-far_znm(r, xi, yi, zi) * J) * (o.Am - phii * o.Bm);
+    }
 }
 
 template <bool MIRROR, bool STORE_H, bool WRITE32>
@@ -116,63 +274,58 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
         const int e0 = pptr[p], e1 = pptr[p + 1];
         for (int e = e0; e < e1; ++e) {
             const double* __restrict__ q = rec + (size_t)TQ_STRIDE * e;
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(q + 28);
-            const uint32_t v0 = w[0], v1 = w[1], v2 = w[2], sl = w[3];
-            const double nx = q[21], ny = q[22], nz = q[23], J = q[24];
-            // every quadrature point lies in the triangle's plane: (y_q - x).n is one number
-            const double dy = q[1] - yi, dz = q[2] - zi;
-            const double zn = (q[0] - xi) * nx + dy * ny + dz * nz;
-            double g0 = 0, g1 = 0, g2 = 0, s0 = 0, s1 = 0, s2 = 0;
-            double gm0 = 0, gm1 = 0, gm2 = 0, sm0 = 0, sm1 = 0, sm2 = 0;
+            // the whole record before the variant is chosen: one batch of
+            // scalar loads an entry
+            double rv[28];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) {
-                const double zx = q[3 * k] - xi, zy = q[3 * k + 1] - yi, zz = q[3 * k + 2] - zi;
-                const double yz2 = zy * zy + zz * zz;
-                const double inv = rsqrt_nr(zx * zx + yz2);
-                const double inv3 = inv * inv * inv;
-                g0 += basis_w(k, 0) * inv; g1 += basis_w(k, 1) * inv; g2 += basis_w(k, 2) * inv;
-                s0 += basis_w(k, 0) * inv3; s1 += basis_w(k, 1) * inv3; s2 += basis_w(k, 2) * inv3;
-                if constexpr (MIRROR) {
-                    const double zxm = -q[3 * k] - xi;
-                    const double invm = rsqrt_nr(zxm * zxm + yz2);
-                    const double invm3 = invm * invm * invm;
-                    gm0 += basis_w(k, 0) * invm; gm1 += basis_w(k, 1) * invm; gm2 += basis_w(k, 2) * invm;
-                    sm0 += basis_w(k, 0) * invm3; sm1 += basis_w(k, 1) * invm3; sm2 += basis_w(k, 2) * invm3;
+            for (int i = 0; i < 28; ++i) rv[i] = q[i];
+            const uint32_t* qw = reinterpret_cast<const uint32_t*>(q + 28);
+            const uint32_t vv[3] = {qw[0], qw[1], qw[2]};
+            const uint32_t pk = qw[3];
+            const uint32_t sl[3] = {pk & 0xFFu, (pk >> 8) & 0xFFu, (pk >> 16) & 0xFFu};
+            const uint32_t r = (uint32_t)row;
+            if constexpr (!STORE_H) {
+                // G for the vertices in this patch only, and the triangle's
+                // H phi once, on its primary entry
+                const bool primary = (pk >> 24) & 1u;
+                const int ng = (sl[0] != 0xFFu) + (sl[1] != 0xFFu) + (sl[2] != 0xFFu);
+                // the in-patch basis indices when not all three are
+                const int ka = sl[0] != 0xFFu ? 0 : (sl[1] != 0xFFu ? 1 : 2);
+                const int kb = ng == 2 ? (sl[2] != 0xFFu ? 2 : 1) : ka;
+                if (primary) {
+                    if (ng == 3) far_entry<MIRROR, true, 3>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
+                    else if (ng == 2) far_entry<MIRROR, true, 2>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
+                    else far_entry<MIRROR, true, 1>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
+                } else {
+                    if (ng == 3) far_entry<MIRROR, false, 3>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
+                    else if (ng == 2) far_entry<MIRROR, false, 2>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
+                    else far_entry<MIRROR, false, 1>(rv, vv, r, xi, yi, zi, phii, sl, ka, kb, accG, tid, hphi);
                 }
-            }
-            // i in T: the direct part is the self panel (added by k_self) and H = 0.
-            // The quadrature points are strictly inside T (barycentric >= 0.0597),
-            // so g, s stay finite there and a zero Jacobian removes them.
-            const bool far = ((uint32_t)row != v0) & ((uint32_t)row != v1) & ((uint32_t)row != v2);
-            const double Jf = far ? J : 0.0;
-            const double Hf = -zn * Jf;
-            double G0 = Jf * g0, G1 = Jf * g1, G2 = Jf * g2;
-            double H0 = Hf * s0, H1 = Hf * s1, H2 = Hf * s2;
-            if constexpr (MIRROR) {
-                // image point R y (R: x -> -x) with normal R n: (Ry - x).(Rn) = (y - Rx).n
-                const double znm = (q[0] + xi) * nx + dy * ny + dz * nz;
-                const double Hm = -znm * J;
-                G0 += J * gm0; G1 += J * gm1; G2 += J * gm2;
-                H0 += Hm * sm0; H1 += Hm * sm1; H2 += Hm * sm2;
-            }
-            const uint32_t sa = sl & 0xFFu, sb = (sl >> 8) & 0xFFu, sc = (sl >> 16) & 0xFFu;
-            if (sa != 0xFFu) {
-                accG[sa][tid] += G0;
-                if constexpr (STORE_H) accH[sa][tid] += H0;
-                hphi += H0 * (q[25] - phii);
-                if constexpr (STORE_H) hrow += H0;
-            }
-            if (sb != 0xFFu) {
-                accG[sb][tid] += G1;
-                if constexpr (STORE_H) accH[sb][tid] += H1;
-                hphi += H1 * (q[26] - phii);
-                if constexpr (STORE_H) hrow += H1;
-            }
-            if (sc != 0xFFu) {
-                accG[sc][tid] += G2;
-                if constexpr (STORE_H) accH[sc][tid] += H2;
-                hphi += H2 * (q[27] - phii);
-                if constexpr (STORE_H) hrow += H2;
+            } else {
+                // stored H: all three G and H of the entry, each in-patch
+                // vertex's into its columns, H phi and the H row sum per vertex
+                const FarGeo f = far_geo<MIRROR>(rv, xi, yi, zi);
+                FarSums o;
+                far_sums<MIRROR, 3, true, false>(rv, f, 0, 0, o);
+                const bool far = (r != vv[0]) & (r != vv[1]) & (r != vv[2]);
+                const double J = rv[R_J];
+                const double Jf = far ? J : 0.0;
+                const double Hf = -far_zn(rv, xi, yi, zi) * Jf;
+                const double Hm = MIRROR ? -far_znm(rv, xi, yi, zi) * J : 0.0;
+                const double phk[3] = {rv[R_PHIA], q[R_PHIB], q[R_PHIC]};
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    if (sl[l] == 0xFFu) continue;
+                    double Gv = Jf * o.g[l], Hv = Hf * o.s[l];
+                    if constexpr (MIRROR) {
+                        Gv += J * o.gm[l];
+                        Hv += Hm * o.sm[l];
+                    }
+                    accG[sl[l]][tid] += Gv;
+                    accH[sl[l]][tid] += Hv;
+                    hphi += Hv * (phk[l] - phii);
+                    hrow += Hv;
+                }
             }
         }
         if (valid) {

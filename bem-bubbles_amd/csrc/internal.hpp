@@ -69,7 +69,8 @@ struct DevBuf {
 
 // ----------------------------------------------------- mesh topology --
 // Host-built, topology-only tables (rebuilt when the connectivity changes).
-constexpr int PATCH = 16;          // columns per assembly patch (16: 5 waves per SIMD; 32: 2.5, 25% slower)
+constexpr int PATCH = 12;          // columns per assembly patch (12: 6 waves per SIMD; measured best of 8-16)
+constexpr int ASM_WG_TARGET = 32768;   // assembly work-groups per launch (measured: 8192 28.2 ms, 32768 25.9 ms at N = 40962)
 constexpr int ASM_ROWS = 128;      // collocation rows per assembly work-group
 
 // One dense BEM problem inside a mesh: all vertices of the mesh (coupled

@@ -181,6 +181,10 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
                     const uint32_t sl = (ps[l] == ps[k]) ? (uint32_t)slot[t[l]] : 0xFFu;
                     packed |= sl << (8 * l);
                 }
+                // bit 24: the triangle's primary entry (the one of its first
+                // vertex's patch; exactly one a triangle), which carries the
+                // triangle's whole H phi row contribution
+                if (k == 0) packed |= 1u << 24;
                 const int32_t e = fill[ps[k]]++;
                 T.patch_tri[e] = (uint32_t)j;
                 T.patch_slots[e] = packed;
@@ -228,14 +232,17 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
         pr.p1 = T.npatch;
         T.probs.push_back(pr);
     }
-    // about 8192 work-groups in total, shared in proportion to n^2
+    // about ASM_WG_TARGET work-groups in total, shared in proportion to n^2
+    // (many small items: the last round of work-groups leaves fewer compute
+    // units idle)
+    const double wg_target = ASM_WG_TARGET;
     double work = 0.0;
     for (const AsmProb& pr : T.probs) work += (double)pr.n * pr.n;
     for (size_t q = 0; q < T.probs.size(); ++q) {
         AsmProb& pr = T.probs[q];
         const int rb = (pr.n + ASM_ROWS - 1) / ASM_ROWS;
         const int npp = pr.p1 - pr.p0;
-        const double want = std::max<double>(rb, 8192.0 * (double)pr.n * pr.n / work);
+        const double want = std::max<double>(rb, wg_target * (double)pr.n * pr.n / work);
         int groups = (int)std::ceil(want / rb);
         groups = std::max(1, std::min(groups, npp));
         const int ppg = (npp + groups - 1) / groups;
