@@ -61,15 +61,24 @@ __device__ __forceinline__ double basis_w_sel(int q, int k) {
 
 // The record of one patch entry (a triangle as seen from one patch), 32
 // doubles, streamed with scalar loads:
-//   [0..2] a   [3..5] 2 ab   [6..8] 2 bc   [9..15] c_q = |u_q ab + v_q bc|^2
-//   [16..22] w_q phi(y_q)   [23..25] unit normal   [26] J = |ab x bc|
+//   [0..2] 2 a   [3..5] 4 ab   [6..8] 4 bc   [9..15] 4 c_q, c_q = |u_q ab + v_q bc|^2
+//   [16..22] w_q phi(y_q)   [23..25] half the unit normal   [26] J = |ab x bc|
 //   [27], [30], [31] phi at a, b, c   [28..29] vertex ids a, b, c and the
 //   packed patch slots (bits 0-23) with the primary-entry flag (bit 24)
-// |y_q - x|^2 = |a - x|^2 + c_q + u_q (a - x).(2 ab) + v_q (a - x).(2 bc): the
-// dot products are formed once per (row, entry), so a quadrature point costs
-// one add and two FMAs instead of three differences and three products.
+// With d = 2 (a - x):  4 |y_q - x|^2 = |d|^2 + 4 c_q + u_q d.(4 ab) + v_q d.(4 bc).
+// The dot products are formed once per (row, entry), so a quadrature point
+// costs one add and two FMAs for 4 r^2; the factors 2 and 4 (exact) make
+// rsq(4 r^2) = 1 / (2 r), whose Newton step yh (3 - 4 r^2 yh^2) is 1 / r in
+// three operations.
 enum : int { R_A = 0, R_E1 = 3, R_E2 = 6, R_C = 9, R_WPHI = 16, R_N = 23, R_J = 26, R_PHIA = 27, R_PHIB = 30,
              R_PHIC = 31 };
+
+// 1 / r from a4 = 4 r^2: the hardware estimate of 1 / (2 r), refined by one
+// Newton step for 1 / sqrt(a4) and doubled in the same operations
+__device__ __forceinline__ double inv_r_from4(double a4) {
+    const double yh = __builtin_amdgcn_rsq(a4);
+    return yh * fma(-(a4 * yh), yh, 3.0);
+}
 
 __global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __restrict__ xyz,
                                                     const double* __restrict__ phi, const uint32_t* __restrict__ tri,
@@ -91,27 +100,27 @@ __global__ __launch_bounds__(256) void k_entry_prep(int nent, const double* __re
     nx *= rj; ny *= rj; nz *= rj;
     const double pa = phi ? phi[ia] : 0.0, pb = phi ? phi[ib] : 0.0, pc = phi ? phi[ic] : 0.0;
     double* o = rec + (size_t)TQ_STRIDE * e;
-    o[R_A] = ax; o[R_A + 1] = ay; o[R_A + 2] = az;
-    o[R_E1] = 2.0 * abx; o[R_E1 + 1] = 2.0 * aby; o[R_E1 + 2] = 2.0 * abz;
-    o[R_E2] = 2.0 * bcx; o[R_E2 + 1] = 2.0 * bcy; o[R_E2 + 2] = 2.0 * bcz;
+    o[R_A] = 2.0 * ax; o[R_A + 1] = 2.0 * ay; o[R_A + 2] = 2.0 * az;
+    o[R_E1] = 4.0 * abx; o[R_E1 + 1] = 4.0 * aby; o[R_E1 + 2] = 4.0 * abz;
+    o[R_E2] = 4.0 * bcx; o[R_E2 + 1] = 4.0 * bcy; o[R_E2 + 2] = 4.0 * bcz;
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
         const double u = q_u(q), v = q_v(q);
         const double wx = u * abx + v * bcx, wy = u * aby + v * bcy, wz = u * abz + v * bcz;
-        o[R_C + q] = wx * wx + wy * wy + wz * wz;
+        o[R_C + q] = 4.0 * (wx * wx + wy * wy + wz * wz);
         o[R_WPHI + q] = basis_w(q, 0) * pa + basis_w(q, 1) * pb + basis_w(q, 2) * pc;
     }
-    o[R_N] = nx; o[R_N + 1] = ny; o[R_N + 2] = nz;
+    o[R_N] = 0.5 * nx; o[R_N + 1] = 0.5 * ny; o[R_N + 2] = 0.5 * nz;
     o[R_J] = J;
     o[R_PHIA] = pa; o[R_PHIB] = pb; o[R_PHIC] = pc;
     uint32_t* w = reinterpret_cast<uint32_t*>(o + 28);
     w[0] = ia; w[1] = ib; w[2] = ic; w[3] = pslots[e];
 }
 
-// x's view of one entry: d0 = a - x and the dot products of the expansion
-// (and of the image point R x, R: x -> -x, in wall mode)
+// x's view of one entry: d = 2 (a - x) and the dot products of the expansion
+// (and of the image point R x, R: x -> -x, in wall mode); x2 = 2 x
 struct FarGeo {
-    double D, P, Q;      // |a - x|^2, (a - x).(2 ab), (a - x).(2 bc)
+    double D, P, Q;      // |d|^2, d.(4 ab), d.(4 bc)
     double Dm, Pm, Qm;   // the same for the image triangle R T and x
 };
 
@@ -126,7 +135,7 @@ __device__ __forceinline__ FarGeo far_geo(const double (&r)[28], double xi, doub
     f.P = dx * r[R_E1] + pyz;
     f.Q = dx * r[R_E2] + qyz;
     if constexpr (MIRROR) {
-        // R T has vertex R a and edges R ab, R bc: (R a - x) = (-a_x - x, ..)
+        // R T has vertex R a and edges R ab, R bc: 2 (R a - x) = (-2 a_x - 2 x, ..)
         const double dxm = -r[R_A] - xi;
         f.Dm = dxm * dxm + dyz;
         f.Pm = -dxm * r[R_E1] + pyz;
@@ -155,7 +164,7 @@ __device__ __forceinline__ void far_sums(const double (&r)[28], const FarGeo& f,
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
         const double c = r[R_C + k];
-        const double inv = rsqrt_nr(fma(q_v(k), f.Q, fma(q_u(k), f.P, f.D + c)));
+        const double inv = inv_r_from4(fma(q_v(k), f.Q, fma(q_u(k), f.P, f.D + c)));
         if constexpr (NG == 3) {
             o.g[0] += basis_w(k, 0) * inv; o.g[1] += basis_w(k, 1) * inv; o.g[2] += basis_w(k, 2) * inv;
         } else {
@@ -173,7 +182,7 @@ __device__ __forceinline__ void far_sums(const double (&r)[28], const FarGeo& f,
             }
         }
         if constexpr (MIRROR) {
-            const double invm = rsqrt_nr(fma(q_v(k), f.Qm, fma(q_u(k), f.Pm, f.Dm + c)));
+            const double invm = inv_r_from4(fma(q_v(k), f.Qm, fma(q_u(k), f.Pm, f.Dm + c)));
             if constexpr (NG == 3) {
                 o.gm[0] += basis_w(k, 0) * invm; o.gm[1] += basis_w(k, 1) * invm; o.gm[2] += basis_w(k, 2) * invm;
             } else {
@@ -197,7 +206,7 @@ __device__ __forceinline__ void far_sums(const double (&r)[28], const FarGeo& f,
 
 // (y_q - x).n for every quadrature point (they lie in the triangle's plane),
 // and (y_q - R x).n for the image (image point R y with normal R n:
-// (R y - x).(R n) = (y - R x).n)
+// (R y - x).(R n) = (y - R x).n); x2 = 2 x, the record holds 2 a and n / 2
 __device__ __forceinline__ double far_zn(const double (&r)[28], double xi, double yi, double zi) {
     return (r[R_A] - xi) * r[R_N] + (r[R_A + 1] - yi) * r[R_N + 1] + (r[R_A + 2] - zi) * r[R_N + 2];
 }
@@ -260,7 +269,8 @@ __global__ __launch_bounds__(ASM_ROWS) void k_far(
     const bool valid = rloc < pr.n;
     const int row = pr.v0 + (valid ? rloc : pr.n - 1);   // global vertex id
     const size_t ld = (size_t)pr.n;
-    const double xi = xyz[3 * row], yi = xyz[3 * row + 1], zi = xyz[3 * row + 2];
+    // twice the collocation point: the records hold 2 a (exact scalings)
+    const double xi = 2.0 * xyz[3 * row], yi = 2.0 * xyz[3 * row + 1], zi = 2.0 * xyz[3 * row + 2];
     const double phii = phi ? phi[row] : 0.0;
     double hphi = 0.0, hrow = 0.0;
 
