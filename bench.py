@@ -41,12 +41,12 @@ PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "l
 # factorisation's launches), from two rocprofv3 --pmc runs (FETCH_SIZE, then
 # WRITE_SIZE, each scaled by what it reads for a plain copy of known size):
 # profiles/lu16_update_gemm_nu64_hbm_traffic.md (fp16 LU),
-# profiles/lu_update_gemm_and_assembly_nu64_hbm_traffic.md (split-fp16 LU)
-TRAFFIC_NU64 = {"lu16_ir64": 3.767e9, "lu16x3_ir64": 5.99e9}
+# profiles/split16_update_gemm_nu64_counters.md (split-fp16 LU)
+TRAFFIC_NU64 = {"lu16_ir64": 3.767e9, "lu16x3_ir64": 6.21e9}
 # MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
 # GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
-# profiles/lu16_update_gemm_nu64_mfma_utilisation.md, profiles/lu_update_gemm_nu64_mfma_utilisation.md
-MFMA_BUSY_NU64 = {"lu16_ir64": (0.252, 2.05), "lu16x3_ir64": (0.511, 1.82)}
+# profiles/lu16_update_gemm_nu64_mfma_utilisation.md, profiles/split16_update_gemm_nu64_counters.md
+MFMA_BUSY_NU64 = {"lu16_ir64": (0.252, 2.05), "lu16x3_ir64": (0.517, 1.82)}
 DTYPE = {"lu16_ir64": "fp16-mfma lu (fp32 acc, fp32 factor)+f64-ir", "lu16x3_ir64": "f32-lu(split-f16 mfma)+f64-ir",
          "lu32_ir64": "f32-lu+f64-ir", "lu64": "f64", "bicgstab64": "f64"}
 
@@ -468,6 +468,13 @@ def main():
         sim.set_solver("bicgstab64", tol=1e-10)
         sim.compute_psi()
         errb = float(np.max(np.abs(sim.get_psi() - psi64)) / np.max(np.abs(psi64)))
+        # the FP32-class mixed solver (every block product as three fp16
+        # products, FP32 accumulation) on the same state
+        x3 = "lu16x3_ir64"
+        if args.solver != x3:
+            sim.set_solver(x3)
+            sim.compute_psi()
+            errx3 = float(np.max(np.abs(sim.get_psi() - psi64)) / np.max(np.abs(psi64)))
         sim.set_solver("lu64")
         sim.accum_stats(reset=True)
         t0 = time.perf_counter()
@@ -491,6 +498,18 @@ def main():
                                 "iterations_per_stage": ab["iterations"] / 4.0,
                                 "psi_rel_err_vs_fp64_lu": errb,
                                 "speedup_of_mixed": round(bcg_ms / ms_stage, 3)}
+        if args.solver != x3:
+            sim.set_solver(x3)
+            sim.accum_stats(reset=True)
+            t0 = time.perf_counter()
+            sim.evolve_system_RK4(case.dt, True)
+            x3_ms = (time.perf_counter() - t0) * 1e3 / 4
+            ax3 = sim.accum_stats(reset=True)
+            log(f"[bench] fp32-class {x3} {x3_ms:.2f} ms/stage err={errx3:.3e} {ax3}")
+            out["fp32_class_leg"] = {"solver": x3, "dtype": DTYPE[x3], "ms_per_stage": round(x3_ms, 3),
+                                     "psi_rel_err_vs_fp64_lu": errx3,
+                                     "speedup_vs_fp64": round(fp64_ms / x3_ms, 3),
+                                     "refinement_sweeps_per_stage": ax3["iterations"] / 4.0}
     sim.close()
     ctx.close()
 
