@@ -1379,7 +1379,8 @@ static void split_gemm(Solver* ctx, rocblas_handle h, int m, int n, int k, int c
 // split_a / split_b and A, B are not read here.
 template <class T>
 static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m, int n, int k, const T* A, int lda,
-                            const T* B, int ldb, T* C, int ldc, bool split16, int col0, int p, int slot) {
+                            const T* B, int ldb, T* C, int ldc, bool split16, int col0, int p, int slot,
+                            const Mm16Tiles* tiles = nullptr) {
     if (m <= 0 || n <= 0 || k <= 0) return;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->timing) {
@@ -1401,7 +1402,7 @@ static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m
         if (ctx->lu16)
             mm16_gemm(st, MM_SUB, 0, ctx->lu.t_l21[p].p, k, ctx->lu.t_u12.p, k, col0, m, n, k, C, ldc,
                       std::ldexp(1.0f, ctx->lu16_e - 22),
-                      ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0);
+                      ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0, tiles);
         else if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc, p, slot);
         else gemm_minus(h, m, n, k, A, lda, B, ldb, C, ldc);
     } else {
@@ -1443,7 +1444,8 @@ static void lu16_panel_products(Solver* ctx, Lane L, int m, int n, const float* 
     if (l21 != A + n) {
         mm16_pack(L.s, true, n, n, ctx->lu.winv_u.as<float>(), n, std::ldexp(1.0f, ctx->lu16_e - 2), ctx->lu.t_wu.p,
                   log);
-        mm16_pack(L.s, false, m - n, n, A + n, lda, lu16_sa(ctx), ctx->lu.t_a21.p, log);
+        // A21's tiles, unless the update that produced A21 wrote them
+        if (!ctx->lu16_a21_ready) mm16_pack(L.s, false, m - n, n, A + n, lda, lu16_sa(ctx), ctx->lu.t_a21.p, log);
         const Mm16Tiles tl{ctx->lu.t_l21[p].p, false, n, 0, 16384.0f, log, maxl};
         mm16_gemm(L.s, MM_STORE, 2, ctx->lu.t_a21.p, n, ctx->lu.t_wu.p, n, 0, m - n, n, n, l21, ldl,
                   std::ldexp(1.0f, -6), 0, &tl);
@@ -1456,17 +1458,20 @@ static void lu16_panel_products(Solver* ctx, Lane L, int m, int n, const float* 
 
 // columns [c0, c0 + nc) of step p's block row (jb rows at A12, ld lda):
 // U12 = L11^-1 A12 into the factor with its B tiles, then the trailing update
-// of those columns (rest rows below the block row)
-static void lu16_cols(Solver* ctx, Lane L, int jb, int rest, int c0, int nc, float* A12, int lda, int p) {
+// of those columns (rest rows below the block row).  a12_tiled: A12's tiles
+// were written by the previous step's update; upd: tiles the update writes of
+// its result for the next step (the next panel's A21 or the next block row).
+static void lu16_cols(Solver* ctx, Lane L, int jb, int rest, int c0, int nc, float* A12, int lda, int p,
+                      bool a12_tiled = false, const Mm16Tiles* upd = nullptr) {
     if (nc <= 0) return;
     unsigned int* log = ctx->lu.f16log.as<unsigned int>();
     const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;   // bytes of c0 / 32 column blocks
     float* a12 = A12 + (size_t)c0 * lda;
-    mm16_pack(L.s, true, nc, jb, a12, lda, lu16_sa(ctx), ctx->lu.t_a12.as<char>() + off, log);
+    if (!a12_tiled) mm16_pack(L.s, true, nc, jb, a12, lda, lu16_sa(ctx), ctx->lu.t_a12.as<char>() + off, log);
     const Mm16Tiles tu{ctx->lu.t_u12.p, true, jb, c0, lu16_sa(ctx), log, nullptr};
     mm16_gemm(L.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda,
               std::ldexp(1.0f, ctx->lu16_e - 8), 0, &tu);
-    trailing_update<float>(ctx, L.s, L.h, rest, nc, jb, nullptr, 0, nullptr, 0, a12 + jb, lda, false, c0, p, 0);
+    trailing_update<float>(ctx, L.s, L.h, rest, nc, jb, nullptr, 0, nullptr, 0, a12 + jb, lda, false, c0, p, 0, upd);
 }
 
 // ------------------------------------------------------------ factorise --
@@ -1737,6 +1742,11 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
         }
         cv.notify_all();
     };
+    // fp16 LU: the updates of step k also write the tiles step k + 1 packs
+    // otherwise: the next panel's A21 (narrow update) and the next block row
+    // (wide update; valid when the next panel takes no row interchanges)
+    bool a12_tiled = false;
+    ctx->lu16_a21_ready = false;
     try {
         int k = 0, jb = width(N), p = 0;
         for (;;) {
@@ -1758,10 +1768,19 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
             // U12 product before it); otherwise one copy at the end.
             bool in_factor = false;
             bool f16 = false;
+            // the block row's tiles from the previous update only if this
+            // step's panel kept its rows
+            const bool a12_ok = a12_tiled && !pivoted;
+            a12_tiled = false;
             if constexpr (std::is_same<T, float>::value) {
                 if (ctx->lu16) {
-                    // U12 goes into the factor with its tiles
-                    lu16_cols(ctx, main, jb, rest, 0, jb2, A12, lda, p);
+                    // U12 goes into the factor with its tiles; the narrow update
+                    // writes the next panel's A21 tiles (rows below its top block)
+                    const bool a21 = rest > jb2;
+                    const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb2, 0, lu16_sa(ctx), ctx->lu.f16log.as<unsigned int>(),
+                                       nullptr, jb2};
+                    lu16_cols(ctx, main, jb, rest, 0, jb2, A12, lda, p, a12_ok, a21 ? &ta : nullptr);
+                    ctx->lu16_a21_ready = a21;
                     in_factor = f16 = true;
                 }
             }
@@ -1777,7 +1796,14 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
                 T* Uw = U + (size_t)jb2 * jb;
                 T* A12w = A12 + (size_t)jb2 * lda;
                 if constexpr (std::is_same<T, float>::value)
-                    if (f16) lu16_cols(ctx, main, jb, rest, jb2, rest - jb2, A12, lda, p);
+                    if (f16) {
+                        // the wide update writes the next block row's tiles
+                        // (its first jb2 rows), used if the next panel keeps its rows
+                        const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb2, 0, lu16_sa(ctx),
+                                           ctx->lu.f16log.as<unsigned int>(), nullptr};
+                        lu16_cols(ctx, main, jb, rest, jb2, rest - jb2, A12, lda, p, a12_ok, &tb);
+                        a12_tiled = true;
+                    }
                 if (!f16) {
                     gemm_nn(main.h, jb, rest - jb2, jb, W, jb, A12w, lda, Uw, jb);
                     if constexpr (std::is_same<T, float>::value)
@@ -1808,6 +1834,7 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
     }
     post(Job{0, 0, 0, true});
     panels.join();
+    ctx->lu16_a21_ready = false;
     if (err) std::rethrow_exception(err);
 }
 
@@ -1825,6 +1852,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     const int LU_NB = f16 ? ctx->nb16 : split16 ? ctx->nb_split : (sizeof(T) == 8 ? ctx->nb64 : ctx->nb32);
     BH_CHECK(!f16 || LU_NB % MM_TILE == 0, BEMHIP_ERR_ARG, "fp16 LU block must be a multiple of 256");
     BH_CHECK(LU_NB >= LEAF && LU_NB % LEAF == 0, BEMHIP_ERR_ARG, "LU block must be a multiple of 32");
+    ctx->lu16_a21_ready = false;   // the first panel packs its own A21
     ctx->lu.panel_ws.ensure(PanelWs::BYTES);
     char* ws = ctx->lu.panel_ws.as<char>();
     // work buffers at their largest (the first step's) before anything is

@@ -182,6 +182,9 @@ struct Solver {
     // triangular solve's error word, so every waiting work-group gives up at
     // once and the solve must report BEMHIP_ERR_HIP
     bool trsv_force_giveup = false;
+    // fp16 LU: the next panel's A21 tiles were written by the update that
+    // produced A21 (set by the main thread before it hands the panel over)
+    bool lu16_a21_ready = false;
     // single problems of at most SMALL_MAX_N unknowns through the one-work-group
     // FP64 solver (BEMHIP_SMALL_SINGLE=1; leaves no factors for a transposed solve)
     bool small_single = false;
@@ -287,6 +290,7 @@ struct Mm16Tiles {
     float scale;
     unsigned int* maxlog;        // max |scale x| (float bits)
     unsigned long long* amax64;  // A only: max |x| (double bits)
+    int r0 = 0;                  // A only: C's rows from r0 on (a multiple of 256) are the buffer's rows
 };
 void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
                int K, float* C, long ldc, float alpha, int max_wg = 0, const Mm16Tiles* tiles = nullptr);

@@ -196,11 +196,12 @@ struct MmArgs {
     long ldc;
     float alpha;
     int KBa, KBb, cb0, M, N, nks, tiles_m, tiles_n, per_xcd, wg_xcd;
-    // MM_STORE only: the stored result also as fp16 tiles (tkind 1: A tiles,
-    // k = column of C; 2: B tiles, k = row of C, column blocks from tcb0),
-    // tKB blocks per row (column) block, scaled by tscale; logs as the packs
+    // the result (stored, or C after the subtraction) also as fp16 tiles
+    // (tkind 1: A tiles, k = column of C, C's rows from tr0 on; 2: B tiles,
+    // k = row of C, column blocks from tcb0), tKB blocks per row (column)
+    // block, scaled by tscale; logs as the packs
     h8* tout;
-    int tkind, tKB, tcb0;
+    int tkind, tKB, tcb0, tr0;
     float tscale;
     unsigned int* maxlog;
     unsigned long long* amax64;
@@ -331,7 +332,8 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 *reinterpret_cast<float4*>(ep + (l & 31) * CP + i * 32 + g * 8 + (l >> 5) * 4) = v;
             }
         if (EPI == MM_STORE && g.tkind == 1) {
-            // A tiles of C: this chunk is 4 row blocks x 2 k blocks
+            // A tiles of C: this chunk is 4 row blocks x 2 k blocks (stores
+            // have tr0 = 0)
 #pragma unroll
             for (int ib = 0; ib < 4; ++ib)
 #pragma unroll
@@ -374,12 +376,20 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 g.tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
             }
         }
+        // subtraction with tiles: each lane puts its final values back into
+        // the wave's LDS (zeros beyond M and N, as the packs pad), and the
+        // tiles are cut from there after the chunk
+        const bool sub_tiles = EPI == MM_SUB && g.tkind != 0 &&
+                               (g.tkind == 1 ? row0 + 128 > g.tr0 : row0 < g.tKB * 16);
 #pragma unroll
         for (int c2 = 0; c2 < 16; ++c2) {
             const int cl = 2 * c2 + (l >> 5);          // column within the 32
             const int col = col0 + j * 32 + cl;
             const float4 a = *reinterpret_cast<const float4*>(ep + cl * CP + rr);
-            if (col >= N) continue;
+            if (col >= N) {
+                if (sub_tiles) *reinterpret_cast<float4*>(ep + cl * CP + rr) = float4{0.f, 0.f, 0.f, 0.f};
+                continue;
+            }
             float* cc = C + (size_t)col * ldc + row0 + rr;
             if (rows_ok) {
                 float4 v;
@@ -396,20 +406,68 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                     v.w = alpha * a.w;
                 }
                 __builtin_memcpy(cc, &v, 16);
+                if (sub_tiles) *reinterpret_cast<float4*>(ep + cl * CP + rr) = v;
             } else {
                 const float av[4] = {a.x, a.y, a.z, a.w};
+                float fv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     if (row0 + rr + e < M) {
-                        if (EPI == MM_SUB) cc[e] -= alpha * av[e];
-                        else cc[e] = alpha * av[e];
+                        if (EPI == MM_SUB) fv[e] = cc[e] - alpha * av[e];
+                        else fv[e] = alpha * av[e];
+                        cc[e] = fv[e];
                     }
+                if (sub_tiles) *reinterpret_cast<float4*>(ep + cl * CP + rr) = float4{fv[0], fv[1], fv[2], fv[3]};
+            }
+        }
+        if (sub_tiles && g.tkind == 1) {
+            // A tiles of the result's rows from tr0 on: 4 row blocks x 2 k blocks
+#pragma unroll
+            for (int ib = 0; ib < 4; ++ib) {
+                const int rb = row0 + ib * 32 - g.tr0;
+                if (rb < 0) continue;
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+                    const int kb = (col0 + j * 32) / 16 + kk;
+                    if (kb >= g.tKB) continue;
+                    const int rl = ib * 32 + (l & 31), kl = kk * 16 + (l >> 5) * 8;
+                    h8 v;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float y = g.tscale * ep[(kl + e) * CP + rl];
+                        const float a = fabsf(y);
+                        tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
+                        v[e] = (_Float16)y;
+                    }
+                    g.tout[((size_t)(rb / 32) * g.tKB + kb) * 64 + l] = v;
+                }
+            }
+        } else if (sub_tiles && g.tkind == 2) {
+            // B tiles of the result: one column block x 8 k blocks (rows)
+            const int cb = g.tcb0 + (col0 + j * 32) / 32;
+#pragma unroll
+            for (int kbl = 0; kbl < 8; ++kbl) {
+                const int kb = row0 / 16 + kbl;
+                if (kb >= g.tKB) continue;
+                const int kl = kbl * 16 + (l >> 5) * 8;
+                const float4 p0 = *reinterpret_cast<const float4*>(ep + (l & 31) * CP + kl);
+                const float4 p1 = *reinterpret_cast<const float4*>(ep + (l & 31) * CP + kl + 4);
+                const float xs[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                h8 v;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float y = g.tscale * xs[e];
+                    const float a = fabsf(y);
+                    tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
+                    v[e] = (_Float16)y;
+                }
+                g.tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
             }
         }
     }
     __syncthreads();   // the epilogue's LDS is the next tile's staging buffer
     }
-    if (EPI == MM_STORE && g.tkind) pack_logs(tmax, tamax, g.maxlog, g.tkind == 1 ? g.amax64 : nullptr);
+    if (g.tkind) pack_logs(tmax, tamax, g.maxlog, EPI == MM_STORE && g.tkind == 1 ? g.amax64 : nullptr);
 }
 
 // max_wg > 0: at most that many work-groups (one per compute unit), each
@@ -420,7 +478,9 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
     if (M <= 0 || N <= 0) return;
     BH_CHECK(c0 % MM_TILE == 0, BEMHIP_ERR_ARG, "mm16: column offset must be a multiple of 256");
     BH_CHECK(K <= Ka && K <= Kb, BEMHIP_ERR_ARG, "mm16: K exceeds the operand tiles");
-    BH_CHECK(!(tiles && epi != MM_STORE), BEMHIP_ERR_ARG, "mm16: tiles of the result only for stores");
+    BH_CHECK(!(tiles && epi == MM_SUB && tiles->amax64), BEMHIP_ERR_ARG, "mm16: no max |x| for subtractions");
+    BH_CHECK(!(tiles && epi != MM_SUB && tiles->r0 != 0), BEMHIP_ERR_ARG, "mm16: row offset only for subtractions");
+    BH_CHECK(!(tiles && tiles->r0 % MM_TILE != 0), BEMHIP_ERR_ARG, "mm16: tile row offset must be a multiple of 256");
     MmArgs g{};
     g.At = static_cast<const h8*>(At);
     g.Bt = static_cast<const h8*>(Bt);
@@ -445,6 +505,7 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
         g.tscale = tiles->scale;
         g.maxlog = tiles->maxlog;
         g.amax64 = tiles->amax64;
+        g.tr0 = tiles->b_operand ? 0 : tiles->r0;
         BH_CHECK(tiles->c0 % 32 == 0, BEMHIP_ERR_ARG, "mm16: tile column offset must be a multiple of 32");
     }
     const dim3 grid(8 * g.wg_xcd);
