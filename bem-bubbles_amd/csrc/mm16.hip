@@ -182,9 +182,9 @@ void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long 
 // upper triangular (tile column tn only needs k < 256 (tn + 1)).
 //
 // A work-group (4 waves) owns a 256 x 256 tile of C; wave (wm, wn) a 128 x 128
-// quarter as 4 x 4 MFMA 32x32 accumulators (256 FP32 per lane).  Per k-step of
-// 64 the work-group stages 32 KB of A and 32 KB of B in LDS by direct-to-LDS
-// loads, double-buffered.  The launch covers
+// quarter as 4 x 4 MFMA 32x32 accumulators (256 FP32 per lane).  Per stage of
+// 32 k the work-group stages 16 KB of A and 16 KB of B in LDS by direct-to-LDS
+// loads, four stages deep.  The launch covers
 // the tiles in groups of 8 tile rows, and consecutive work-group ids (which
 // the hardware deals round-robin to the 8 XCDs) are remapped so that every XCD
 // works through a contiguous range of groups: the work-groups sharing an L2
@@ -216,9 +216,13 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const float alpha = g.alpha;
     const int KBa = g.KBa, KBb = g.KBb, cb0 = g.cb0, M = g.M, N = g.N, nks = g.nks;
     const int tiles_m = g.tiles_m, tiles_n = g.tiles_n, per_xcd = g.per_xcd, wg_xcd = g.wg_xcd;
-    // two stages of [A 32 KB | B 32 KB]
-    constexpr int STAGE = 2 * MM_RB * MM_BKB * 64;   // h8 per stage
-    __shared__ h8 lds[2 * STAGE];
+    // four stages of [A 16 KB | B 16 KB], 32 k each: three stages' loads are
+    // in flight while one is computed (two 64-k stages left the next stage's
+    // loads a single stage of compute to arrive in, which they often missed)
+    constexpr int SKB = 2;                          // 16-k blocks per stage
+    constexpr int NST = 4;                          // stages
+    constexpr int STAGE = 2 * MM_RB * SKB * 64;     // h8 per stage
+    __shared__ h8 lds[NST * STAGE];
     const int xcd = blockIdx.x & 7;
     float tmax = 0.0f;    // max |tscale x| of the fused tiles
     double tamax = 0.0;   // max |x|
@@ -233,6 +237,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     int kend = nks;
     if (TRI == 1) kend = min(nks, (tm + 1) * (MM_TILE / MM_BK));
     if (TRI == 2) kend = min(nks, (tn + 1) * (MM_TILE / MM_BK));
+    const int nst = kend * (MM_BKB / SKB);          // stages of this tile
     // the wave index as a scalar: every staging address is then a scalar base
     // plus the lane's 16 bytes
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
@@ -240,22 +245,39 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const h8* abase = At + (size_t)tm * MM_RB * KBa * 64;
     const h8* bbase = Bt + (size_t)(cb0 + tn * MM_RB) * KBb * 64;
 
-    // wave w stages blocks 8w .. 8w+7 of A and of B (block a: local row block
-    // a >> 2, k block a & 3; LDS slot a)
-    auto issue = [&](int ks) {
-        h8* sA = lds + (ks & 1) * STAGE;
+    // wave w stages blocks 4w .. 4w+3 of A and of B (block a: local row block
+    // a >> 1, k block a & 1 of the stage; LDS slot a): 8 loads a lane
+    auto issue = [&](int st) {
+        h8* sA = lds + (st % NST) * STAGE;
         h8* sB = sA + STAGE / 2;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int a = w * 8 + u;
-            const h8* src = abase + ((size_t)(a >> 2) * KBa + ks * MM_BKB + (a & 3)) * 64;
+        for (int u = 0; u < 4; ++u) {
+            const int a = w * 4 + u;
+            const h8* src = abase + ((size_t)(a >> 1) * KBa + st * SKB + (a & 1)) * 64;
             lds_dma16(src + l, sA + a * 64);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int b = w * 8 + u;
-            const h8* src = bbase + ((size_t)(b >> 2) * KBb + ks * MM_BKB + (b & 3)) * 64;
+        for (int u = 0; u < 4; ++u) {
+            const int b = w * 4 + u;
+            const h8* src = bbase + ((size_t)(b >> 1) * KBb + st * SKB + (b & 1)) * 64;
             lds_dma16(src + l, sB + b * 64);
+        }
+    };
+    // the epilogue's view of this wave's 128 x 128 quarter of C
+    const int row0 = tm * MM_TILE + wm * 128, col0 = tn * MM_TILE + wn * 128;
+    const int rr = 4 * (l & 31);   // this lane's 4 rows (of 128) when reading back
+    const bool rows_ok = row0 + rr + 3 < M;
+    // C of a 32-column chunk is loaded while the one before is transposed and
+    // written: 16 loads of 16 bytes per lane in flight
+    float4 cv[2][16];
+    auto load_chunk = [&](int j, float4* c) {
+        if (EPI != MM_SUB) return;
+#pragma unroll
+        for (int c2 = 0; c2 < 16; ++c2) {
+            const int col = col0 + j * 32 + 2 * c2 + (l >> 5);
+            const float* cc = C + (size_t)col * ldc + row0 + rr;
+            if (col < N && rows_ok) __builtin_memcpy(&c[c2], cc, 16);
+            else c[c2] = float4{0.f, 0.f, 0.f, 0.f};
         }
     };
     f32x16 acc[4][4];
@@ -263,29 +285,47 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x16{};
-    // Per k-step: wait for the stage and for every wave to be done with the
-    // previous one, start the next stage's loads into the other half of LDS,
-    // then read this stage's fragments and run its 64 MFMAs while those loads
-    // are in flight.
-    if (kend > 0) issue(0);
-    for (int ks = 0; ks < kend; ++ks) {
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // Per stage: wait for its loads (the younger stages' may stay in flight)
+    // and for every wave to be done with the oldest slot, start the loads of
+    // the stage three ahead into it, then read this stage's fragments and run
+    // its 32 MFMAs.
+#pragma unroll
+    for (int st = 0; st < NST - 1; ++st)
+        if (st < nst) issue(st);
+    for (int st = 0; st < nst; ++st) {
+        const int ahead = min(NST - 2, nst - 1 - st);   // younger stages in flight (8 loads each)
+        if (ahead >= 2) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+        else if (ahead == 1) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if (ks + 1 < kend) issue(ks + 1);
-        const h8* sA = lds + (ks & 1) * STAGE;
+        if (st + NST - 1 < nst) issue(st + NST - 1);
+        const h8* sA = lds + (st % NST) * STAGE;
         const h8* sB = sA + STAGE / 2;
+        // fragments double-buffered in registers: the eight LDS reads of k
+        // block s + 1 are in flight while block s's 16 MFMAs run (read one
+        // at a time, each read's latency stalled the single wave of a SIMD)
+        h8 fa[2][4], fb[2][4];
 #pragma unroll
-        for (int s = 0; s < MM_BKB; ++s) {
-            h8 a[4], b[4];
+        for (int i = 0; i < 4; ++i) fa[0][i] = sA[((wm * 4 + i) * SKB) * 64 + l];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sA[((wm * 4 + i) * MM_BKB + s) * 64 + l];
+        for (int j = 0; j < 4; ++j) fb[0][j] = sB[((wn * 4 + j) * SKB) * 64 + l];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = sB[((wn * 4 + j) * MM_BKB + s) * 64 + l];
+        for (int s = 0; s < SKB; ++s) {
+            const int cur = s & 1, nxt = cur ^ 1;
+            if (s + 1 < SKB) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) fa[nxt][i] = sA[((wm * 4 + i) * SKB + s + 1) * 64 + l];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fb[nxt][j] = sB[((wn * 4 + j) * SKB + s + 1) * 64 + l];
+            }
+            // keep the reads ahead of the MFMAs (the scheduler would otherwise
+            // move each read down to its first use)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cur][i], fb[cur][j], acc[i][j], 0, 0, 0);
         }
     }
 
@@ -300,22 +340,6 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     constexpr int CP = 132;
     __syncthreads();   // every wave is done with the staging buffers
     float* ep = reinterpret_cast<float*>(lds) + w * (32 * CP);
-    const int row0 = tm * MM_TILE + wm * 128, col0 = tn * MM_TILE + wn * 128;
-    const int rr = 4 * (l & 31);   // this lane's 4 rows (of 128) when reading back
-    const bool rows_ok = row0 + rr + 3 < M;
-    // C of the next 32-column chunk is loaded while the current one is
-    // transposed and written: 16 loads of 16 bytes per lane in flight
-    float4 cv[2][16];
-    auto load_chunk = [&](int j, float4* c) {
-        if (EPI != MM_SUB) return;
-#pragma unroll
-        for (int c2 = 0; c2 < 16; ++c2) {
-            const int col = col0 + j * 32 + 2 * c2 + (l >> 5);
-            const float* cc = C + (size_t)col * ldc + row0 + rr;
-            if (col < N && rows_ok) __builtin_memcpy(&c[c2], cc, 16);
-            else c[c2] = float4{0.f, 0.f, 0.f, 0.f};
-        }
-    };
     load_chunk(0, cv[0]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
