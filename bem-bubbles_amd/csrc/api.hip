@@ -105,6 +105,97 @@ void merge_stats(bemhip_solve_stats& sum, const bemhip_solve_stats& one) {
     sum.panels_pivoted += one.panels_pivoted;
 }
 
+// the small problems' one launch on stream ls (job table uploaded there)
+static void small_launch(bemhip_ctx* ctx, hipStream_t ls, double* G64, const double* d_hphi, double* d_psi) {
+    const int nsmall = (int)ctx->small_jobs_h.size();
+    SmallJob* dj = static_cast<SmallJob*>(ctx->small_jobs.ensure(sizeof(SmallJob) * nsmall));
+    int* dinfo = static_cast<int*>(ctx->small_info.ensure(sizeof(int) * nsmall));
+    // the host table stays alive in the context until the next call, which
+    // comes after this one's results were read back
+    BH_HIP(hipMemcpyAsync(dj, ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall, hipMemcpyHostToDevice, ls));
+    BH_HIP(hipEventRecord(ctx->small_ev[0], ls));
+    small_solve_launch(ls, dj, nsmall, G64, d_hphi, d_psi, dinfo);
+    BH_HIP(hipEventRecord(ctx->small_ev[1], ls));
+}
+
+// the small problems' info words and launch time
+static void small_finish(bemhip_ctx* ctx, hipStream_t ls, bemhip_solve_stats& sum) {
+    const int nsmall = (int)ctx->small_jobs_h.size();
+    std::vector<int> info(nsmall);
+    BH_HIP(hipMemcpyAsync(info.data(), ctx->small_info.p, sizeof(int) * nsmall, hipMemcpyDeviceToHost, ls));
+    BH_HIP(hipStreamSynchronize(ls));
+    float ms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&ms, ctx->small_ev[0], ctx->small_ev[1]));
+    sum.ms_factor += ms;
+    for (int q = 0; q < nsmall; ++q)
+        BH_CHECK(info[q] == 0, BEMHIP_ERR_SINGULAR,
+                 "zero pivot in column " + std::to_string(info[q]) + " of a batch problem with " +
+                     std::to_string(ctx->small_jobs_h[q].n) + " unknowns");
+}
+
+// lu16_ir64 batches: the problems of each size class factored in lock step
+// (class_lu16_enqueue), every class on a solver lane of its own (largest
+// first; lanes reused round robin when there are more classes), the small
+// problems' launch on the next lane.  All work is enqueued before the first
+// read-back, so the classes overlap on the device.
+static void solve_classes(bemhip_ctx* ctx, const DevTopology& topo, const std::vector<int>& big, double* G64,
+                          float* G32, const double* d_hphi, double* d_psi, double tol, int maxit,
+                          bemhip_solve_stats* st) {
+    std::vector<ClassJob> cls;
+    {
+        std::vector<int> order(big);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return topo.probs[a].n > topo.probs[b].n; });
+        for (int q : order) {
+            const AsmProb& pr = topo.probs[q];
+            if (cls.empty() || cls.back().n != pr.n) {
+                cls.emplace_back();
+                cls.back().n = pr.n;
+                cls.back().gofs0 = pr.gofs;
+            }
+            ClassJob& c = cls.back();
+            // the G layout puts a class's blocks one after the other, in part order
+            BH_CHECK(pr.gofs == c.gofs0 + (int64_t)c.cnt * gblock_stride(pr.n), BEMHIP_ERR_ARG,
+                     "batched LU: a size class's G blocks are not evenly spaced");
+            c.v0.push_back(pr.v0);
+            c.cnt += 1;
+        }
+    }
+    const int nsmall = (int)ctx->small_jobs_h.size();
+    const int nc = (int)cls.size();
+    const int L = std::max(1, std::min(ctx->max_lanes, nc + (nsmall > 0 ? 1 : 0)));
+    while ((int)ctx->lanes.size() < L - 1) {
+        ctx->lanes.emplace_back(new Solver());
+        ctx->lanes.back()->init(ctx);
+    }
+    std::vector<Solver*> lane(L);
+    lane[0] = ctx;
+    for (int l = 1; l < L; ++l) lane[l] = ctx->lanes[l - 1].get();
+    while ((int)ctx->class_work.size() < nc) ctx->class_work.emplace_back(new ClassWork());
+    hipEvent_t ready = ctx->ev[0];
+    BH_HIP(hipEventRecord(ready, ctx->stream));
+    for (int l = 1; l < L; ++l) BH_HIP(hipStreamWaitEvent(lane[l]->stream, ready, 0));
+    const int sweeps = std::max(0, std::min(ctx->batch_sweeps, maxit > 0 ? maxit : ctx->batch_sweeps));
+    for (int i = 0; i < nc; ++i)
+        class_lu16_enqueue(lane[i % L], ctx->class_work[i].get(), cls[i], G64, G32, d_hphi, tol, sweeps);
+    const int l_small = nc % L;
+    if (nsmall > 0) small_launch(ctx, lane[l_small]->stream, G64, d_hphi, d_psi);
+    bemhip_solve_stats sum{};
+    sum.converged = 1;
+    for (int i = 0; i < nc; ++i) {
+        bemhip_solve_stats one{};
+        class_lu16_finish(lane[i % L], ctx->class_work[i].get(), cls[i], G64, G32, d_hphi, d_psi, tol, maxit, &one);
+        merge_stats(sum, one);
+    }
+    if (nsmall > 0) small_finish(ctx, lane[l_small]->stream, sum);
+    for (int l = 1; l < L; ++l) {
+        hipEvent_t done = ctx->ev[1];
+        BH_HIP(hipEventRecord(done, lane[l]->stream));
+        BH_HIP(hipStreamWaitEvent(ctx->stream, done, 0));
+    }
+    *st = sum;
+}
+
 // Independent problems of a batch: the largest first, dealt to up to
 // max_lanes solvers by accumulated n^3 (the LU cost); every solver has its
 // own streams and work buffers and is driven by its own host thread, so the
@@ -127,6 +218,10 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
     const int nsmall = (int)ctx->small_jobs_h.size();
     if (nsmall > 0 && !ctx->small_ev[0])
         for (auto& e : ctx->small_ev) BH_HIP(hipEventCreate(&e));
+    if (solver == BEMHIP_SOLVE_LU16_IR64 && ctx->batch_lu && G32) {
+        solve_classes(ctx, topo, big, G64, G32, d_hphi, d_psi, tol, maxit, st);
+        return;
+    }
     const int np = (int)big.size();
     const int L = std::max(1, std::min(ctx->max_lanes, np));
     while ((int)ctx->lanes.size() < L - 1) {
@@ -169,18 +264,7 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
                              d_psi + pr.v0, solver, tol, maxit, &one, report_residual);
                 merge_stats(sum, one);
             }
-            if (l == l_small && nsmall > 0) {
-                hipStream_t ls = lane[l]->stream;
-                SmallJob* dj = static_cast<SmallJob*>(ctx->small_jobs.ensure(sizeof(SmallJob) * nsmall));
-                int* dinfo = static_cast<int*>(ctx->small_info.ensure(sizeof(int) * nsmall));
-                // the host table stays alive in the context until the next
-                // call, which comes after this one's results were read back
-                BH_HIP(hipMemcpyAsync(dj, ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall,
-                                      hipMemcpyHostToDevice, ls));
-                BH_HIP(hipEventRecord(ctx->small_ev[0], ls));
-                small_solve_launch(ls, dj, nsmall, G64, d_hphi, d_psi, dinfo);
-                BH_HIP(hipEventRecord(ctx->small_ev[1], ls));
-            }
+            if (l == l_small && nsmall > 0) small_launch(ctx, lane[l]->stream, G64, d_hphi, d_psi);
         } catch (...) {
             fail[l] = std::current_exception();
         }
@@ -196,19 +280,7 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
     }
     bemhip_solve_stats sum{};
     sum.converged = 1;
-    if (nsmall > 0 && !fail[l_small]) {
-        hipStream_t ls = lane[l_small]->stream;
-        std::vector<int> info(nsmall);
-        BH_HIP(hipMemcpyAsync(info.data(), ctx->small_info.p, sizeof(int) * nsmall, hipMemcpyDeviceToHost, ls));
-        BH_HIP(hipStreamSynchronize(ls));
-        float ms = 0.0f;
-        BH_HIP(hipEventElapsedTime(&ms, ctx->small_ev[0], ctx->small_ev[1]));
-        sum.ms_factor += ms;
-        for (int q = 0; q < nsmall; ++q)
-            BH_CHECK(info[q] == 0, BEMHIP_ERR_SINGULAR,
-                     "zero pivot in column " + std::to_string(info[q]) + " of a batch problem with " +
-                         std::to_string(ctx->small_jobs_h[q].n) + " unknowns");
-    }
+    if (nsmall > 0 && !fail[l_small]) small_finish(ctx, lane[l_small]->stream, sum);
     for (auto& f : fail) if (f) std::rethrow_exception(f);
     for (const auto& p : part) merge_stats(sum, p);
     *st = sum;
@@ -438,6 +510,10 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (s1 && s1[0] == '1') ctx->small_single = true;
         const char* sf = std::getenv("BEMHIP_SPLIT_FIXED");
         if (sf && sf[0] == '0') ctx->split_fixed = false;
+        const char* bl = std::getenv("BEMHIP_BATCH_LU");
+        if (bl && bl[0] == '0') ctx->batch_lu = false;
+        if (const char* v = std::getenv("BEMHIP_BATCH_SWEEPS")) ctx->batch_sweeps = std::max(0, std::atoi(v));
+        if (const char* v = std::getenv("BEMHIP_BATCH_FORCE_REDO")) ctx->batch_force_redo = std::atoi(v);
         ctx->init(nullptr);
     });
     if (rc != BEMHIP_OK) {
@@ -454,6 +530,7 @@ int bemhip_destroy(bemhip_ctx* ctx) {
     if (!ctx) return BEMHIP_OK;
     (void)hipSetDevice(ctx->device);
     for (auto& l : ctx->lanes) l->release();
+    ctx->class_work.clear();
     ctx->lanes.clear();
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->ev) if (e) (void)hipEventDestroy(e);

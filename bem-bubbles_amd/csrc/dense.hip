@@ -738,9 +738,17 @@ __device__ inline floatx16 mfma_rows32p(const float* __restrict__ W, const float
 // diagonal block) goes through LDS.  Only final values are stored: the U rows
 // of every step, the factored diagonal block and the multipliers below it.
 // backup (optional, ld n) receives the block's original values.
+// Batched (gticket set): one launch factors the top blocks of a batch of
+// problems, `stripes` stripes each; the ticket taken from gticket names the
+// problem (ticket / stripes) and the stripe, and problem b's block, sync words
+// and published inverses start b sA, b sstride, b slinv elements further.
+// Tickets are dealt in dispatch order, so every stripe waited for belongs to
+// the same problem and to an already running work-group.
 __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, int lda,
                                                   unsigned int* __restrict__ sync, unsigned int epoch,
-                                                  float* __restrict__ linv, float* __restrict__ backup) {
+                                                  float* __restrict__ linv, float* __restrict__ backup,
+                                                  unsigned int* __restrict__ gticket, int stripes, long long sA,
+                                                  int sstride, long long slinv) {
     constexpr int MAXW = 8;
     __shared__ float Us[NPB * NPB];      // U rows of step j: Us[i * NPB + c]
     __shared__ float Wt[NPB * NPB];      // published L^-1 of stripe j (transposed) / own U^-1
@@ -749,15 +757,21 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
     __shared__ float invd[NPB];
     __shared__ unsigned long long red[MAXW];
     __shared__ int s_stripe, s_go;
-    unsigned int* ticket = sync + 2;
-    unsigned int* err = sync + 3;
-    unsigned int* flags = sync + 16;
     const int tid = threadIdx.x, lane = tid & 63, nt = blockDim.x;
     const int wv = tid >> 6, nw = nt >> 6;
     const int li = lane & 31, kh = lane >> 5;
-    if (tid == 0) s_stripe = (int)atomicAdd(ticket, 1u);
+    if (tid == 0) s_stripe = (int)atomicAdd(gticket ? gticket : sync + 2, 1u);
     __syncthreads();
-    const int s = s_stripe;
+    int s = s_stripe;
+    if (gticket) {
+        const int q = s / stripes;
+        s -= q * stripes;
+        A += q * sA;
+        sync += (long long)q * sstride;
+        linv += q * slinv;
+    }
+    unsigned int* err = sync + 3;
+    unsigned int* flags = sync + 16;
     const int c0 = s * NPB, w = min(NPB, n - c0);
     float* S = A + (size_t)c0 * lda;
     floatx16 sv[TOP_RBW];
@@ -1540,7 +1554,7 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
     if (top32<T>(ctx, n)) {
         if constexpr (std::is_same<T, float>::value)
             k_np_top32<<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, ctx->lu.diag.as<T>(),
-                                                               ctx->lu.backup.as<T>());
+                                                               ctx->lu.backup.as<T>(), nullptr, stripes, 0, 0, 0);
     } else if (ctx->np_top && stripes <= TOP_STRIPES && n <= top_max_rows<T>()) {
         T* linv = ctx->lu.diag.as<T>();
         k_np_top<T><<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, linv);
@@ -2126,10 +2140,14 @@ void pivots_to_perm(const std::vector<int>& ipiv, std::vector<int>& perm) {
 // ---- persistent triangular solves -------------------------------------
 // FP64 inverses of the TS x TS diagonal blocks (column-major per block), one
 // work-group per block; thread c builds column c of the inverse.
+// grid.y: problem of a batch (A and inv sA, sInv elements further)
 template <class T, bool LOWER>
-__global__ __launch_bounds__(TS) void k_diag_inv(int N, const T* __restrict__ A, double* __restrict__ inv) {
+__global__ __launch_bounds__(TS) void k_diag_inv(int N, const T* __restrict__ A, double* __restrict__ inv,
+                                                 long long sA, long long sInv) {
     extern __shared__ double ys[];   // [TS][TS]: ys[r * TS + c]
     const int b = blockIdx.x, c = threadIdx.x;
+    A += blockIdx.y * sA;
+    inv += blockIdx.y * sInv;
     const int j0 = b * TS, bs = min(TS, N - j0);
     const T* Ab = A + (size_t)j0 * N + j0;
     if (LOWER) {
@@ -2167,22 +2185,41 @@ __global__ __launch_bounds__(TS) void k_diag_inv(int N, const T* __restrict__ A,
 // x_i = inv_i (b_i - sum) is published and the block's flag raised.
 constexpr int TRSV_BATCH = 16;
 
+// Batched (tb.gticket set): the solves of a batch of problems in one launch;
+// the ticket names the problem (ticket / blocks) and its block in solve order,
+// problem q's factor, inverses, x (ld N) and sync words start q sA, q sInv,
+// q N, q sstride elements further, and the problems with active[q] == 0 are
+// skipped (all their blocks return at once, so nothing waits for them).
+struct TrsvBat {
+    unsigned int* gticket;
+    long long sA, sInv;
+    int sstride;
+    const int* active;
+};
 template <class T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict__ A, const double* __restrict__ inv,
                                                       double* __restrict__ x, unsigned int* __restrict__ sync,
-                                                      unsigned int epoch) {
+                                                      unsigned int epoch, TrsvBat tb) {
     __shared__ double xs[TRSV_BATCH][TS];
     __shared__ double part[2][TS];
     __shared__ int s_t, s_ready;
-    unsigned int* ticket = sync;
+    const int nb = (N + TS - 1) / TS;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_t = (int)atomicAdd(tb.gticket ? tb.gticket : sync, 1u);
+    __syncthreads();
+    int t = s_t;
+    if (tb.gticket) {
+        const int q = t / nb;
+        t -= q * nb;
+        if (tb.active && !tb.active[q]) return;   // the same for the whole work-group
+        A += q * tb.sA;
+        inv += q * tb.sInv;
+        x += (long long)q * N;
+        sync += (long long)q * tb.sstride;
+    }
     unsigned int* err = sync + 1;      // this launch: a work-group gave up waiting
     unsigned int* sticky = sync + 2;   // any launch since the host cleared it
     unsigned int* flags = sync + 16;
-    const int nb = (N + TS - 1) / TS;
-    const int tid = threadIdx.x;
-    if (tid == 0) s_t = (int)atomicAdd(ticket, 1u);
-    __syncthreads();
-    const int t = s_t;
     const int ib = LOWER ? t : nb - 1 - t;
     const int i0 = ib * TS, bs = min(TS, N - i0);
     const int r = tid & (TS - 1), h = tid >> 7;
@@ -2311,8 +2348,8 @@ template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
     const size_t sh = sizeof(double) * TS * TS;
     BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_diag_inv<T, true><<<nb, TS, sh, ctx->stream>>>(N, LU, il);
-    k_diag_inv<T, false><<<nb, TS, sh, ctx->stream>>>(N, LU, iu);
+    k_diag_inv<T, true><<<nb, TS, sh, ctx->stream>>>(N, LU, il, 0, 0);
+    k_diag_inv<T, false><<<nb, TS, sh, ctx->stream>>>(N, LU, iu, 0, 0);
     BH_HIP(hipGetLastError());
     const size_t fb = sizeof(unsigned int) * (16 + (size_t)nb);
     if (ctx->lu.flag.bytes < fb) {
@@ -2331,8 +2368,11 @@ template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, dou
     const unsigned int epoch = ++ctx->trsv_epoch;
     k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error word
     if (ctx->trsv_force_giveup) k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync + 1, 1, 1u);
-    if (lower) k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch);
-    else k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), x, sync, epoch);
+    const TrsvBat one{nullptr, 0, 0, 0, nullptr};
+    if (lower)
+        k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch, one);
+    else
+        k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), x, sync, epoch, one);
     BH_HIP(hipGetLastError());
 }
 
@@ -2381,11 +2421,18 @@ template void lu_solve<float>(Solver*, int, const float*, const int*, double*);
 // ------------------------------------------------------ gemv, reductions --
 constexpr int GV_CHUNKS = 32;
 
+// grid.z: problem q of a batch (A sA elements further, x and the partials
+// q N and q gridDim.y N further); problems with active[q] == 0 are skipped
 __global__ __launch_bounds__(256) void k_gemv_part(int N, const double* __restrict__ A, const double* __restrict__ x,
-                                                   double* __restrict__ part, int chunk) {
+                                                   double* __restrict__ part, int chunk, long long sA,
+                                                   const int* __restrict__ active) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int c0 = blockIdx.y * chunk, c1 = min(N, c0 + chunk);
-    if (r >= N) return;
+    const int q = blockIdx.z;
+    if (r >= N || (active && !active[q])) return;
+    A += q * sA;
+    x += (long long)q * N;
+    part += (long long)q * gridDim.y * N;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int c = c0;
     const double* col = A + (size_t)c0 * N + r;
@@ -2402,10 +2449,16 @@ __global__ __launch_bounds__(256) void k_gemv_part(int N, const double* __restri
     part[(size_t)blockIdx.y * N + r] = (a0 + a1) + (a2 + a3);
 }
 
+// grid.y: problem q of a batch (vectors q N further)
 __global__ __launch_bounds__(256) void k_gemv_finish(int N, int chunks, const double* __restrict__ part,
-                                                     const double* __restrict__ b, double* __restrict__ r) {
+                                                     const double* __restrict__ b, double* __restrict__ r,
+                                                     const int* __restrict__ active) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    const int q = blockIdx.y;
+    if (i >= N || (active && !active[q])) return;
+    part += (long long)q * chunks * N;
+    if (b) b += (long long)q * N;
+    r += (long long)q * N;
     double s = 0.0;
     for (int k = 0; k < chunks; ++k) s += part[(size_t)k * N + i];
     r[i] = b ? b[i] - s : s;
@@ -2416,8 +2469,8 @@ void gemv64(Solver* ctx, int N, const double* A, const double* x, const double* 
     const int chunk = (N + chunks - 1) / chunks;
     const int nch = (N + chunk - 1) / chunk;
     double* part = static_cast<double*>(ctx->gemv_part.ensure(sizeof(double) * (size_t)nch * N));
-    k_gemv_part<<<dim3((N + 255) / 256, nch), 256, 0, ctx->stream>>>(N, A, x, part, chunk);
-    k_gemv_finish<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, nch, part, b, r);
+    k_gemv_part<<<dim3((N + 255) / 256, nch), 256, 0, ctx->stream>>>(N, A, x, part, chunk, 0, nullptr);
+    k_gemv_finish<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, nch, part, b, r, nullptr);
     BH_HIP(hipGetLastError());
 }
 
@@ -2781,6 +2834,517 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
     st->correction = cn;
 }
 
+// ------------------------------------------- batched fp16 LU (size classes) --
+// The problems of one size class advance in lock step: every launch of a step
+// (top blocks, triangular inverses, packs, the three block products) covers
+// all of them, so a step's launches are as large as one big problem's, and no
+// decision goes through the host.  Panels are factored without interchanges;
+// max |l| over the whole factorisation is kept per problem and checked once at
+// the end (<= 1: partial pivoting would have kept the diagonal at every column,
+// so the factors are GEPP's); a problem that fails it is solved again on its
+// own.  Scales are per problem: e[q] from max |diag A| as in the single fp16 LU.
+
+ClassWork::~ClassWork() {
+    for (hipEvent_t& v : ev)
+        if (v) (void)hipEventDestroy(v);
+}
+
+// per problem: the scale exponent from max |diag A|, and the words the
+// factorisation merges into cleared (top-block max |l| and error words, the L21
+// max |l|, the fp16 log)
+__global__ __launch_bounds__(256) void k_cls_prep(int n, const float* __restrict__ A, long long sA, int* __restrict__ e,
+                                                  unsigned int* __restrict__ top_sync, int tss,
+                                                  unsigned long long* __restrict__ maxl64,
+                                                  unsigned int* __restrict__ f16log) {
+    __shared__ float red[256];
+    const int q = blockIdx.x, t = threadIdx.x;
+    const float* Aq = A + q * sA;
+    float m = 0.0f;
+    for (int i = t; i < n; i += 256) {
+        const float a = fabsf(Aq[(size_t)i * n + i]);
+        m = (a <= 3.0e38f) ? fmaxf(m, a) : __builtin_inff();
+    }
+    red[t] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = fmaxf(red[t], red[t + w]);
+        __syncthreads();
+    }
+    if (t == 0) {
+        const float dm = red[0];
+        int ex = 0;
+        if (dm > 0.0f && dm <= 3.0e38f) frexpf(dm, &ex);
+        e[q] = max(-100, min(100, ex));
+        unsigned int* s = top_sync + (long long)q * tss;
+        s[0] = 0u;
+        s[1] = 0u;
+        s[3] = 0u;
+        maxl64[q] = 0ull;
+        f16log[q] = 0u;
+    }
+}
+
+// word `word` of each of cnt regions (stride words apart) = v, and *ticket = 0
+__global__ void k_cls_words(int cnt, unsigned int* __restrict__ base, int stride, int word, unsigned int v,
+                            unsigned int* __restrict__ ticket) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < cnt) base[(long long)q * stride + word] = v;
+    if (q == 0 && ticket) *ticket = 0u;
+}
+
+// dst[q n + i] = src[v0[q] + i]
+__global__ void k_cls_gather(int n, const double* __restrict__ src, const int* __restrict__ v0,
+                             double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i < n) dst[(long long)q * n + i] = src[v0[q] + i];
+}
+// x[v0[q] + i] = src[q n + i]
+__global__ void k_cls_scatter(int n, const double* __restrict__ src, const int* __restrict__ v0,
+                              double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i < n) x[v0[q] + i] = src[(long long)q * n + i];
+}
+
+// refinement state of a class (cnt problems): doubles bn, corr, then ints
+// active, conv, iters
+struct IrState {
+    double* bn;
+    double* corr;
+    int* active;
+    int* conv;
+    int* iters;
+};
+__device__ inline double block_max256(double v, double* red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] = fmax(red[t], red[t + w]);
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+// max |x| with NaN propagated (fmax would drop it)
+__device__ inline double absmax_nan(double m, double x) {
+    const double a = fabs(x);
+    return (a != a || m != m) ? __longlong_as_double(0x7ff8000000000000ll) : fmax(m, a);
+}
+// one work-group per problem: |b|_inf, every problem refining
+__global__ __launch_bounds__(256) void k_cls_ir_init(int n, const double* __restrict__ b, IrState s) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = absmax_nan(m, b[(long long)q * n + i]);
+    m = block_max256(m, red);
+    if (threadIdx.x == 0) {
+        s.bn[q] = m;
+        s.corr[q] = 0.0;
+        s.active[q] = 1;
+        s.conv[q] = 0;
+        s.iters[q] = 0;
+    }
+}
+// the residual test before a sweep: stop once |r|_inf <= rtol |b|_inf (or r
+// is not finite: stopped, not converged)
+__global__ __launch_bounds__(256) void k_cls_ir_res(int n, const double* __restrict__ r, double rtol, IrState s) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    if (!s.active[q]) return;   // the same for the whole work-group
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = absmax_nan(m, r[(long long)q * n + i]);
+    m = block_max256(m, red);
+    if (threadIdx.x == 0) {
+        if (!(m <= 1.0e308)) s.active[q] = 0;
+        else if (m <= rtol * s.bn[q]) { s.active[q] = 0; s.conv[q] = 1; }
+    }
+}
+// after a sweep's solves: x += d, then stop once |d|_inf <= tol |x|_inf
+__global__ __launch_bounds__(256) void k_cls_ir_corr(int n, double* __restrict__ x, const double* __restrict__ d,
+                                                     double tol, IrState s) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    if (!s.active[q]) return;
+    double dm = 0.0, xm = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const long long k = (long long)q * n + i;
+        const double dv = d[k], xv = x[k] + dv;
+        x[k] = xv;
+        dm = absmax_nan(dm, dv);
+        xm = absmax_nan(xm, xv);
+    }
+    dm = block_max256(dm, red);
+    xm = block_max256(xm, red);
+    if (threadIdx.x == 0) {
+        const double cn = xm > 0.0 ? dm / xm : dm;
+        s.iters[q] += 1;
+        s.corr[q] = cn;
+        if (!(cn <= 1.0e308)) s.active[q] = 0;
+        else if (cn <= tol) { s.active[q] = 0; s.conv[q] = 1; }
+    }
+}
+
+namespace {
+constexpr int CLS_TRSV_WORDS_MIN = 16;
+inline float p2(int k) { return std::ldexp(1.0f, k); }
+
+struct ClassPtrs {
+    ClassWork* w;
+    float* A;          // problem 0's G32 block
+    long long sA;      // elements between problems
+    int n, cnt, NB;
+    hipStream_t main, side;
+    int max_wg;        // persistent update grid (0: one work-group per tile)
+};
+
+IrState ir_state(ClassWork* w) {
+    double* d = w->scal.as<double>();
+    int* i = reinterpret_cast<int*>(d + 2 * (size_t)w->cnt);
+    return IrState{d, d + w->cnt, i, i + w->cnt, i + 2 * (size_t)w->cnt};
+}
+
+// the panel of the step at column k (m rows, jb columns) of every problem:
+// top block, both triangular inverses, L21 = A21 U11^-1 in place with its
+// tiles, L11^-1 as tiles (parity p)
+void cls_panel(const ClassPtrs& c, hipStream_t s, int k, int m, int jb, int p, bool a21_ready) {
+    ClassWork* w = c.w;
+    const int n = c.n, cnt = c.cnt;
+    float* Ap = c.A + (size_t)k * n + k;
+    const int stripes = (jb + NPB - 1) / NPB;
+    unsigned int* gt = w->tickets.as<unsigned int>();
+    k_fill_u32<<<1, 64, 0, s>>>(gt, 1, 0u);
+    k_np_top32<<<stripes * cnt, top_threads<float>(jb), 0, s>>>(
+        jb, Ap, n, w->top_sync.as<unsigned int>(), ++w->top_epoch, w->linv.as<float>(), nullptr, gt, stripes, c.sA,
+        TOP_SYNC_WORDS, (long long)TOP_STRIPES * NPB * NPB);
+    BH_HIP(hipGetLastError());
+    if (m <= jb) return;
+    const long long sW = (long long)c.NB * c.NB;
+    const size_t te = tri_inv_tmp_elems(jb);
+    float* tinv = w->tinv.as<float>();
+    const TriBat tb{cnt, c.sA, sW, (int64_t)(2 * te)};
+    tri_inv2<float>(s, jb, Ap, n, w->winv_l[p].as<float>(), w->winv_u.as<float>(), tinv, tinv + te, &tb);
+    unsigned int* log = w->f16log.as<unsigned int>();
+    const int* e = w->e.as<int>();
+    Mm16Bat bw{};
+    bw.n = cnt; bw.sx = sW; bw.sa = (long long)w->tb_w; bw.e = e; bw.ek = 1; bw.ls = 1;
+    mm16_pack(s, true, jb, jb, w->winv_u.as<float>(), jb, p2(-2), w->t_wu.p, log, nullptr, 0, nullptr, &bw);
+    if (!a21_ready) {
+        Mm16Bat ba{};
+        ba.n = cnt; ba.sx = c.sA; ba.sa = (long long)w->tb_a21; ba.e = e; ba.ek = -1; ba.ls = 1;
+        mm16_pack(s, false, m - jb, jb, Ap + jb, n, p2(8), w->t_a21.p, log, nullptr, 0, nullptr, &ba);
+    }
+    const Mm16Tiles tl{w->t_l21[p].p, false, jb, 0, 16384.0f, log, w->maxl64.as<unsigned long long>()};
+    Mm16Bat bl{};
+    bl.n = cnt; bl.sx = c.sA; bl.sa = (long long)w->tb_a21; bl.sb = (long long)w->tb_w; bl.st = (long long)w->tb_a21;
+    bl.ls = 1;
+    mm16_gemm(s, MM_STORE, 2, w->t_a21.p, jb, w->t_wu.p, jb, 0, m - jb, jb, jb, Ap + jb, n, p2(-6), 0, &tl, &bl);
+    Mm16Bat bi{};
+    bi.n = cnt; bi.sx = sW; bi.sa = (long long)w->tb_w; bi.ls = 1;
+    mm16_pack(s, false, jb, jb, w->winv_l[p].as<float>(), jb, 1.0f, w->t_wl[p].p, log, nullptr, 0, nullptr, &bi);
+}
+
+// columns [c0, c0 + nc) of the block row of the step at column k (jb rows):
+// U12 = L11^-1 A12 in place with its tiles, then the trailing update of those
+// columns; upd: tiles the update writes of its result for the next step
+void cls_cols(const ClassPtrs& c, int k, int jb, int rest, int c0, int nc, int p, bool a12_tiled,
+              const Mm16Tiles* upd, size_t upd_stride) {
+    if (nc <= 0) return;
+    ClassWork* w = c.w;
+    const int n = c.n, cnt = c.cnt;
+    unsigned int* log = w->f16log.as<unsigned int>();
+    const int* e = w->e.as<int>();
+    float* a12 = c.A + (size_t)(k + jb + c0) * n + k;
+    if (!a12_tiled) {
+        const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;
+        Mm16Bat bp{};
+        bp.n = cnt; bp.sx = c.sA; bp.sa = (long long)w->tb_a12; bp.e = e; bp.ek = -1; bp.ls = 1;
+        mm16_pack(c.main, true, nc, jb, a12, n, p2(8), w->t_a12.as<char>() + off, log, nullptr, 0, nullptr, &bp);
+    }
+    const Mm16Tiles tu{w->t_u12.p, true, jb, c0, p2(8), log, nullptr};
+    Mm16Bat bu{};
+    bu.n = cnt; bu.sx = c.sA; bu.sa = (long long)w->tb_w; bu.sb = (long long)w->tb_a12; bu.st = (long long)w->tb_a12;
+    bu.e = e; bu.ek = 1; bu.ek2 = -1; bu.ls = 1;
+    mm16_gemm(c.main, MM_STORE, 1, w->t_wl[p].p, jb, w->t_a12.p, jb, c0, jb, nc, jb, a12, n, p2(-8), 0, &tu, &bu);
+    Mm16Bat bg{};
+    bg.n = cnt; bg.sx = c.sA; bg.sa = (long long)w->tb_a21; bg.sb = (long long)w->tb_a12;
+    bg.st = (long long)upd_stride; bg.e = e; bg.ek = 1; bg.ek2 = -1; bg.ls = 1;
+    mm16_gemm(c.main, MM_SUB, 0, w->t_l21[p].p, jb, w->t_u12.p, jb, c0, rest, nc, jb, a12 + jb, n, p2(-22),
+              c.max_wg, upd, &bg);
+}
+
+// the FP32 triangular solves of every problem (active: the refining ones) on x
+// (ld n per problem), in place
+void cls_trsv(const ClassPtrs& c, double* x, const int* active) {
+    ClassWork* w = c.w;
+    const int n = c.n, cnt = c.cnt, nbk = w->nbk;
+    const int tss = CLS_TRSV_WORDS_MIN + nbk;
+    unsigned int* ts = w->tsync.as<unsigned int>();
+    unsigned int* gt = w->tickets.as<unsigned int>() + 1;
+    const long long sInv = (long long)nbk * TS * TS;
+    for (int lower = 1; lower >= 0; --lower) {
+        k_cls_words<<<(cnt + 255) / 256, 256, 0, c.main>>>(cnt, ts, tss, 1, 0u, gt);
+        const TrsvBat tb{gt, c.sA, sInv, tss, active};
+        if (lower)
+            k_trsv_persist<float, true><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_lo.as<double>(), x, ts,
+                                                                      ++w->trsv_epoch, tb);
+        else
+            k_trsv_persist<float, false><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_up.as<double>(), x, ts,
+                                                                       ++w->trsv_epoch, tb);
+        BH_HIP(hipGetLastError());
+    }
+}
+
+// one refinement sweep of the refining problems: r = b - G x (FP64), the
+// residual test, d = (LU)^-1 r, x += d and the correction test
+void cls_sweep(const ClassPtrs& c, const double* G64, const double* bb, double* xx, double* rr, double tol) {
+    ClassWork* w = c.w;
+    const int n = c.n, cnt = c.cnt, nch = w->nch;
+    const int chunk = (n + nch - 1) / nch;
+    const IrState s = ir_state(w);
+    k_gemv_part<<<dim3((n + 255) / 256, nch, cnt), 256, 0, c.main>>>(n, G64, xx, w->part.as<double>(), chunk, c.sA,
+                                                                     s.active);
+    k_gemv_finish<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, nch, w->part.as<double>(), bb, rr, s.active);
+    k_cls_ir_res<<<cnt, 256, 0, c.main>>>(n, rr, IR_RTOL, s);
+    BH_HIP(hipGetLastError());
+    cls_trsv(c, rr, s.active);
+    k_cls_ir_corr<<<cnt, 256, 0, c.main>>>(n, xx, rr, tol, s);
+    BH_HIP(hipGetLastError());
+}
+
+ClassPtrs class_ptrs(Solver* ctx, ClassWork* w, const ClassJob& cj, float* G32) {
+    ClassPtrs c{};
+    c.w = w;
+    c.A = G32 + cj.gofs0;
+    c.sA = gblock_stride(cj.n);
+    c.n = cj.n;
+    c.cnt = cj.cnt;
+    c.NB = w->nb;
+    c.main = ctx->stream;
+    c.side = ctx->lookahead ? ctx->stream2 : ctx->stream;
+    c.max_wg = ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0;
+    return c;
+}
+
+// class vectors: b, x, r (cnt x n doubles each)
+double* cls_vec(ClassWork* w, int i) { return w->vecs.as<double>() + (size_t)i * w->cnt * w->n; }
+
+void class_work_ensure(Solver* ctx, ClassWork* w, const ClassJob& cj) {
+    const int n = cj.n, cnt = cj.cnt;
+    const int NB = std::min(ctx->nb16, n);
+    BH_CHECK(ctx->nb16 % MM_TILE == 0 && ctx->nb16 <= top_max_rows<float>(), BEMHIP_ERR_ARG,
+             "batched fp16 LU: block must be a multiple of 256 and at most 1024");
+    BH_CHECK(cnt >= 1 && (long long)cnt * TOP_STRIPES <= (1ll << 30), BEMHIP_ERR_ARG, "batched fp16 LU: bad class");
+    const int nbk = (n + TS - 1) / TS;
+    const bool fresh = w->n != n || w->cnt != cnt || w->nb != NB;
+    w->n = n;
+    w->cnt = cnt;
+    w->nb = NB;
+    w->nbk = nbk;
+    {
+        // column chunks of the residual GEMV, as gemv64 cuts them
+        const int chunks = std::min(GV_CHUNKS, n), chunk = (n + chunks - 1) / chunks;
+        w->nch = (n + chunk - 1) / chunk;
+    }
+    w->tb_a21 = mm_tile_bytes(n, NB);
+    w->tb_w = mm_tile_bytes(NB, NB);
+    w->tb_a12 = mm_tile_bytes(n + MM_TILE, NB);
+    const size_t C = (size_t)cnt;
+    w->e.ensure(sizeof(int) * C);
+    w->tickets.ensure(256);
+    w->maxl64.ensure(sizeof(unsigned long long) * C);
+    w->f16log.ensure(sizeof(unsigned int) * C);
+    w->linv.ensure(sizeof(float) * C * TOP_STRIPES * NPB * NPB);
+    for (int p = 0; p < 2; ++p) {
+        w->winv_l[p].ensure(sizeof(float) * C * NB * NB);
+        w->t_l21[p].ensure(C * w->tb_a21);
+        w->t_wl[p].ensure(C * w->tb_w);
+    }
+    w->winv_u.ensure(sizeof(float) * C * NB * NB);
+    w->tinv.ensure(sizeof(float) * C * 2 * tri_inv_tmp_elems(NB));
+    w->t_a21.ensure(C * w->tb_a21);
+    w->t_wu.ensure(C * w->tb_w);
+    w->t_a12.ensure(C * w->tb_a12);
+    w->t_u12.ensure(C * w->tb_a12);
+    w->inv_lo.ensure(sizeof(double) * C * nbk * TS * TS);
+    w->inv_up.ensure(sizeof(double) * C * nbk * TS * TS);
+    w->vecs.ensure(sizeof(double) * 3 * C * n);
+    w->part.ensure(sizeof(double) * C * w->nch * n);
+    w->scal.ensure((2 * sizeof(double) + 3 * sizeof(int)) * C);
+    w->v0.ensure(sizeof(int) * C);
+    const size_t tsw = C * TOP_SYNC_WORDS, trw = C * (CLS_TRSV_WORDS_MIN + nbk);
+    if (fresh || w->top_sync.bytes < sizeof(unsigned int) * tsw || w->tsync.bytes < sizeof(unsigned int) * trw) {
+        // flags start below every publication stamp
+        w->top_sync.ensure(sizeof(unsigned int) * tsw);
+        w->tsync.ensure(sizeof(unsigned int) * trw);
+        k_fill_u32<<<(int)((tsw + 255) / 256), 256, 0, ctx->stream>>>(w->top_sync.as<unsigned int>(), (int)tsw, 0u);
+        k_fill_u32<<<(int)((trw + 255) / 256), 256, 0, ctx->stream>>>(w->tsync.as<unsigned int>(), (int)trw, 0u);
+        BH_HIP(hipGetLastError());
+        w->top_epoch = 0;
+        w->trsv_epoch = 0;
+    }
+    if (!w->ev[0])
+        for (hipEvent_t& v : w->ev) BH_HIP(hipEventCreate(&v));
+}
+}  // namespace
+
+void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
+                        double tol, int sweeps) {
+    BH_CHECK(cj.n > 0 && cj.cnt > 0 && (int)cj.v0.size() == cj.cnt, BEMHIP_ERR_ARG, "batched fp16 LU: bad class");
+    BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "batched fp16 LU needs the FP32 copy of G");
+    if (tol <= 0.0) tol = IR_TOL;
+    class_work_ensure(ctx, w, cj);
+    const ClassPtrs c = class_ptrs(ctx, w, cj, G32);
+    const int n = c.n, cnt = c.cnt, NB = c.NB;
+    hipEvent_t ev_ready = w->ev[0], ev_panel = w->ev[1];
+    BH_HIP(hipEventRecord(w->ev[2], c.main));
+    BH_HIP(hipMemcpyAsync(w->v0.p, cj.v0.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, c.main));
+    k_cls_prep<<<cnt, 256, 0, c.main>>>(n, c.A, c.sA, w->e.as<int>(), w->top_sync.as<unsigned int>(), TOP_SYNC_WORDS,
+                                        w->maxl64.as<unsigned long long>(), w->f16log.as<unsigned int>());
+    BH_HIP(hipGetLastError());
+    // factorisation: panel 0 on the main stream, then per step the next
+    // panel's columns, the next panel on the side stream while the main stream
+    // updates the rest (buffers of a panel's products alternate by parity)
+    int k = 0, jb = std::min(NB, n), p = 0;
+    cls_panel(c, c.main, 0, n, jb, 0, false);
+    BH_HIP(hipEventRecord(ev_panel, c.main));
+    bool a12_tiled = false;
+    for (;;) {
+        BH_HIP(hipStreamWaitEvent(c.main, ev_panel, 0));
+        const int rest = n - k - jb;
+        if (rest <= 0) break;
+        const int jb2 = std::min(NB, rest);
+        const bool a21 = rest > jb2;
+        const Mm16Tiles ta{w->t_a21.p, false, jb2, 0, p2(8), w->f16log.as<unsigned int>(), nullptr, jb2};
+        cls_cols(c, k, jb, rest, 0, jb2, p, a12_tiled, a21 ? &ta : nullptr, w->tb_a21);
+        BH_HIP(hipEventRecord(ev_ready, c.main));
+        if (c.side != c.main) BH_HIP(hipStreamWaitEvent(c.side, ev_ready, 0));
+        cls_panel(c, c.side, k + jb, rest, jb2, p ^ 1, a21);
+        BH_HIP(hipEventRecord(ev_panel, c.side));
+        if (rest > jb2) {
+            const Mm16Tiles tb{w->t_a12.p, true, jb2, 0, p2(8), w->f16log.as<unsigned int>(), nullptr};
+            cls_cols(c, k, jb, rest, jb2, rest - jb2, p, a12_tiled, &tb, w->tb_a12);
+            a12_tiled = true;
+        } else {
+            a12_tiled = false;
+        }
+        k += jb;
+        jb = jb2;
+        p ^= 1;
+    }
+    BH_HIP(hipEventRecord(w->ev[3], c.main));
+    // solves: FP64 inverses of the 128 x 128 diagonal blocks, x0 = (LU)^-1 b,
+    // then the refinement sweeps
+    const size_t sh = sizeof(double) * TS * TS;
+    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    const long long sInv = (long long)w->nbk * TS * TS;
+    k_diag_inv<float, true><<<dim3(w->nbk, cnt), TS, sh, c.main>>>(n, c.A, w->inv_lo.as<double>(), c.sA, sInv);
+    k_diag_inv<float, false><<<dim3(w->nbk, cnt), TS, sh, c.main>>>(n, c.A, w->inv_up.as<double>(), c.sA, sInv);
+    const int tss = CLS_TRSV_WORDS_MIN + w->nbk;
+    k_cls_words<<<(cnt + 255) / 256, 256, 0, c.main>>>(cnt, w->tsync.as<unsigned int>(), tss, 2, 0u, nullptr);
+    double* bb = cls_vec(w, 0);
+    double* xx = cls_vec(w, 1);
+    double* rr = cls_vec(w, 2);
+    k_cls_gather<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, b, w->v0.as<int>(), bb);
+    BH_HIP(hipGetLastError());
+    BH_HIP(hipMemcpyAsync(xx, bb, sizeof(double) * (size_t)cnt * n, hipMemcpyDeviceToDevice, c.main));
+    cls_trsv(c, xx, nullptr);
+    k_cls_ir_init<<<cnt, 256, 0, c.main>>>(n, bb, ir_state(w));
+    BH_HIP(hipGetLastError());
+    for (int s = 0; s < sweeps; ++s) cls_sweep(c, G64 + cj.gofs0, bb, xx, rr, tol);
+}
+
+void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
+                       double* x, double tol, int maxit, bemhip_solve_stats* st) {
+    if (tol <= 0.0) tol = IR_TOL;
+    if (maxit <= 0) maxit = 30;
+    const ClassPtrs c = class_ptrs(ctx, w, cj, G32);
+    const int n = c.n, cnt = c.cnt;
+    double* bb = cls_vec(w, 0);
+    double* xx = cls_vec(w, 1);
+    double* rr = cls_vec(w, 2);
+    const int tss = CLS_TRSV_WORDS_MIN + w->nbk;
+    std::vector<unsigned int> top((size_t)cnt * TOP_SYNC_WORDS), ts((size_t)cnt * tss), f16((size_t)cnt);
+    std::vector<unsigned long long> ml((size_t)cnt);
+    std::vector<char> sc(w->scal.bytes);
+    auto read_status = [&] {
+        BH_HIP(hipMemcpyAsync(sc.data(), w->scal.p, sc.size(), hipMemcpyDeviceToHost, c.main));
+        BH_HIP(hipMemcpyAsync(ts.data(), w->tsync.p, sizeof(unsigned int) * ts.size(), hipMemcpyDeviceToHost, c.main));
+        BH_HIP(hipStreamSynchronize(c.main));
+    };
+    BH_HIP(hipMemcpyAsync(top.data(), w->top_sync.p, sizeof(unsigned int) * top.size(), hipMemcpyDeviceToHost, c.main));
+    BH_HIP(hipMemcpyAsync(ml.data(), w->maxl64.p, sizeof(unsigned long long) * cnt, hipMemcpyDeviceToHost, c.main));
+    BH_HIP(hipMemcpyAsync(f16.data(), w->f16log.p, sizeof(unsigned int) * cnt, hipMemcpyDeviceToHost, c.main));
+    read_status();
+    float fms = 0.0f;
+    BH_HIP(hipEventElapsedTime(&fms, w->ev[2], w->ev[3]));
+    w->ms_factor = fms;
+    const double* sd = reinterpret_cast<const double*>(sc.data());
+    auto sint = [&](int which, int q) { return reinterpret_cast<const int*>(sd + 2 * (size_t)cnt)[(size_t)which * cnt + q]; };
+    // problems to solve again on their own: a multiplier above 1 (GEPP would
+    // have interchanged rows), an fp16 overflow, a top block that gave up waiting
+    std::vector<int> redo;
+    int sweeps_done = 0;
+    for (int q = 0; q < cnt; ++q) {
+        const unsigned int* tq = top.data() + (size_t)q * TOP_SYNC_WORDS;
+        unsigned long long tl = 0;
+        std::memcpy(&tl, tq, sizeof(tl));
+        double l1, l2;
+        std::memcpy(&l1, &tl, sizeof(l1));
+        std::memcpy(&l2, &ml[q], sizeof(l2));
+        float fm;
+        std::memcpy(&fm, &f16[q], sizeof(fm));
+        BH_CHECK(tq[3] == 0u, BEMHIP_ERR_HIP, "batched panel top-block factorisation gave up waiting");
+        if (!(l1 <= 1.0 && l2 <= 1.0 && fm <= 32768.0f) || q == ctx->batch_force_redo) redo.push_back(q);
+        sweeps_done = std::max(sweeps_done, sint(2, q));
+    }
+    for (int q = 0; q < cnt; ++q)
+        BH_CHECK(ts[(size_t)q * tss + 2] == 0u, BEMHIP_ERR_HIP,
+                 "persistent triangular solve gave up waiting for a block (result invalid)");
+    // more sweeps while a problem (not solved again) still refines
+    int extra = 0;
+    for (;;) {
+        bool any = false;
+        for (int q = 0; q < cnt; ++q)
+            if (sint(0, q) && std::find(redo.begin(), redo.end(), q) == redo.end()) any = true;
+        if (!any || sweeps_done + extra >= maxit) break;
+        cls_sweep(c, G64 + cj.gofs0, bb, xx, rr, tol);
+        ++extra;
+        read_status();
+        for (int q = 0; q < cnt; ++q)
+            BH_CHECK(ts[(size_t)q * tss + 2] == 0u, BEMHIP_ERR_HIP,
+                     "persistent triangular solve gave up waiting for a block (result invalid)");
+    }
+    k_cls_scatter<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, xx, w->v0.as<int>(), x);
+    BH_HIP(hipGetLastError());
+    bemhip_solve_stats sum{};
+    sum.converged = 1;
+    for (int q = 0; q < cnt; ++q) {
+        if (std::find(redo.begin(), redo.end(), q) != redo.end()) continue;
+        sum.iterations = std::max(sum.iterations, sint(2, q));
+        sum.converged = sum.converged && sint(1, q);
+        sum.correction = std::max(sum.correction, sd[cnt + q]);
+    }
+    sum.panels = cnt * ((n + c.NB - 1) / c.NB);
+    for (int q : redo) {
+        const size_t off = (size_t)cj.gofs0 + (size_t)q * c.sA;
+        k_d2f_mat<<<4096, 256, 0, c.main>>>((size_t)n * n, G64 + off, G32 + off);
+        BH_HIP(hipGetLastError());
+        bemhip_solve_stats one{};
+        solve_device(ctx, n, G64 + off, G32 + off, b + cj.v0[q], x + cj.v0[q], BEMHIP_SOLVE_LU16_IR64, tol, maxit,
+                     &one, false);
+        sum.iterations = std::max(sum.iterations, one.iterations);
+        sum.converged = sum.converged && one.converged;
+        sum.correction = std::max(sum.correction, one.correction);
+        sum.panels += one.panels;
+        sum.panels_pivoted += one.panels_pivoted;
+        sum.ms_factor += one.ms_factor;
+    }
+    sum.ms_factor += fms;
+    *st = sum;
+}
+
 }  // namespace bemhip
 
 namespace bemhip {
@@ -2795,6 +3359,7 @@ void Solver::init(const Solver* cfg) {
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split; nb16 = cfg->nb16;
         trsv_persistent = cfg->trsv_persistent;
         trsv_force_giveup = cfg->trsv_force_giveup;
+        batch_force_redo = cfg->batch_force_redo;
         mm_reserve = cfg->mm_reserve;
         timing = cfg->timing;
         split_fixed = cfg->split_fixed;

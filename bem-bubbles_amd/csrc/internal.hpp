@@ -84,6 +84,9 @@ struct AsmProb {
     int32_t p0, p1;   // its patches
     int32_t pad;
 };
+// elements between the G blocks of two problems of n unknowns laid out one
+// after the other (256-B aligned in FP32 and FP64)
+inline int64_t gblock_stride(int n) { return ((int64_t)n * n + 63) / 64 * 64; }
 // One assembly work-group: 128 rows of a problem x a range of its patches
 struct AsmItem {
     int32_t prob, row0, p0, p1, g, pad0, pad1, pad2;
@@ -138,6 +141,8 @@ struct SmallJob {
     int32_t n, v0;    // unknowns, first vertex (offset into b and x)
 };
 
+struct ClassWork;
+
 struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
     // per outer step: inverse of the panel's unit-lower diagonal block and its
@@ -182,6 +187,9 @@ struct Solver {
     // triangular solve's error word, so every waiting work-group gives up at
     // once and the solve must report BEMHIP_ERR_HIP
     bool trsv_force_giveup = false;
+    // test hook (BEMHIP_BATCH_FORCE_REDO=q): problem q of every size class of a
+    // batched fp16 LU is solved again on its own as if its check had failed
+    int batch_force_redo = -1;
     // fp16 LU: the next panel's A21 tiles were written by the update that
     // produced A21 (set by the main thread before it hands the panel over)
     bool lu16_a21_ready = false;
@@ -247,6 +255,13 @@ struct bemhip_ctx : bemhip::Solver {
     hipEvent_t small_ev[2] = {};   // start, end of the launch
     std::vector<bemhip::SmallJob> small_jobs_h;
     bemhip::DevBuf small_jobs, small_info;
+    // batches: problems of more than small_n unknowns solved by the lu16_ir64
+    // solver are factored in lock step per size class (BEMHIP_BATCH_LU=0: on
+    // the solver lanes instead); refinement sweeps enqueued before the first
+    // host read-back of a class
+    bool batch_lu = true;
+    int batch_sweeps = 2;
+    std::vector<std::unique_ptr<bemhip::ClassWork>> class_work;
 };
 
 namespace bemhip {
@@ -274,8 +289,23 @@ inline size_t mm_tile_bytes(int R, int K) { return (size_t)mm_pad(R, MM_TILE) * 
 // (A operand); copy (optional, B only, ld ldc) receives X at the same positions;
 // maxlog (optional) the max |scale x| as float bits, amax64 (optional, A only)
 // max |x| as the bits of a non-negative double
+// A batch of equally shaped products (or packs): problem b's operands start at
+// the given pointer plus b times the stride of that operand, and its scales
+// carry the factor 2^(ek e[b]) (e: per-problem exponents on the device; null
+// for none).  The logs (maxlog, amax64) are per problem, ls elements apart.
+struct Mm16Bat {
+    int n = 1;                    // problems
+    long long sx = 0;             // pack: source X / gemm: C, in floats
+    long long sa = 0, sb = 0;     // pack: output tiles (sa) / gemm: A and B tiles, in bytes
+    long long st = 0;             // gemm: output tiles (Mm16Tiles::out), in bytes
+    const int* e = nullptr;
+    int ek = 0;                   // pack: scale / gemm: alpha
+    int ek2 = 0;                  // gemm: the output tiles' scale
+    int ls = 0;
+};
 void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long ld, float scale, void* tiles,
-               unsigned int* maxlog, float* copy = nullptr, long ldc = 0, unsigned long long* amax64 = nullptr);
+               unsigned int* maxlog, float* copy = nullptr, long ldc = 0, unsigned long long* amax64 = nullptr,
+               const Mm16Bat* bat = nullptr);
 // C (M x N, ld ldc) -= alpha A B (MM_SUB) or = alpha A B (MM_STORE) over the first
 // K of the k extent; A: tiles of M x Ka, B: tiles of Kb x (c0 + N) read from
 // column c0 (a multiple of MM_TILE); tri 1: A lower, 2: B upper triangular;
@@ -293,7 +323,8 @@ struct Mm16Tiles {
     int r0 = 0;                  // A only: C's rows from r0 on (a multiple of 256) are the buffer's rows
 };
 void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
-               int K, float* C, long ldc, float alpha, int max_wg = 0, const Mm16Tiles* tiles = nullptr);
+               int K, float* C, long ldc, float alpha, int max_wg = 0, const Mm16Tiles* tiles = nullptr,
+               const Mm16Bat* bat = nullptr);
 
 // ---------------------------------------------------------- dense LA --
 // LU with partial pivoting (column-major, lda = N), pivots as absolute rows.
@@ -311,8 +342,15 @@ void exterior_scratch_init(bemhip_ctx* ctx);
 // triangle of A (ld lda), each with its other triangle written as zeros, in
 // the same launches (a null W skips its triangle); tmp_l and tmp_u hold
 // tri_inv_tmp_elems(n) elements each (trinv.hip)
+// A batch of problems for tri_inv2: problem q's A, W and tmp start q sA, q sW,
+// q sT elements further
+struct TriBat {
+    int n;
+    int64_t sA, sW, sT;
+};
 template <class T>
-void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l, T* tmp_u);
+void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l, T* tmp_u,
+              const TriBat* bat = nullptr);
 size_t tri_inv_tmp_elems(int n);
 template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split16 = false);
 // x (double, device) <- (LU)^-1 P x with LU in precision T
@@ -329,6 +367,48 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
 // solvers, whose transposed solve is refined against it)
 void solve_transposed_device(Solver* ctx, int N, const double* G64, const float* G32, int solver,
                              const double* b, double* x, double tol, int maxit, bemhip_solve_stats* st);
+// ---------------------------------------- batched fp16 LU (size classes) --
+// A size class of a batch: cnt independent problems of n unknowns whose G
+// blocks (FP64 and the FP32 copy) lie gblock_stride(n) elements apart from
+// element offset gofs0; problem q's right-hand side and solution start at
+// v0[q] of b and x.
+struct ClassJob {
+    int n = 0, cnt = 0;
+    int64_t gofs0 = 0;
+    std::vector<int32_t> v0;
+};
+// Device workspace of one class (grown on demand, kept between stages).
+struct ClassWork {
+    int n = 0, cnt = 0, nb = 0, nbk = 0, nch = 0;
+    size_t tb_a21 = 0, tb_w = 0, tb_a12 = 0;   // bytes of a problem's A21 / NB x NB / block-row tiles
+    DevBuf e, top_sync, tickets, maxl64, f16log, linv, winv_l[2], winv_u, tinv, t_a21, t_l21[2], t_wu, t_wl[2], t_a12,
+        t_u12;
+    DevBuf inv_lo, inv_up, tsync, vecs, part, scal, v0;
+    unsigned int top_epoch = 0, trsv_epoch = 0;
+    hipEvent_t ev[4] = {};   // next panel's columns ready, panel done, solve start, end
+    float ms_factor = 0.0f;
+    ClassWork() = default;
+    ClassWork(const ClassWork&) = delete;
+    ClassWork& operator=(const ClassWork&) = delete;
+    ~ClassWork();
+};
+// Enqueues, on the solver's two streams and without any host
+// synchronisation, the lock-step fp16 LU of every problem of the class (one
+// launch per block product over all of them), the FP32 triangular solves and
+// `sweeps` FP64 refinement sweeps, each of which skips the problems whose
+// refinement has already stopped.  Panels are factored without row
+// interchanges; a problem whose multipliers exceed 1 in magnitude (partial
+// pivoting would have interchanged rows) or whose fp16 operands overflowed is
+// flagged for class_lu16_finish.
+void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
+                        double tol, int sweeps);
+// Reads the class's status back, runs further sweeps for problems still
+// refining (at most maxit in all), writes the solutions to x and solves every
+// flagged problem again on its own with the single-problem solver (partial
+// pivoting or FP32 products from a fresh FP32 copy of G).
+void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
+                       double* x, double tol, int maxit, bemhip_solve_stats* st);
+
 // small problems of a batch: one work-group each, FP64 GEPP + solve in place
 // on the problem's G block (destroyed), all in one launch; info[q] = LAPACK info
 void small_solve_launch(hipStream_t st, const SmallJob* d_jobs, int count, double* G64, const double* b, double* x,

@@ -66,13 +66,29 @@ __device__ inline void pack_logs(float m, double am, unsigned int* maxlog, unsig
     }
 }
 
+// per-problem offsets of a batched pack (blockIdx.z = problem): source and
+// tiles strides (floats, h8), log stride, scale exponents 2^(ek e[b])
+struct PackBat {
+    long long sx, so;
+    int ls, ek;
+    const int* e;
+};
+
 // A operand: X is R x K column-major (element (r, k) = X[r + k ld]).  A
 // work-group packs 256 rows x 16 k (8 fragment blocks): grid (RP / 256, KB).
 __global__ __launch_bounds__(256) void k_pack_a16(int R, int K, int KB, const float* __restrict__ X, long ld,
                                                   float scale, h8* __restrict__ out, unsigned int* __restrict__ maxlog,
-                                                  unsigned long long* __restrict__ amax64) {
+                                                  unsigned long long* __restrict__ amax64, PackBat pb) {
     __shared__ float s[16][256];
     const int r0 = blockIdx.x * 256, kb = blockIdx.y, k0 = kb * 16, t = threadIdx.x;
+    {
+        const int b = blockIdx.z;   // problem of a batch
+        X += b * pb.sx;
+        out += b * pb.so;
+        if (maxlog) maxlog += b * pb.ls;
+        if (amax64) amax64 += b * pb.ls;
+        if (pb.e) scale = ldexpf(scale, pb.ek * pb.e[b]);
+    }
     const int r = r0 + t;
 #pragma unroll
     for (int k = 0; k < 16; ++k) s[k][t] = (r < R && k0 + k < K) ? X[(size_t)(k0 + k) * ld + r] : 0.0f;
@@ -107,9 +123,16 @@ __global__ __launch_bounds__(256) void k_pack_a16(int R, int K, int KB, const fl
 constexpr int PB_K = 128;
 __global__ __launch_bounds__(256) void k_pack_b16(int C, int K, int KB, const float* __restrict__ X, long ld,
                                                   float scale, h8* __restrict__ out, unsigned int* __restrict__ maxlog,
-                                                  float* __restrict__ copy, long ldc) {
+                                                  float* __restrict__ copy, long ldc, PackBat pb) {
     __shared__ float s[32][PB_K + 4];
     const int c0 = blockIdx.y * 32, k0 = blockIdx.x * PB_K, t = threadIdx.x;
+    {
+        const int b = blockIdx.z;   // problem of a batch (no copy in a batch)
+        X += b * pb.sx;
+        out += b * pb.so;
+        if (maxlog) maxlog += b * pb.ls;
+        if (pb.e) scale = ldexpf(scale, pb.ek * pb.e[b]);
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int idx = t + 256 * j, c = idx >> 5, q = idx & 31;
@@ -156,18 +179,25 @@ __global__ __launch_bounds__(256) void k_pack_b16(int C, int K, int KB, const fl
 }
 
 void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long ld, float scale, void* tiles,
-               unsigned int* maxlog, float* copy, long ldc, unsigned long long* amax64) {
+               unsigned int* maxlog, float* copy, long ldc, unsigned long long* amax64, const Mm16Bat* bat) {
     if (R <= 0 || K <= 0) return;
     const int RP = mm_pad(R, MM_TILE), KB = mm_pad(K, MM_BK) / 16;
+    PackBat pb{0, 0, 0, 0, nullptr};
+    int nb = 1;
+    if (bat) {
+        BH_CHECK(bat->n >= 1 && bat->n <= 65535 && bat->sa % 16 == 0 && copy == nullptr, BEMHIP_ERR_ARG,
+                 "mm16_pack: bad batch");
+        pb = PackBat{bat->sx, bat->sa / 16, bat->ls, bat->ek, bat->e};
+        nb = bat->n;
+    }
     if (kcontig) {
         BH_CHECK(amax64 == nullptr, BEMHIP_ERR_ARG, "mm16_pack: no max |x| for B operands");
-        k_pack_b16<<<dim3((KB * 16 + PB_K - 1) / PB_K, RP / 32), 256, 0, st>>>(R, K, KB, X, ld, scale,
-                                                                              static_cast<h8*>(tiles), maxlog, copy,
-                                                                              ldc);
+        k_pack_b16<<<dim3((KB * 16 + PB_K - 1) / PB_K, RP / 32, nb), 256, 0, st>>>(
+            R, K, KB, X, ld, scale, static_cast<h8*>(tiles), maxlog, copy, ldc, pb);
     } else {
         BH_CHECK(copy == nullptr, BEMHIP_ERR_ARG, "mm16_pack: no copy for A operands");
-        k_pack_a16<<<dim3(RP / 256, KB), 256, 0, st>>>(R, K, KB, X, ld, scale, static_cast<h8*>(tiles), maxlog,
-                                                        amax64);
+        k_pack_a16<<<dim3(RP / 256, KB, nb), 256, 0, st>>>(R, K, KB, X, ld, scale, static_cast<h8*>(tiles), maxlog,
+                                                            amax64, pb);
     }
     BH_HIP(hipGetLastError());
 }
@@ -205,15 +235,18 @@ struct MmArgs {
     float tscale;
     unsigned int* maxlog;
     unsigned long long* amax64;
+    // a batch of nbat equally shaped products: problem b's A, B and output
+    // tiles start b * sAt, sBt, sT h8 further, its C b * sC floats further;
+    // alpha and tscale carry 2^(ek e[b]), 2^(ek2 e[b]); logs ls apart
+    int nbat;
+    long long sAt, sBt, sC, sT;
+    const int* ebat;
+    int ek, ek2, ls;
 };
 
 template <int EPI, int TRI>
 __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
-    const h8* __restrict__ At = g.At;
-    const h8* __restrict__ Bt = g.Bt;
-    float* __restrict__ C = g.C;
     const long ldc = g.ldc;
-    const float alpha = g.alpha;
     const int KBa = g.KBa, KBb = g.KBb, cb0 = g.cb0, M = g.M, N = g.N, nks = g.nks;
     const int tiles_m = g.tiles_m, tiles_n = g.tiles_n, per_xcd = g.per_xcd, wg_xcd = g.wg_xcd;
     // four stages of [A 16 KB | B 16 KB], 32 k each: three stages' loads are
@@ -226,9 +259,18 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const int xcd = blockIdx.x & 7;
     float tmax = 0.0f;    // max |tscale x| of the fused tiles
     double tamax = 0.0;   // max |x|
+    const int tpp = tiles_m * tiles_n;   // tiles per problem
     for (int t = blockIdx.x >> 3; t < per_xcd; t += wg_xcd) {
-    const int L = xcd * per_xcd + t;
-    if (L >= tiles_m * tiles_n) break;
+    int L = xcd * per_xcd + t;
+    if (L >= tpp * g.nbat) break;
+    const int bq = L / tpp;   // problem of a batch
+    L -= bq * tpp;
+    const h8* __restrict__ At = g.At + bq * g.sAt;
+    const h8* __restrict__ Bt = g.Bt + bq * g.sBt;
+    float* __restrict__ C = g.C + bq * g.sC;
+    h8* __restrict__ tout = g.tout ? g.tout + bq * g.sT : nullptr;
+    const float alpha = g.ebat ? ldexpf(g.alpha, g.ek * g.ebat[bq]) : g.alpha;
+    const float tscale = g.ebat ? ldexpf(g.tscale, g.ek2 * g.ebat[bq]) : g.tscale;
     constexpr int GM = 8;
     const int fm = (L / (GM * tiles_n)) * GM;
     const int gs = min(tiles_m - fm, GM);
@@ -369,14 +411,14 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const float x = alpha * ep[(kl + e) * CP + rl];
-                        const float y = g.tscale * x;
+                        const float y = tscale * x;
                         const float a = fabsf(y);
                         tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
                         tamax = fmax(tamax, fabs((double)x));
                         if (x != x) tamax = __longlong_as_double(0x7ff8000000000000ll);
                         v[e] = (_Float16)y;
                     }
-                    g.tout[((size_t)((row0 + ib * 32) / 32) * g.tKB + kb) * 64 + l] = v;
+                    tout[((size_t)((row0 + ib * 32) / 32) * g.tKB + kb) * 64 + l] = v;
                 }
         } else if (EPI == MM_STORE && g.tkind == 2) {
             // B tiles of C: this chunk is one column block x 8 k blocks
@@ -392,12 +434,12 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 h8 v;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const float y = g.tscale * (alpha * xs[e]);
+                    const float y = tscale * (alpha * xs[e]);
                     const float a = fabsf(y);
                     tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
                     v[e] = (_Float16)y;
                 }
-                g.tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
+                tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
             }
         }
         // subtraction with tiles: each lane puts its final values back into
@@ -458,12 +500,12 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                     h8 v;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
-                        const float y = g.tscale * ep[(kl + e) * CP + rl];
+                        const float y = tscale * ep[(kl + e) * CP + rl];
                         const float a = fabsf(y);
                         tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
                         v[e] = (_Float16)y;
                     }
-                    g.tout[((size_t)(rb / 32) * g.tKB + kb) * 64 + l] = v;
+                    tout[((size_t)(rb / 32) * g.tKB + kb) * 64 + l] = v;
                 }
             }
         } else if (sub_tiles && g.tkind == 2) {
@@ -480,25 +522,32 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 h8 v;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const float y = g.tscale * xs[e];
+                    const float y = tscale * xs[e];
                     const float a = fabsf(y);
                     tmax = (a <= 3.0e38f) ? fmaxf(tmax, a) : __builtin_inff();
                     v[e] = (_Float16)y;
                 }
-                g.tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
+                tout[((size_t)cb * g.tKB + kb) * 64 + l] = v;
             }
         }
     }
     __syncthreads();   // the epilogue's LDS is the next tile's staging buffer
+    if (g.tkind && g.nbat > 1) {
+        // a batch's logs are per problem: merged after every tile
+        pack_logs(tmax, tamax, g.maxlog ? g.maxlog + bq * g.ls : nullptr,
+                  EPI == MM_STORE && g.tkind == 1 && g.amax64 ? g.amax64 + bq * g.ls : nullptr);
+        tmax = 0.0f;
+        tamax = 0.0;
     }
-    if (g.tkind) pack_logs(tmax, tamax, g.maxlog, EPI == MM_STORE && g.tkind == 1 ? g.amax64 : nullptr);
+    }
+    if (g.tkind && g.nbat == 1) pack_logs(tmax, tamax, g.maxlog, EPI == MM_STORE && g.tkind == 1 ? g.amax64 : nullptr);
 }
 
 // max_wg > 0: at most that many work-groups (one per compute unit), each
 // looping over tiles, so that the remaining compute units stay free for
 // kernels of another stream
 void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
-               int K, float* C, long ldc, float alpha, int max_wg, const Mm16Tiles* tiles) {
+               int K, float* C, long ldc, float alpha, int max_wg, const Mm16Tiles* tiles, const Mm16Bat* bat) {
     if (M <= 0 || N <= 0) return;
     BH_CHECK(c0 % MM_TILE == 0, BEMHIP_ERR_ARG, "mm16: column offset must be a multiple of 256");
     BH_CHECK(K <= Ka && K <= Kb, BEMHIP_ERR_ARG, "mm16: K exceeds the operand tiles");
@@ -519,7 +568,21 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
     g.nks = mm_pad(K, MM_BK) / MM_BK;
     g.tiles_m = (M + MM_TILE - 1) / MM_TILE;
     g.tiles_n = (N + MM_TILE - 1) / MM_TILE;
-    g.per_xcd = (g.tiles_m * g.tiles_n + 7) / 8;
+    g.nbat = 1;
+    if (bat) {
+        BH_CHECK(bat->n >= 1 && bat->sa % 16 == 0 && bat->sb % 16 == 0 && bat->st % 16 == 0, BEMHIP_ERR_ARG,
+                 "mm16: bad batch");
+        g.nbat = bat->n;
+        g.sAt = bat->sa / 16;
+        g.sBt = bat->sb / 16;
+        g.sC = bat->sx;
+        g.sT = bat->st / 16;
+        g.ebat = bat->e;
+        g.ek = bat->ek;
+        g.ek2 = bat->ek2;
+        g.ls = bat->ls;
+    }
+    g.per_xcd = (int)(((long long)g.tiles_m * g.tiles_n * g.nbat + 7) / 8);
     g.wg_xcd = max_wg > 0 ? std::max(1, std::min(g.per_xcd, max_wg / 8)) : g.per_xcd;
     if (tiles) {
         g.tout = static_cast<h8*>(tiles->out);

@@ -212,16 +212,25 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
         for (int64_t i = 1; i < N; ++i)
             BH_CHECK(T.vpart[i] >= T.vpart[i - 1], BEMHIP_ERR_BAD_MESH,
                      "independent parts must be contiguous vertex ranges in increasing order");
-        int64_t gofs = 0;
         for (int c = 0; c < T.nparts; ++c) {
             AsmProb pr{};
             pr.v0 = (int32_t)first[c];
             pr.n = (int32_t)count[c];
-            pr.gofs = gofs;
             pr.p0 = T.part_pptr[c];
             pr.p1 = T.part_pptr[c + 1];
-            gofs += ((int64_t)pr.n * pr.n + 63) / 64 * 64;   // 256-B aligned blocks in FP32 too
             T.probs.push_back(pr);
+        }
+        // G blocks grouped by size (largest first, then part order), each
+        // 256-B aligned in FP32 too: the problems of one size then sit at a
+        // uniform stride, so that a batched factorisation addresses problem b
+        // of a size class as first + b * stride
+        std::vector<int> ord(T.nparts);
+        for (int c = 0; c < T.nparts; ++c) ord[c] = c;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return T.probs[a].n > T.probs[b].n; });
+        int64_t gofs = 0;
+        for (int c : ord) {
+            T.probs[c].gofs = gofs;
+            gofs += gblock_stride(T.probs[c].n);
         }
     } else {
         AsmProb pr{};

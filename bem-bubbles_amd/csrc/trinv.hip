@@ -36,14 +36,17 @@ constexpr int TI_B = 32;   // diagonal block of the first launch
 // null W skips that triangle).
 constexpr int TI_ZWG = 16;   // work-groups a triangle that write its zero triangle
 
+// grid.z: problem of a batch (A and W sA, sW elements further each)
 template <class T>
 __global__ __launch_bounds__(256) void k_trinv_diag(int n, const T* __restrict__ A, int lda, T* __restrict__ Wl,
-                                                    T* __restrict__ Wu) {
+                                                    T* __restrict__ Wu, long long sA, long long sW) {
     __shared__ T s[8][TI_B][TI_B + 1];   // row-major copy of each block: s[b][i][k] = A(i0 + i, i0 + k)
     __shared__ T rb[2][8][TI_B];         // broadcast row of each block, by step parity
     const bool lower = blockIdx.y == 0;
     T* __restrict__ W = lower ? Wl : Wu;
     if (!W) return;   // the same for the whole work-group
+    A += blockIdx.z * sA;
+    W += blockIdx.z * sW;
     const int nbk = (n + TI_B - 1) / TI_B;
     const int ndiag = (nbk + 7) / 8;
     if ((int)blockIdx.x >= ndiag) {
@@ -124,6 +127,8 @@ template <class T> struct TiTask {
     int64_t sa, sb, sc;
     int M, N, K, Ml, Nl, Kl, pairs, skip;
     T alpha;
+    // a batch of problems: the pairs of problem q start qa, qb, qc elements further
+    int64_t qa, qb, qc;
 };
 
 constexpr int TI_KS = 32;   // K slice of the merge products
@@ -135,18 +140,20 @@ constexpr int TI_KS = 32;   // K slice of the merge products
 // slice's products.  At the largest level (s = n / 2) the products are
 // bound by the vector FP32 rate of the compute units the concurrent trailing
 // update leaves free.
+// grid.z: nbat problems x (task 0's pairs, then task 1's)
 template <class T>
 __global__ __launch_bounds__(256) void k_trinv_gemm(TiTask<T> t0, TiTask<T> t1) {
-    const bool second = (int)blockIdx.z >= t0.pairs;
+    const int zz = (int)blockIdx.z % (t0.pairs + t1.pairs), q = (int)blockIdx.z / (t0.pairs + t1.pairs);
+    const bool second = zz >= t0.pairs;
     const TiTask<T>& t = second ? t1 : t0;
-    const int z = second ? (int)blockIdx.z - t0.pairs : (int)blockIdx.z;
+    const int z = second ? zz - t0.pairs : zz;
     const bool last = z == t.pairs - 1;
     const int m = last ? t.Ml : t.M, n = last ? t.Nl : t.N, k = last ? t.Kl : t.K;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
     if (m0 >= m || n0 >= n) return;   // the same for the whole work-group
-    const T* __restrict__ A = t.a + (int64_t)z * t.sa;
-    const T* __restrict__ B = t.b + (int64_t)z * t.sb;
-    T* __restrict__ C = t.c + (int64_t)z * t.sc;
+    const T* __restrict__ A = t.a + (int64_t)z * t.sa + q * t.qa;
+    const T* __restrict__ B = t.b + (int64_t)z * t.sb + q * t.qb;
+    T* __restrict__ C = t.c + (int64_t)z * t.sc + q * t.qc;
     int kb = 0, ke = k;
     if (t.skip == TI_B_LOWER) kb = n0;
     else if (t.skip == TI_A_UPPER) kb = m0;
@@ -216,10 +223,14 @@ size_t tri_inv_tmp_elems(int n) {
 }
 
 template <class T>
-void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l, T* tmp_u) {
+void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l, T* tmp_u, const TriBat* bat) {
     if (n <= 0 || (!Wl && !Wu)) return;
+    const TriBat one{1, 0, 0, 0};
+    if (!bat) bat = &one;
+    BH_CHECK(bat->n >= 1 && bat->n <= 1024, BEMHIP_ERR_ARG, "tri_inv2: bad batch");
+    const int64_t qA = bat->sA, qW = bat->sW, qT = bat->sT;
     const int nbk = (n + TI_B - 1) / TI_B;
-    k_trinv_diag<T><<<dim3((nbk + 7) / 8 + TI_ZWG, 2), 256, 0, st>>>(n, A, lda, Wl, Wu);
+    k_trinv_diag<T><<<dim3((nbk + 7) / 8 + TI_ZWG, 2, bat->n), 256, 0, st>>>(n, A, lda, Wl, Wu, qA, qW);
     BH_HIP(hipGetLastError());
     const TiTask<T> none{};
     for (int s = TI_B; s < n; s *= 2) {
@@ -230,23 +241,25 @@ void tri_inv2(hipStream_t st, int n, const T* A, int lda, T* Wl, T* Wu, T* tmp_l
         const int64_t sa = 2 * (int64_t)s * (lda + 1), sw = 2 * (int64_t)s * (n + 1), st2 = (int64_t)s * s;
         // lower: tmp = C X_AA (sb x s) with C = A(b0.., a0..), then
         // X_BA = -X_BB tmp with X_BB = W(b0.., b0..)
-        const TiTask<T> l1{A + s, Wl, tmp_l, lda, n, s, sa, sw, st2, s, s, s, sb, s, s, pairs, TI_B_LOWER, T(1)};
+        const TiTask<T> l1{A + s, Wl, tmp_l, lda, n, s, sa, sw, st2, s, s, s, sb, s, s, pairs, TI_B_LOWER, T(1),
+                           qA, qW, qT};
         const TiTask<T> l2{Wl + (size_t)s * n + s, tmp_l, Wl + s, n, s, n, sw, st2, sw,
-                           s, s, s, sb, s, sb, pairs, TI_A_LOWER, T(-1)};
+                           s, s, s, sb, s, sb, pairs, TI_A_LOWER, T(-1), qW, qT, qW};
         // upper: tmp = C X_BB (s x sb) with C = A(a0.., b0..), then
         // X_AB = -X_AA tmp
         const TiTask<T> u1{A + (size_t)s * lda, Wu + (size_t)s * n + s, tmp_u, lda, n, s, sa, sw, st2,
-                           s, s, s, s, sb, sb, pairs, TI_B_UPPER, T(1)};
+                           s, s, s, s, sb, sb, pairs, TI_B_UPPER, T(1), qA, qW, qT};
         const TiTask<T> u2{Wu, tmp_u, Wu + (size_t)s * n, n, s, n, sw, st2, sw,
-                           s, s, s, s, sb, s, pairs, TI_A_UPPER, T(-1)};
-        const int z = (Wl ? pairs : 0) + (Wu ? pairs : 0);
+                           s, s, s, s, sb, s, pairs, TI_A_UPPER, T(-1), qW, qT, qW};
+        const int z = ((Wl ? pairs : 0) + (Wu ? pairs : 0)) * bat->n;
         const dim3 grid((s + 63) / 64, (s + 63) / 64, z);
         k_trinv_gemm<T><<<grid, 256, 0, st>>>(Wl ? l1 : u1, Wl && Wu ? u1 : none);
         k_trinv_gemm<T><<<grid, 256, 0, st>>>(Wl ? l2 : u2, Wl && Wu ? u2 : none);
         BH_HIP(hipGetLastError());
     }
 }
-template void tri_inv2<float>(hipStream_t, int, const float*, int, float*, float*, float*, float*);
-template void tri_inv2<double>(hipStream_t, int, const double*, int, double*, double*, double*, double*);
+template void tri_inv2<float>(hipStream_t, int, const float*, int, float*, float*, float*, float*, const TriBat*);
+template void tri_inv2<double>(hipStream_t, int, const double*, int, double*, double*, double*, double*,
+                               const TriBat*);
 
 }  // namespace bemhip
