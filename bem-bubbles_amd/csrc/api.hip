@@ -20,6 +20,9 @@
 using namespace bemhip;
 
 namespace bemhip {
+
+std::atomic<unsigned long long> g_devbuf_allocs{0};
+
 __global__ void k_d2f(size_t n, const double* __restrict__ a, float* __restrict__ b) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = (float)a[i];
@@ -105,43 +108,84 @@ void merge_stats(bemhip_solve_stats& sum, const bemhip_solve_stats& one) {
     sum.panels_pivoted += one.panels_pivoted;
 }
 
-// the small problems' one launch on stream ls (job table uploaded there)
-static void small_launch(bemhip_ctx* ctx, hipStream_t ls, double* G64, const double* d_hphi, double* d_psi) {
+// the small problems' job table on the device (uploaded when it changed)
+static SmallJob* small_upload(bemhip_ctx* ctx, hipStream_t ls) {
     const int nsmall = (int)ctx->small_jobs_h.size();
     SmallJob* dj = static_cast<SmallJob*>(ctx->small_jobs.ensure(sizeof(SmallJob) * nsmall));
-    int* dinfo = static_cast<int*>(ctx->small_info.ensure(sizeof(int) * nsmall));
-    // the host table stays alive in the context until the next call, which
-    // comes after this one's results were read back
-    BH_HIP(hipMemcpyAsync(dj, ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall, hipMemcpyHostToDevice, ls));
-    BH_HIP(hipEventRecord(ctx->small_ev[0], ls));
-    small_solve_launch(ls, dj, nsmall, G64, d_hphi, d_psi, dinfo);
-    BH_HIP(hipEventRecord(ctx->small_ev[1], ls));
+    ctx->small_info.ensure(sizeof(int) * nsmall);
+    const bool same = ctx->small_jobs_up.size() == ctx->small_jobs_h.size() &&
+                      std::memcmp(ctx->small_jobs_up.data(), ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall) == 0;
+    if (!same || ctx->small_jobs_dev != dj) {
+        BH_HIP(hipMemcpyAsync(dj, ctx->small_jobs_h.data(), sizeof(SmallJob) * nsmall, hipMemcpyHostToDevice, ls));
+        BH_HIP(hipStreamSynchronize(ls));
+        ctx->small_jobs_up = ctx->small_jobs_h;
+        ctx->small_jobs_dev = dj;
+    }
+    return dj;
+}
+
+// the small problems' one launch on stream ls (events: its timing)
+static void small_launch(bemhip_ctx* ctx, hipStream_t ls, double* G64, const double* d_hphi, double* d_psi,
+                         bool events = true) {
+    const int nsmall = (int)ctx->small_jobs_h.size();
+    SmallJob* dj = small_upload(ctx, ls);
+    if (events) BH_HIP(hipEventRecord(ctx->small_ev[0], ls));
+    small_solve_launch(ls, dj, nsmall, G64, d_hphi, d_psi, ctx->small_info.as<int>());
+    if (events) BH_HIP(hipEventRecord(ctx->small_ev[1], ls));
 }
 
 // the small problems' info words and launch time
-static void small_finish(bemhip_ctx* ctx, hipStream_t ls, bemhip_solve_stats& sum) {
+static void small_finish(bemhip_ctx* ctx, hipStream_t ls, bemhip_solve_stats& sum, bool events = true) {
     const int nsmall = (int)ctx->small_jobs_h.size();
     std::vector<int> info(nsmall);
     BH_HIP(hipMemcpyAsync(info.data(), ctx->small_info.p, sizeof(int) * nsmall, hipMemcpyDeviceToHost, ls));
     BH_HIP(hipStreamSynchronize(ls));
-    float ms = 0.0f;
-    BH_HIP(hipEventElapsedTime(&ms, ctx->small_ev[0], ctx->small_ev[1]));
-    sum.ms_factor += ms;
+    if (events) {
+        float ms = 0.0f;
+        BH_HIP(hipEventElapsedTime(&ms, ctx->small_ev[0], ctx->small_ev[1]));
+        sum.ms_factor += ms;
+    }
     for (int q = 0; q < nsmall; ++q)
         BH_CHECK(info[q] == 0, BEMHIP_ERR_SINGULAR,
                  "zero pivot in column " + std::to_string(info[q]) + " of a batch problem with " +
                      std::to_string(ctx->small_jobs_h[q].n) + " unknowns");
 }
 
+// small problems (LU solvers, at most small_n unknowns) into ctx->small_jobs_h,
+// the others' indices into big
+static void split_small(bemhip_ctx* ctx, const DevTopology& topo, int solver, std::vector<int>& big) {
+    big.clear();
+    ctx->small_jobs_h.clear();
+    for (int q = 0; q < (int)topo.probs.size(); ++q) {
+        const AsmProb& pr = topo.probs[q];
+        if (solver != BEMHIP_SOLVE_BICGSTAB64 && pr.n <= std::min(ctx->small_n, SMALL_MAX_N))
+            ctx->small_jobs_h.push_back(SmallJob{pr.gofs, pr.n, pr.v0});
+        else
+            big.push_back(q);
+    }
+    if (!ctx->small_jobs_h.empty() && !ctx->small_ev[0])
+        for (auto& e : ctx->small_ev) BH_HIP(hipEventCreate(&e));
+}
+
 // lu16_ir64 batches: the problems of each size class factored in lock step
-// (class_lu16_enqueue), every class on a solver lane of its own (largest
+// (class_lu16_enqueue), every (sub-)class on a solver lane of its own (largest
 // first; lanes reused round robin when there are more classes), the small
 // problems' launch on the next lane.  All work is enqueued before the first
 // read-back, so the classes overlap on the device.
-static void solve_classes(bemhip_ctx* ctx, const DevTopology& topo, const std::vector<int>& big, double* G64,
-                          float* G32, const double* d_hphi, double* d_psi, double tol, int maxit,
-                          bemhip_solve_stats* st) {
+struct ClassPlan {
     std::vector<ClassJob> cls;
+    std::vector<Solver*> lane;
+    int l_small = 0;
+    bool small = false;
+};
+
+// the plan, with every lane, workspace and device table it needs made ready
+// (host work only: none of it is enqueued with the plan's launches)
+static ClassPlan plan_classes(bemhip_ctx* ctx, const DevTopology& topo, int solver) {
+    ClassPlan P;
+    std::vector<int> big;
+    split_small(ctx, topo, solver, big);
+    std::vector<ClassJob>& cls = P.cls;
     {
         std::vector<int> order(big);
         std::stable_sort(order.begin(), order.end(),
@@ -160,37 +204,82 @@ static void solve_classes(bemhip_ctx* ctx, const DevTopology& topo, const std::v
             c.v0.push_back(pr.v0);
             c.cnt += 1;
         }
+        // sub-classes: consecutive problems of a class (still evenly spaced)
+        const int split = std::max(1, ctx->batch_split);
+        std::vector<ClassJob> sub;
+        for (const ClassJob& c : cls) {
+            const int parts = std::min(split, c.cnt);
+            for (int k = 0, q0 = 0; k < parts; ++k) {
+                const int q1 = (int)((int64_t)c.cnt * (k + 1) / parts);
+                ClassJob sj;
+                sj.n = c.n;
+                sj.cnt = q1 - q0;
+                sj.gofs0 = c.gofs0 + (int64_t)q0 * gblock_stride(c.n);
+                sj.v0.assign(c.v0.begin() + q0, c.v0.begin() + q1);
+                sj.lookahead = ctx->batch_lookahead;
+                sub.push_back(std::move(sj));
+                q0 = q1;
+            }
+        }
+        cls.swap(sub);
     }
-    const int nsmall = (int)ctx->small_jobs_h.size();
+    P.small = !ctx->small_jobs_h.empty();
     const int nc = (int)cls.size();
-    const int L = std::max(1, std::min(ctx->max_lanes, nc + (nsmall > 0 ? 1 : 0)));
+    const int L = std::max(1, std::min(ctx->max_lanes, nc + (P.small ? 1 : 0)));
     while ((int)ctx->lanes.size() < L - 1) {
         ctx->lanes.emplace_back(new Solver());
         ctx->lanes.back()->init(ctx);
     }
-    std::vector<Solver*> lane(L);
-    lane[0] = ctx;
-    for (int l = 1; l < L; ++l) lane[l] = ctx->lanes[l - 1].get();
+    P.lane.resize(L);
+    P.lane[0] = ctx;
+    for (int l = 1; l < L; ++l) P.lane[l] = ctx->lanes[l - 1].get();
     while ((int)ctx->class_work.size() < nc) ctx->class_work.emplace_back(new ClassWork());
+    for (int i = 0; i < nc; ++i) class_lu16_prepare(P.lane[i % L], ctx->class_work[i].get(), cls[i]);
+    P.l_small = nc % L;
+    if (P.small) small_upload(ctx, P.lane[P.l_small]->stream);
+    return P;
+}
+
+// every class's factorisation, solves and first sweeps and the small
+// problems' launch, forked from and joined back to the context's stream
+static void enqueue_classes(bemhip_ctx* ctx, const ClassPlan& P, double* G64, float* G32, const double* d_hphi,
+                            double* d_psi, double tol, int maxit, bool events) {
+    const int L = (int)P.lane.size(), nc = (int)P.cls.size();
     hipEvent_t ready = ctx->ev[0];
     BH_HIP(hipEventRecord(ready, ctx->stream));
-    for (int l = 1; l < L; ++l) BH_HIP(hipStreamWaitEvent(lane[l]->stream, ready, 0));
+    for (int l = 1; l < L; ++l) BH_HIP(hipStreamWaitEvent(P.lane[l]->stream, ready, 0));
     const int sweeps = std::max(0, std::min(ctx->batch_sweeps, maxit > 0 ? maxit : ctx->batch_sweeps));
     for (int i = 0; i < nc; ++i)
-        class_lu16_enqueue(lane[i % L], ctx->class_work[i].get(), cls[i], G64, G32, d_hphi, tol, sweeps);
-    const int l_small = nc % L;
-    if (nsmall > 0) small_launch(ctx, lane[l_small]->stream, G64, d_hphi, d_psi);
+        class_lu16_enqueue(P.lane[i % L], ctx->class_work[i].get(), P.cls[i], G64, G32, d_hphi, d_psi, tol, sweeps,
+                           events);
+    if (P.small) small_launch(ctx, P.lane[P.l_small]->stream, G64, d_hphi, d_psi, events);
+    for (int l = 1; l < L; ++l) {
+        // each lane's own join event: a graph capture needs the fork and join
+        // of every forked stream
+        hipEvent_t done = P.lane[l]->ev_la[3];
+        BH_HIP(hipEventRecord(done, P.lane[l]->stream));
+        BH_HIP(hipStreamWaitEvent(ctx->stream, done, 0));
+    }
+}
+
+// read-back, further sweeps, the flagged problems solved again, the small
+// problems' info words
+static void finish_classes(bemhip_ctx* ctx, const ClassPlan& P, double* G64, float* G32, const double* d_hphi,
+                           double* d_psi, double tol, int maxit, bemhip_solve_stats* st, bool events) {
+    const int L = (int)P.lane.size(), nc = (int)P.cls.size();
     bemhip_solve_stats sum{};
     sum.converged = 1;
     for (int i = 0; i < nc; ++i) {
         bemhip_solve_stats one{};
-        class_lu16_finish(lane[i % L], ctx->class_work[i].get(), cls[i], G64, G32, d_hphi, d_psi, tol, maxit, &one);
+        class_lu16_finish(P.lane[i % L], ctx->class_work[i].get(), P.cls[i], G64, G32, d_hphi, d_psi, tol, maxit,
+                          &one);
         merge_stats(sum, one);
     }
-    if (nsmall > 0) small_finish(ctx, lane[l_small]->stream, sum);
+    if (P.small) small_finish(ctx, P.lane[P.l_small]->stream, sum, events);
+    // the context's stream continues after the lanes' further work
     for (int l = 1; l < L; ++l) {
-        hipEvent_t done = ctx->ev[1];
-        BH_HIP(hipEventRecord(done, lane[l]->stream));
+        hipEvent_t done = P.lane[l]->ev_la[3];
+        BH_HIP(hipEventRecord(done, P.lane[l]->stream));
         BH_HIP(hipStreamWaitEvent(ctx->stream, done, 0));
     }
     *st = sum;
@@ -207,21 +296,8 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
     // small problems: one launch for all of them (FP64 GEPP in place, no
     // refinement needed); the rest go to the solver lanes
     std::vector<int> big;
-    ctx->small_jobs_h.clear();
-    for (int q = 0; q < (int)topo.probs.size(); ++q) {
-        const AsmProb& pr = topo.probs[q];
-        if (solver != BEMHIP_SOLVE_BICGSTAB64 && pr.n <= std::min(ctx->small_n, SMALL_MAX_N))
-            ctx->small_jobs_h.push_back(SmallJob{pr.gofs, pr.n, pr.v0});
-        else
-            big.push_back(q);
-    }
+    split_small(ctx, topo, solver, big);
     const int nsmall = (int)ctx->small_jobs_h.size();
-    if (nsmall > 0 && !ctx->small_ev[0])
-        for (auto& e : ctx->small_ev) BH_HIP(hipEventCreate(&e));
-    if (solver == BEMHIP_SOLVE_LU16_IR64 && ctx->batch_lu && G32) {
-        solve_classes(ctx, topo, big, G64, G32, d_hphi, d_psi, tol, maxit, st);
-        return;
-    }
     const int np = (int)big.size();
     const int L = std::max(1, std::min(ctx->max_lanes, np));
     while ((int)ctx->lanes.size() < L - 1) {
@@ -286,6 +362,82 @@ void solve_batch(bemhip_ctx* ctx, const DevTopology& topo, double* G64, float* G
     *st = sum;
 }
 
+// An lu16_ir64 batch's assembly and class solves.  With batch_graph the
+// device work of a stage (assembly, every class's factorisation, solves and
+// first sweeps, the small problems' launch) is captured into a graph the
+// second time the same key (topology, buffers, solver settings) comes by, and
+// replayed from then on; the read-back and whatever it decides (further
+// sweeps, problems solved again) run after it.  A graph is dropped when any
+// device buffer has been reallocated since its capture.
+static void psi_classes(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi,
+                        uint32_t flags, double tol, int maxit, double* G64, float* G32, double* d_hphi,
+                        double* d_psi, bemhip_solve_stats* st) {
+    const ClassPlan P = plan_classes(ctx, topo, BEMHIP_SOLVE_LU16_IR64);
+    hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+    BatchGraphKey key{topo.gen, d_xyz, d_phi, d_hphi, d_psi, G64, G32, flags, tol, maxit,
+                      ctx->nb16, ctx->batch_split, ctx->batch_lookahead ? 1 : 0, ctx->batch_sweeps, ctx->mm_reserve,
+                      ctx->small_n};
+    BatchGraph* g = nullptr;
+    for (auto& e : ctx->batch_graphs)
+        if (e.key == key) g = &e;
+    const bool use_graph = ctx->batch_graph && g != nullptr;
+    if (use_graph && (!g->exec || g->allocs != g_devbuf_allocs.load())) {
+        if (g->exec) {
+            (void)hipGraphExecDestroy(g->exec);
+            g->exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        BH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        try {
+            assemble_device(ctx, topo, d_xyz, d_phi, flags, AsmOut{G64, G32, nullptr, d_hphi});
+            enqueue_classes(ctx, P, G64, G32, d_hphi, d_psi, tol, maxit, false);
+        } catch (...) {
+            (void)hipStreamEndCapture(ctx->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        BH_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BH_HIP(ie);
+        g->allocs = g_devbuf_allocs.load();
+        g->captures += 1;
+    }
+    BH_HIP(hipEventRecord(e0, ctx->stream));
+    if (use_graph) {
+        BH_HIP(hipGraphLaunch(g->exec, ctx->stream));
+        g->launches += 1;
+        // the graph ran on the context's stream only: the lanes, on which the
+        // read-back and any further sweeps of their classes run, wait for it
+        hipEvent_t done = ctx->ev[1];
+        BH_HIP(hipEventRecord(done, ctx->stream));
+        for (size_t l = 1; l < P.lane.size(); ++l) BH_HIP(hipStreamWaitEvent(P.lane[l]->stream, done, 0));
+    } else {
+        assemble_device(ctx, topo, d_xyz, d_phi, flags, AsmOut{G64, G32, nullptr, d_hphi});
+        BH_HIP(hipEventRecord(e1, ctx->stream));
+        enqueue_classes(ctx, P, G64, G32, d_hphi, d_psi, tol, maxit, true);
+        if (ctx->batch_graph) {
+            // seen once: captured on the next call with this key
+            if (ctx->batch_graphs.size() >= 8) {
+                if (ctx->batch_graphs.front().exec) (void)hipGraphExecDestroy(ctx->batch_graphs.front().exec);
+                ctx->batch_graphs.erase(ctx->batch_graphs.begin());
+            }
+            ctx->batch_graphs.push_back(BatchGraph{key});
+        }
+    }
+    finish_classes(ctx, P, G64, G32, d_hphi, d_psi, tol, maxit, st, !use_graph);
+    if (use_graph) {
+        // the replay's span (assembly, factorisations, solves, first sweeps)
+        BH_HIP(hipEventRecord(e1, ctx->stream));
+        const float ms = event_ms(e0, e1);
+        st->ms_factor = ms;
+        st->ms_solve = 0.0f;
+        st->ms_assemble = 0.0f;
+    } else {
+        st->ms_assemble = event_ms(e0, e1);
+    }
+}
+
 // assembly + H phi + solve on device-resident positions / potential
 void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, const double* d_phi, uint32_t flags,
                 int solver, double tol, int maxit, double* d_hphi, double* d_psi, bemhip_solve_stats* st,
@@ -296,6 +448,10 @@ void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, c
     double* G64 = dev<double>(ctx->G64, nn);
     float* G32 = mixed(solver) ? dev<float>(ctx->G32, nn) : nullptr;
     hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+    if (topo.probs.size() > 1 && solver == BEMHIP_SOLVE_LU16_IR64 && ctx->batch_lu) {
+        psi_classes(ctx, topo, d_xyz, d_phi, flags, tol, maxit, G64, G32, d_hphi, d_psi, st);
+        return;
+    }
     BH_HIP(hipEventRecord(e0, ctx->stream));
     assemble_device(ctx, topo, d_xyz, d_phi, flags, AsmOut{G64, G32, nullptr, d_hphi});
     BH_HIP(hipEventRecord(e1, ctx->stream));
@@ -514,6 +670,11 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (bl && bl[0] == '0') ctx->batch_lu = false;
         if (const char* v = std::getenv("BEMHIP_BATCH_SWEEPS")) ctx->batch_sweeps = std::max(0, std::atoi(v));
         if (const char* v = std::getenv("BEMHIP_BATCH_FORCE_REDO")) ctx->batch_force_redo = std::atoi(v);
+        if (const char* v = std::getenv("BEMHIP_BATCH_SPLIT")) ctx->batch_split = std::max(1, std::atoi(v));
+        const char* bla = std::getenv("BEMHIP_BATCH_LOOKAHEAD");
+        if (bla && bla[0] == '0') ctx->batch_lookahead = false;
+        const char* bg = std::getenv("BEMHIP_BATCH_GRAPH");
+        if (bg && bg[0] == '1') ctx->batch_graph = true;
         ctx->init(nullptr);
     });
     if (rc != BEMHIP_OK) {
@@ -529,6 +690,9 @@ int bemhip_create(int device, bemhip_ctx** out) {
 int bemhip_destroy(bemhip_ctx* ctx) {
     if (!ctx) return BEMHIP_OK;
     (void)hipSetDevice(ctx->device);
+    for (auto& g : ctx->batch_graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->batch_graphs.clear();
     for (auto& l : ctx->lanes) l->release();
     ctx->class_work.clear();
     ctx->lanes.clear();

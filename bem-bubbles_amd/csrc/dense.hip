@@ -748,7 +748,12 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
                                                   unsigned int* __restrict__ sync, unsigned int epoch,
                                                   float* __restrict__ linv, float* __restrict__ backup,
                                                   unsigned int* __restrict__ gticket, int stripes, long long sA,
-                                                  int sstride, long long slinv) {
+                                                  int sstride, long long slinv,
+                                                  const unsigned int* __restrict__ ebase) {
+    // the publication stamp: the host's count of launches, plus a device base
+    // that the batched path advances before each factorisation (so launches
+    // replayed from a graph stamp new values)
+    if (ebase) epoch += *ebase;
     constexpr int MAXW = 8;
     __shared__ float Us[NPB * NPB];      // U rows of step j: Us[i * NPB + c]
     __shared__ float Wt[NPB * NPB];      // published L^-1 of stripe j (transposed) / own U^-1
@@ -1554,7 +1559,8 @@ static void panel_np(Solver* ctx, Lane L, int m, int n, T* A, int lda, unsigned 
     if (top32<T>(ctx, n)) {
         if constexpr (std::is_same<T, float>::value)
             k_np_top32<<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, ctx->lu.diag.as<T>(),
-                                                               ctx->lu.backup.as<T>(), nullptr, stripes, 0, 0, 0);
+                                                               ctx->lu.backup.as<T>(), nullptr, stripes, 0, 0, 0,
+                                                               nullptr);
     } else if (ctx->np_top && stripes <= TOP_STRIPES && n <= top_max_rows<T>()) {
         T* linv = ctx->lu.diag.as<T>();
         k_np_top<T><<<stripes, top_threads<T>(n), 0, L.s>>>(n, A, lda, sync, ++ctx->np_epoch, linv);
@@ -2134,46 +2140,67 @@ void pivots_to_perm(const std::vector<int>& ipiv, std::vector<int>& perm) {
     const int n = (int)ipiv.size();
     perm.resize(n);
     for (int i = 0; i < n; ++i) perm[i] = i;
-    for (int k = 0; k < n; ++k) std::swap(perm[k], perm[ipiv[k]]);
+    for (int k = 0; k < n; ++k) {
+        // partial pivoting only ever swaps row k with a row at or below it
+        BH_CHECK(ipiv[k] >= k && ipiv[k] < n, BEMHIP_ERR_HIP,
+                 "LU pivot " + std::to_string(ipiv[k]) + " of column " + std::to_string(k) + " out of range");
+        std::swap(perm[k], perm[ipiv[k]]);
+    }
 }
 
 // ---- persistent triangular solves -------------------------------------
 // FP64 inverses of the TS x TS diagonal blocks (column-major per block), one
-// work-group per block; thread c builds column c of the inverse.
-// grid.y: problem of a batch (A and inv sA, sInv elements further)
+// work-group per block (grid.y: problem of a batch, A and inv sA, sInv
+// elements further).  The block is staged in LDS with coalesced loads; thread
+// (i, h) holds half h of row i of the inverse in registers, and step k
+// broadcasts the finished row k through LDS to the rows it updates
+// (right-looking substitution, one barrier a step, the broadcast row
+// double-buffered; a half that is still zero in row k is skipped):
+//   lower, unit:  X[i,:] = e_i - sum_{k<i} L[i][k] X[k,:]
+//   upper:        X[i,:] = (e_i - sum_{k>i} U[i][k] X[k,:]) / U[i][i]
+// Rows and columns beyond a short last block are the identity's.
 template <class T, bool LOWER>
-__global__ __launch_bounds__(TS) void k_diag_inv(int N, const T* __restrict__ A, double* __restrict__ inv,
-                                                 long long sA, long long sInv) {
-    extern __shared__ double ys[];   // [TS][TS]: ys[r * TS + c]
-    const int b = blockIdx.x, c = threadIdx.x;
+__global__ __launch_bounds__(2 * TS) void k_diag_inv(int N, const T* __restrict__ A, double* __restrict__ inv,
+                                                     long long sA, long long sInv) {
+    constexpr int HW = TS / 2;
+    __shared__ T s[TS][TS + 1];     // s[i][k] = A(j0 + i, j0 + k)
+    __shared__ double rb[2][TS];    // the broadcast row, by step parity
     A += blockIdx.y * sA;
     inv += blockIdx.y * sInv;
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int i = t & (TS - 1), h = t >> 7;   // this thread's row and half (a wave-uniform half)
     const int j0 = b * TS, bs = min(TS, N - j0);
     const T* Ab = A + (size_t)j0 * N + j0;
-    if (LOWER) {
-        for (int r = 0; r < TS; ++r) {
-            double v = (r == c) ? 1.0 : 0.0;
-            if (r < bs && r > c)
-                for (int k = c; k < r; ++k) v -= (double)Ab[(size_t)k * N + r] * ys[k * TS + c];
-            if (r >= bs || c >= bs) v = (r == c) ? 1.0 : 0.0;
-            ys[r * TS + c] = v;
-        }
-    } else {
-        for (int r = TS - 1; r >= 0; --r) {
-            double v = (r == c) ? 1.0 : 0.0;
-            if (r < bs && c < bs) {
-                if (r <= c) {
-                    for (int k = r + 1; k <= c; ++k) v -= (double)Ab[(size_t)k * N + r] * ys[k * TS + c];
-                    v /= (double)Ab[(size_t)r * N + r];
-                } else {
-                    v = 0.0;
-                }
+    for (int c = h; c < TS; c += 2) s[i][c] = (i < bs && c < bs) ? Ab[(size_t)c * N + i] : (i == c ? T(1) : T(0));
+    double x[HW];
+#pragma unroll
+    for (int c = 0; c < HW; ++c) x[c] = (h * HW + c == i) ? 1.0 : 0.0;
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < TS; ++q) {
+        const int k = LOWER ? q : TS - 1 - q;
+        if (i == k) {
+            if (!LOWER) {
+                const double r = 1.0 / (double)s[k][k];
+#pragma unroll
+                for (int c = 0; c < HW; ++c) x[c] *= r;
             }
-            ys[r * TS + c] = v;
+#pragma unroll
+            for (int c = 0; c < HW; ++c) rb[q & 1][h * HW + c] = x[c];
+        }
+        __syncthreads();
+        const bool below = LOWER ? i > k : i < k;
+        // row k is zero left of column k (upper) / right of it (lower)
+        const bool nz = LOWER ? h * HW <= k : h * HW + HW - 1 >= k;
+        if (below && nz) {
+            const double f = (double)s[i][k];
+#pragma unroll
+            for (int c = 0; c < HW; ++c) x[c] -= f * rb[q & 1][h * HW + c];
         }
     }
     double* o = inv + (size_t)b * TS * TS;
-    for (int r = 0; r < TS; ++r) o[(size_t)c * TS + r] = ys[r * TS + c];
+#pragma unroll
+    for (int c = 0; c < HW; ++c) o[(size_t)(h * HW + c) * TS + i] = x[c];
 }
 
 // One work-group per TS-row block, taken in dependency order through a ticket
@@ -2195,6 +2222,7 @@ struct TrsvBat {
     long long sA, sInv;
     int sstride;
     const int* active;
+    const unsigned int* ebase;   // device base added to the launch's stamp (as k_np_top32)
 };
 template <class T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict__ A, const double* __restrict__ inv,
@@ -2208,6 +2236,7 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     if (tid == 0) s_t = (int)atomicAdd(tb.gticket ? tb.gticket : sync, 1u);
     __syncthreads();
     int t = s_t;
+    if (tb.ebase) epoch += *tb.ebase;
     if (tb.gticket) {
         const int q = t / nb;
         t -= q * nb;
@@ -2345,11 +2374,8 @@ template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
     const int nb = (N + TS - 1) / TS;
     double* il = static_cast<double*>(ctx->lu.inv_lo.ensure(sizeof(double) * (size_t)nb * TS * TS));
     double* iu = static_cast<double*>(ctx->lu.inv_up.ensure(sizeof(double) * (size_t)nb * TS * TS));
-    const size_t sh = sizeof(double) * TS * TS;
-    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    k_diag_inv<T, true><<<nb, TS, sh, ctx->stream>>>(N, LU, il, 0, 0);
-    k_diag_inv<T, false><<<nb, TS, sh, ctx->stream>>>(N, LU, iu, 0, 0);
+    k_diag_inv<T, true><<<nb, 2 * TS, 0, ctx->stream>>>(N, LU, il, 0, 0);
+    k_diag_inv<T, false><<<nb, 2 * TS, 0, ctx->stream>>>(N, LU, iu, 0, 0);
     BH_HIP(hipGetLastError());
     const size_t fb = sizeof(unsigned int) * (16 + (size_t)nb);
     if (ctx->lu.flag.bytes < fb) {
@@ -2368,7 +2394,7 @@ template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, dou
     const unsigned int epoch = ++ctx->trsv_epoch;
     k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error word
     if (ctx->trsv_force_giveup) k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync + 1, 1, 1u);
-    const TrsvBat one{nullptr, 0, 0, 0, nullptr};
+    const TrsvBat one{nullptr, 0, 0, 0, nullptr, nullptr};
     if (lower)
         k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch, one);
     else
@@ -2892,6 +2918,11 @@ __global__ void k_cls_words(int cnt, unsigned int* __restrict__ base, int stride
     if (q == 0 && ticket) *ticket = 0u;
 }
 
+__global__ void k_copy_f64(size_t n, const double* __restrict__ a, double* __restrict__ b) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 // dst[q n + i] = src[v0[q] + i]
 __global__ void k_cls_gather(int n, const double* __restrict__ src, const int* __restrict__ v0,
                              double* __restrict__ dst) {
@@ -2985,6 +3016,13 @@ __global__ __launch_bounds__(256) void k_cls_ir_corr(int n, double* __restrict__
     }
 }
 
+// the device base of the class's publication stamps (tickets word CLS_EBASE)
+// advances by CLS_EPOCH_STEP before every factorisation; a factorisation and
+// its solves stamp fewer launches than that
+constexpr int CLS_EBASE = 2;
+constexpr unsigned int CLS_EPOCH_STEP = 1u << 16;
+__global__ void k_epoch_advance(unsigned int* base) { *base += CLS_EPOCH_STEP; }
+
 namespace {
 constexpr int CLS_TRSV_WORDS_MIN = 16;
 inline float p2(int k) { return std::ldexp(1.0f, k); }
@@ -3016,7 +3054,8 @@ void cls_panel(const ClassPtrs& c, hipStream_t s, int k, int m, int jb, int p, b
     k_fill_u32<<<1, 64, 0, s>>>(gt, 1, 0u);
     k_np_top32<<<stripes * cnt, top_threads<float>(jb), 0, s>>>(
         jb, Ap, n, w->top_sync.as<unsigned int>(), ++w->top_epoch, w->linv.as<float>(), nullptr, gt, stripes, c.sA,
-        TOP_SYNC_WORDS, (long long)TOP_STRIPES * NPB * NPB);
+        TOP_SYNC_WORDS, (long long)TOP_STRIPES * NPB * NPB, w->tickets.as<unsigned int>() + CLS_EBASE);
+    BH_CHECK(w->top_epoch < CLS_EPOCH_STEP, BEMHIP_ERR_ARG, "batched fp16 LU: too many top-block launches");
     BH_HIP(hipGetLastError());
     if (m <= jb) return;
     const long long sW = (long long)c.NB * c.NB;
@@ -3084,7 +3123,8 @@ void cls_trsv(const ClassPtrs& c, double* x, const int* active) {
     const long long sInv = (long long)nbk * TS * TS;
     for (int lower = 1; lower >= 0; --lower) {
         k_cls_words<<<(cnt + 255) / 256, 256, 0, c.main>>>(cnt, ts, tss, 1, 0u, gt);
-        const TrsvBat tb{gt, c.sA, sInv, tss, active};
+        const TrsvBat tb{gt, c.sA, sInv, tss, active, w->tickets.as<unsigned int>() + CLS_EBASE};
+        BH_CHECK(w->trsv_epoch + 1 < CLS_EPOCH_STEP, BEMHIP_ERR_ARG, "batched fp16 LU: too many triangular solves");
         if (lower)
             k_trsv_persist<float, true><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_lo.as<double>(), x, ts,
                                                                       ++w->trsv_epoch, tb);
@@ -3121,8 +3161,9 @@ ClassPtrs class_ptrs(Solver* ctx, ClassWork* w, const ClassJob& cj, float* G32) 
     c.cnt = cj.cnt;
     c.NB = w->nb;
     c.main = ctx->stream;
-    c.side = ctx->lookahead ? ctx->stream2 : ctx->stream;
-    c.max_wg = ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0;
+    const bool la = ctx->lookahead && cj.lookahead;
+    c.side = la ? ctx->stream2 : ctx->stream;
+    c.max_wg = la && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0;
     return c;
 }
 
@@ -3173,12 +3214,14 @@ void class_work_ensure(Solver* ctx, ClassWork* w, const ClassJob& cj) {
     w->scal.ensure((2 * sizeof(double) + 3 * sizeof(int)) * C);
     w->v0.ensure(sizeof(int) * C);
     const size_t tsw = C * TOP_SYNC_WORDS, trw = C * (CLS_TRSV_WORDS_MIN + nbk);
+    if (fresh) w->v0_h.clear();
     if (fresh || w->top_sync.bytes < sizeof(unsigned int) * tsw || w->tsync.bytes < sizeof(unsigned int) * trw) {
         // flags start below every publication stamp
         w->top_sync.ensure(sizeof(unsigned int) * tsw);
         w->tsync.ensure(sizeof(unsigned int) * trw);
         k_fill_u32<<<(int)((tsw + 255) / 256), 256, 0, ctx->stream>>>(w->top_sync.as<unsigned int>(), (int)tsw, 0u);
         k_fill_u32<<<(int)((trw + 255) / 256), 256, 0, ctx->stream>>>(w->tsync.as<unsigned int>(), (int)trw, 0u);
+        k_fill_u32<<<1, 64, 0, ctx->stream>>>(w->tickets.as<unsigned int>(), 64, 0u);
         BH_HIP(hipGetLastError());
         w->top_epoch = 0;
         w->trsv_epoch = 0;
@@ -3188,17 +3231,31 @@ void class_work_ensure(Solver* ctx, ClassWork* w, const ClassJob& cj) {
 }
 }  // namespace
 
+void class_lu16_prepare(Solver* ctx, ClassWork* w, const ClassJob& cj) {
+    BH_CHECK(cj.n > 0 && cj.cnt > 0 && (int)cj.v0.size() == cj.cnt, BEMHIP_ERR_ARG, "batched fp16 LU: bad class");
+    class_work_ensure(ctx, w, cj);
+    if (w->v0_h != cj.v0) {
+        // the problems' offsets into b and x, uploaded when they change
+        BH_HIP(hipMemcpyAsync(w->v0.p, cj.v0.data(), sizeof(int) * cj.cnt, hipMemcpyHostToDevice, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        w->v0_h = cj.v0;
+    }
+}
+
 void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
-                        double tol, int sweeps) {
+                        double* x, double tol, int sweeps, bool events) {
     BH_CHECK(cj.n > 0 && cj.cnt > 0 && (int)cj.v0.size() == cj.cnt, BEMHIP_ERR_ARG, "batched fp16 LU: bad class");
     BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "batched fp16 LU needs the FP32 copy of G");
     if (tol <= 0.0) tol = IR_TOL;
-    class_work_ensure(ctx, w, cj);
+    class_lu16_prepare(ctx, w, cj);
     const ClassPtrs c = class_ptrs(ctx, w, cj, G32);
     const int n = c.n, cnt = c.cnt, NB = c.NB;
     hipEvent_t ev_ready = w->ev[0], ev_panel = w->ev[1];
-    BH_HIP(hipEventRecord(w->ev[2], c.main));
-    BH_HIP(hipMemcpyAsync(w->v0.p, cj.v0.data(), sizeof(int) * cnt, hipMemcpyHostToDevice, c.main));
+    if (events) BH_HIP(hipEventRecord(w->ev[2], c.main));
+    w->timed = events;
+    k_epoch_advance<<<1, 1, 0, c.main>>>(w->tickets.as<unsigned int>() + CLS_EBASE);
+    w->top_epoch = 0;
+    w->trsv_epoch = 0;
     k_cls_prep<<<cnt, 256, 0, c.main>>>(n, c.A, c.sA, w->e.as<int>(), w->top_sync.as<unsigned int>(), TOP_SYNC_WORDS,
                                         w->maxl64.as<unsigned long long>(), w->f16log.as<unsigned int>());
     BH_HIP(hipGetLastError());
@@ -3232,15 +3289,12 @@ void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G
         jb = jb2;
         p ^= 1;
     }
-    BH_HIP(hipEventRecord(w->ev[3], c.main));
+    if (events) BH_HIP(hipEventRecord(w->ev[3], c.main));
     // solves: FP64 inverses of the 128 x 128 diagonal blocks, x0 = (LU)^-1 b,
     // then the refinement sweeps
-    const size_t sh = sizeof(double) * TS * TS;
-    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    BH_HIP(hipFuncSetAttribute((const void*)k_diag_inv<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     const long long sInv = (long long)w->nbk * TS * TS;
-    k_diag_inv<float, true><<<dim3(w->nbk, cnt), TS, sh, c.main>>>(n, c.A, w->inv_lo.as<double>(), c.sA, sInv);
-    k_diag_inv<float, false><<<dim3(w->nbk, cnt), TS, sh, c.main>>>(n, c.A, w->inv_up.as<double>(), c.sA, sInv);
+    k_diag_inv<float, true><<<dim3(w->nbk, cnt), 2 * TS, 0, c.main>>>(n, c.A, w->inv_lo.as<double>(), c.sA, sInv);
+    k_diag_inv<float, false><<<dim3(w->nbk, cnt), 2 * TS, 0, c.main>>>(n, c.A, w->inv_up.as<double>(), c.sA, sInv);
     const int tss = CLS_TRSV_WORDS_MIN + w->nbk;
     k_cls_words<<<(cnt + 255) / 256, 256, 0, c.main>>>(cnt, w->tsync.as<unsigned int>(), tss, 2, 0u, nullptr);
     double* bb = cls_vec(w, 0);
@@ -3248,11 +3302,14 @@ void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G
     double* rr = cls_vec(w, 2);
     k_cls_gather<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, b, w->v0.as<int>(), bb);
     BH_HIP(hipGetLastError());
-    BH_HIP(hipMemcpyAsync(xx, bb, sizeof(double) * (size_t)cnt * n, hipMemcpyDeviceToDevice, c.main));
+    k_copy_f64<<<std::min(4096, (int)(((size_t)cnt * n + 255) / 256)), 256, 0, c.main>>>((size_t)cnt * n, bb, xx);
     cls_trsv(c, xx, nullptr);
     k_cls_ir_init<<<cnt, 256, 0, c.main>>>(n, bb, ir_state(w));
     BH_HIP(hipGetLastError());
     for (int s = 0; s < sweeps; ++s) cls_sweep(c, G64 + cj.gofs0, bb, xx, rr, tol);
+    k_cls_scatter<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, xx, w->v0.as<int>(), x);
+    BH_HIP(hipGetLastError());
+    if (events) BH_HIP(hipEventRecord(w->ev[4], c.main));
 }
 
 void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
@@ -3277,8 +3334,11 @@ void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G6
     BH_HIP(hipMemcpyAsync(ml.data(), w->maxl64.p, sizeof(unsigned long long) * cnt, hipMemcpyDeviceToHost, c.main));
     BH_HIP(hipMemcpyAsync(f16.data(), w->f16log.p, sizeof(unsigned int) * cnt, hipMemcpyDeviceToHost, c.main));
     read_status();
-    float fms = 0.0f;
-    BH_HIP(hipEventElapsedTime(&fms, w->ev[2], w->ev[3]));
+    float fms = 0.0f, sms = 0.0f;
+    if (w->timed) {
+        BH_HIP(hipEventElapsedTime(&fms, w->ev[2], w->ev[3]));
+        BH_HIP(hipEventElapsedTime(&sms, w->ev[3], w->ev[4]));
+    }
     w->ms_factor = fms;
     const double* sd = reinterpret_cast<const double*>(sc.data());
     auto sint = [&](int which, int q) { return reinterpret_cast<const int*>(sd + 2 * (size_t)cnt)[(size_t)which * cnt + q]; };
@@ -3316,8 +3376,10 @@ void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G6
             BH_CHECK(ts[(size_t)q * tss + 2] == 0u, BEMHIP_ERR_HIP,
                      "persistent triangular solve gave up waiting for a block (result invalid)");
     }
-    k_cls_scatter<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, xx, w->v0.as<int>(), x);
-    BH_HIP(hipGetLastError());
+    if (extra > 0) {
+        k_cls_scatter<<<dim3((n + 255) / 256, cnt), 256, 0, c.main>>>(n, xx, w->v0.as<int>(), x);
+        BH_HIP(hipGetLastError());
+    }
     bemhip_solve_stats sum{};
     sum.converged = 1;
     for (int q = 0; q < cnt; ++q) {
@@ -3342,6 +3404,7 @@ void class_lu16_finish(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G6
         sum.ms_factor += one.ms_factor;
     }
     sum.ms_factor += fms;
+    sum.ms_solve += sms;
     *st = sum;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -43,6 +44,10 @@ struct Error : std::runtime_error {
     } while (0)
 
 // ------------------------------------------------------- device buffers --
+// device allocations made by DevBuf so far (a captured graph is only replayed
+// while no buffer it may address has been reallocated since its capture)
+extern std::atomic<unsigned long long> g_devbuf_allocs;
+
 // Grow-only device allocation (contents are not preserved on growth).
 struct DevBuf {
     void* p = nullptr;
@@ -62,6 +67,7 @@ struct DevBuf {
         if (n == 0) n = 256;
         BH_HIP(hipMalloc(&p, n));
         bytes = n;
+        g_devbuf_allocs.fetch_add(1);
         return p;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
@@ -122,6 +128,7 @@ Topology build_topology(int64_t N, const double* xyz, int64_t M, const uint32_t*
 
 // Device copies of the topology tables
 struct DevTopology {
+    uint64_t gen = 0;                  // distinct for every upload (in the process)
     int64_t N = 0, M = 0;
     int npatch = 0, nparts = 0, nentries = 0, nitems = 0, max_groups = 0;
     bool independent = false;
@@ -142,6 +149,32 @@ struct SmallJob {
 };
 
 struct ClassWork;
+
+// what a captured batch stage depends on besides the buffers' contents
+struct BatchGraphKey {
+    uint64_t topo_gen;
+    const double* xyz;
+    const double* phi;
+    double* hphi;
+    double* psi;
+    double* G64;
+    float* G32;
+    uint32_t flags;
+    double tol;
+    int maxit, nb16, split, lookahead, sweeps, reserve, small_n;
+    bool operator==(const BatchGraphKey& o) const {
+        return topo_gen == o.topo_gen && xyz == o.xyz && phi == o.phi && hphi == o.hphi && psi == o.psi &&
+               G64 == o.G64 && G32 == o.G32 && flags == o.flags && tol == o.tol && maxit == o.maxit &&
+               nb16 == o.nb16 && split == o.split && lookahead == o.lookahead && sweeps == o.sweeps &&
+               reserve == o.reserve && small_n == o.small_n;
+    }
+};
+struct BatchGraph {
+    BatchGraphKey key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long allocs = 0;   // g_devbuf_allocs at capture
+    int captures = 0, launches = 0;
+};
 
 struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
@@ -261,7 +294,20 @@ struct bemhip_ctx : bemhip::Solver {
     // host read-back of a class
     bool batch_lu = true;
     int batch_sweeps = 2;
+    // each size class split into this many sub-classes on lanes of their own
+    // (their panels and updates then overlap without a fixed split of the
+    // compute units); look-ahead inside a (sub-)class
+    int batch_split = 1;
+    bool batch_lookahead = true;
     std::vector<std::unique_ptr<bemhip::ClassWork>> class_work;
+    std::vector<bemhip::SmallJob> small_jobs_up;   // the job table last uploaded
+    const bemhip::SmallJob* small_jobs_dev = nullptr;
+    // lu16_ir64 batches: a stage's device work captured into a graph the
+    // second time its key comes by, and replayed.  Off: with it on, RK4
+    // stepping of a batch ended in a host segmentation fault whose cause has
+    // not been found (BEMHIP_BATCH_GRAPH=1 switches it on)
+    bool batch_graph = false;
+    std::vector<bemhip::BatchGraph> batch_graphs;
 };
 
 namespace bemhip {
@@ -376,6 +422,10 @@ struct ClassJob {
     int n = 0, cnt = 0;
     int64_t gofs0 = 0;
     std::vector<int32_t> v0;
+    // the next panel of every problem factored on the solver's second stream
+    // while the update of the current step runs (else one stream, every
+    // launch free to take the whole device)
+    bool lookahead = true;
 };
 // Device workspace of one class (grown on demand, kept between stages).
 struct ClassWork {
@@ -385,7 +435,9 @@ struct ClassWork {
         t_u12;
     DevBuf inv_lo, inv_up, tsync, vecs, part, scal, v0;
     unsigned int top_epoch = 0, trsv_epoch = 0;
-    hipEvent_t ev[4] = {};   // next panel's columns ready, panel done, solve start, end
+    std::vector<int32_t> v0_h;   // host copy of the uploaded offsets
+    bool timed = false;          // the last enqueue recorded its timing events
+    hipEvent_t ev[5] = {};   // next panel's columns ready, panel done, factorisation start, end, sweeps end
     float ms_factor = 0.0f;
     ClassWork() = default;
     ClassWork(const ClassWork&) = delete;
@@ -396,12 +448,18 @@ struct ClassWork {
 // synchronisation, the lock-step fp16 LU of every problem of the class (one
 // launch per block product over all of them), the FP32 triangular solves and
 // `sweeps` FP64 refinement sweeps, each of which skips the problems whose
-// refinement has already stopped.  Panels are factored without row
+// refinement has already stopped, and writes the solutions to x.  Nothing in
+// it reads the host or allocates once the class is prepared, so it may be
+// captured into a graph (events = false: no timing events).  Panels are factored without row
 // interchanges; a problem whose multipliers exceed 1 in magnitude (partial
 // pivoting would have interchanged rows) or whose fp16 operands overflowed is
 // flagged for class_lu16_finish.
 void class_lu16_enqueue(Solver* ctx, ClassWork* w, const ClassJob& cj, double* G64, float* G32, const double* b,
-                        double tol, int sweeps);
+                        double* x, double tol, int sweeps, bool events = true);
+// sizes the class's workspace and uploads its offsets (host synchronous; called
+// by class_lu16_enqueue, and before a stream capture so that none of this
+// happens inside it)
+void class_lu16_prepare(Solver* ctx, ClassWork* w, const ClassJob& cj);
 // Reads the class's status back, runs further sweeps for problems still
 // refining (at most maxit in all), writes the solutions to x and solves every
 // flagged problem again on its own with the single-problem solver (partial

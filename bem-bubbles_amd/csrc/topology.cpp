@@ -278,6 +278,8 @@ template <class V> static void up(DevBuf& b, const std::vector<V>& v, hipStream_
 }
 
 void DevTopology::upload(const Topology& t, hipStream_t s) {
+    static std::atomic<uint64_t> uploads{0};
+    gen = uploads.fetch_add(1) + 1;
     N = t.N;
     M = t.M;
     npatch = t.npatch;

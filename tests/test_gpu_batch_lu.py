@@ -140,3 +140,48 @@ def test_class_lu_rk4_matches_lanes(ctx, ctx_nobatch):
     (x, ph, psi, tm), (xl, phl, psil, tml) = out
     assert tm == tml
     assert rel(x, xl) < 1e-10 and rel(ph, phl) < 1e-8 and rel(psi, psil) < 1e-8
+
+
+@pytest.fixture(scope="module")
+def ctx_graph():
+    c = _context(BEMHIP_BATCH_GRAPH="1")
+    yield c
+    c.close()
+
+
+@pytest.mark.skip(reason="graph capture of batch stages is switched off: RK4 stepping of a batch with "
+                         "BEMHIP_BATCH_GRAPH=1 ended in a host segmentation fault twice (the second run tested "
+                         "the lanes' ordering after the graph launch); cause not found")
+def test_class_lu_graph_replays_match_eager(ctx_graph, ctx):
+    """A batch stage's device work is captured into a graph on the second
+    call with the same buffers and replayed after that (a replayed call
+    reports ms_assemble = 0, its span in ms_factor).  Every replay equals the
+    eager solve exactly, with other device work and host synchronisations
+    between the replays, and a replay follows new inputs (set_phi) like the
+    eager path."""
+    cases = _class_cases(seed=5)
+    v, t, phi, parts, _, _ = meshgen.join_batch(cases)
+    other = _class_cases(seed=6)[2]
+    s = bemhip.ColocSim(ctx_graph, v, t, 1.0, 1.0, 0.0, 1.0, solver="lu16_ir64", independent_parts=True)
+    e = bemhip.ColocSim(ctx, v, t, 1.0, 1.0, 0.0, 1.0, solver="lu16_ir64", independent_parts=True)
+    s.set_phi(phi)
+    e.set_phi(phi)
+    ref = e.compute_psi().copy()
+    first = s.compute_psi().copy()
+    assert np.array_equal(first, ref)
+    for k in range(4):
+        ctx_graph.compute_psi(other.verts, other.tris, other.phi, "lu64")   # eager work in between
+        s.accum_stats(reset=True)
+        got = s.compute_psi().copy()
+        st = s.accum_stats(reset=True)
+        say(f"  call {k + 2}: {st}")
+        assert st["ms_assemble"] == 0.0 and st["ms_factor"] > 0.0   # ran from the graph
+        assert np.array_equal(got, ref), rel(got, ref)
+    phi2 = phi * 1.1 + 0.01 * meshgen.random_sh_field(v, 1, 3, 77)
+    s.set_phi(phi2)
+    e.set_phi(phi2)
+    ref2 = e.compute_psi().copy()
+    got2 = s.compute_psi().copy()
+    assert np.array_equal(got2, ref2) and rel(ref2, ref) > 1e-3
+    s.close()
+    e.close()
