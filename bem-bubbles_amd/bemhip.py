@@ -59,6 +59,14 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("nb", C.c_int32), ("split", C.c_int32), ("lookahead", C.c_int32), ("reserve", C.c_int32),
+                ("ms", C.c_double), ("chosen", C.c_int32), ("pad_", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
 class SimParams(C.Structure):
     _fields_ = [("p_inf", C.c_double), ("epsilon", C.c_double), ("sigma", C.c_double), ("gamma", C.c_double),
                 ("V_0", C.c_double), ("field_amp", C.c_double), ("field_freq", C.c_double),
@@ -114,6 +122,7 @@ def load(path: str = LIB_PATH):
     L.bemhip_sim_evolve_rk4.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.bemhip_sim_evolve_euler.argtypes = [vp, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.bemhip_sim_last_stats.argtypes = [vp, C.POINTER(SolveStats)]
+    L.bemhip_sim_tune.argtypes = [vp, C.c_int32, C.POINTER(TuneEntry), C.POINTER(C.c_int32)]
     L.bemhip_sim_accum_stats.argtypes = [vp, C.POINTER(SolveStats), C.c_int]
     L.bemhip_sim_set_params.argtypes = [vp, C.POINTER(SimParams)]
     L.bemhip_sim_set_mesh.argtypes = [vp, C.c_int64, f64p, C.c_int64, u32p]
@@ -489,6 +498,14 @@ class ColocSim:
         st = SolveStats()
         self._check(self.ctx.L.bemhip_sim_last_stats(self.h, C.byref(st)))
         return st.as_dict()
+
+    # solver tuning (no reference counterpart): every candidate setting timed
+    # on the current state, the fastest kept on the context
+    def tune(self):
+        n = C.c_int32(0)
+        buf = (TuneEntry * 64)()
+        self._check(self.ctx.L.bemhip_sim_tune(self.h, 64, buf, C.byref(n)))
+        return [buf[i].as_dict() for i in range(min(64, n.value))]
 
     def accum_stats(self, reset=False):
         st = SolveStats()

@@ -1247,6 +1247,76 @@ int bemhip_sim_evolve_euler(bemhip_sim* s, double dp, int fixdt, double* dt_out)
     });
 }
 
+int bemhip_sim_tune(bemhip_sim* s, int32_t cap, bemhip_tune_entry* entries, int32_t* count) {
+    if (!s) return BEMHIP_ERR_ARG;
+    return guarded(s->ctx, [&] {
+        BH_CHECK(cap >= 0 && (cap == 0 || entries), BEMHIP_ERR_ARG, "bad tuning output");
+        bemhip_ctx* ctx = s->ctx;
+        const int solver = s->P.solver;
+        const bool batch = s->topo.probs.size() > 1 && solver == BEMHIP_SOLVE_LU16_IR64 && ctx->batch_lu;
+        std::vector<bemhip_tune_entry> cand;
+        auto add = [&](int nb, int split, int la, int reserve) {
+            cand.push_back(bemhip_tune_entry{nb, split, la, reserve, 0.0, 0, 0});
+        };
+        if (batch) {
+            for (int nb : {512, 1024})
+                for (int split : {1, 2, 3})
+                    for (int la : {1, 0}) add(nb, split, la, ctx->mm_reserve);
+        } else if (solver == BEMHIP_SOLVE_LU16_IR64) {
+            for (int nb : {768, 1024})
+                for (int r : {16, 32, 48}) add(nb, 0, -1, r);
+        } else if (solver == BEMHIP_SOLVE_LU16X3_IR64) {
+            for (int nb : {768, 1024, 1536}) add(nb, 0, -1, ctx->mm_reserve);
+        } else if (solver == BEMHIP_SOLVE_LU32_IR64) {
+            for (int nb : {256, 512, 768}) add(nb, 0, -1, ctx->mm_reserve);
+        } else if (solver == BEMHIP_SOLVE_LU64) {
+            for (int nb : {512, 768, 1024}) add(nb, 0, -1, ctx->mm_reserve);
+        }
+        // the knobs of a candidate on the context and its solver lanes
+        auto apply = [&](const bemhip_tune_entry& c) {
+            auto set = [&](Solver* sv) {
+                if (solver == BEMHIP_SOLVE_LU16_IR64) sv->nb16 = c.nb;
+                else if (solver == BEMHIP_SOLVE_LU16X3_IR64) sv->nb_split = c.nb;
+                else if (solver == BEMHIP_SOLVE_LU32_IR64) sv->nb32 = c.nb;
+                else if (solver == BEMHIP_SOLVE_LU64) sv->nb64 = c.nb;
+                sv->mm_reserve = c.reserve;
+            };
+            set(ctx);
+            for (auto& l : ctx->lanes) set(l.get());
+            if (batch) {
+                ctx->batch_split = c.split;
+                ctx->batch_lookahead = c.lookahead != 0;
+            }
+        };
+        double* hphi = dev<double>(s->hphi, s->N);
+        double* psi = s->psi.as<double>();
+        hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+        auto run = [&]() {
+            bemhip_solve_stats st{};
+            BH_HIP(hipEventRecord(e0, ctx->stream));
+            sim_psi(s, s->x.as<double>(), s->phi.as<double>(), hphi, psi, &st);
+            BH_HIP(hipEventRecord(e1, ctx->stream));
+            BH_CHECK(st.converged, BEMHIP_ERR_NOT_CONVERGED, "solver stopped above tolerance while tuning");
+            return (double)event_ms(e0, e1);
+        };
+        int best = -1;
+        if (!cand.empty()) {
+            apply(cand[0]);
+            (void)run();   // first use: workspaces sized, lanes created
+            for (size_t i = 0; i < cand.size(); ++i) {
+                apply(cand[i]);
+                const double a = run(), b = run();
+                cand[i].ms = std::min(a, b);
+                if (best < 0 || cand[i].ms < cand[best].ms) best = (int)i;
+            }
+            cand[best].chosen = 1;
+            apply(cand[best]);
+        }
+        for (int i = 0; i < std::min<int>(cap, (int)cand.size()); ++i) entries[i] = cand[i];
+        if (count) *count = (int32_t)cand.size();
+    });
+}
+
 int bemhip_sim_last_stats(bemhip_sim* s, bemhip_solve_stats* st) {
     if (!s || !st) return BEMHIP_ERR_ARG;
     *st = s->last;

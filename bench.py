@@ -132,6 +132,14 @@ def batch_bench(args):
         t0 = time.perf_counter()
         sim.evolve_system_RK4(1e-2, True)
         log(f"[bench] warmup step {w} {time.perf_counter() - t0:.3f}s")
+    tuning = None
+    if not args.no_tune:
+        # the library's tuner picks the batch schedule on the warm state
+        # (outside the timed steps)
+        t0 = time.perf_counter()
+        tuning = sim.tune()
+        kept = [e for e in tuning if e["chosen"]]
+        log(f"[bench] tuned in {time.perf_counter() - t0:.1f}s: kept {kept}")
     sim.accum_stats(reset=True)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -160,6 +168,12 @@ def batch_bench(args):
         "phases_ms_per_stage": {k: round(acc[k] / stages, 3) for k in ("ms_assemble", "ms_factor", "ms_solve")},
         "refinement_sweeps_max": acc["iterations"],
     }
+    if tuning:
+        kept = [e for e in tuning if e["chosen"]][0]
+        out["config"]["tuned"] = {"lu_block": kept["nb"], "subclasses": kept["split"],
+                                  "lookahead": kept["lookahead"]}
+        out["tuning_ms_per_compute_psi"] = {f"nb{e['nb']}_split{e['split']}_la{e['lookahead']}": round(e["ms"], 1)
+                                            for e in tuning}
     if not args.no_fp64:
         sim.compute_psi()
         psi_mixed = sim.get_psi()
@@ -339,6 +353,7 @@ def main():
     ap.add_argument("--solver", default="lu16_ir64", choices=sorted(bemhip.SOLVERS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batch: keep the default schedule (no tuner run)")
     ap.add_argument("--workload", default="sphere", choices=["sphere", "two-bubbles", "adjoint", "exterior"],
                     help="sphere: the N=40962 RK4 stage (default); two-bubbles: N=20484 coupled pair, "
                          "mixed vs FP64 over 10 steps; adjoint: loss + gradient over 50 steps at N=2562; "

@@ -185,6 +185,27 @@ int bemhip_sim_last_stats(bemhip_sim* sim, bemhip_solve_stats* stats);
  * last reset; residual / correction hold the maxima */
 int bemhip_sim_accum_stats(bemhip_sim* sim, bemhip_solve_stats* stats, int reset);
 
+/* ---- solver tuning (no counterpart in the reference) ---------------------
+ * Times the simulation's assembly + solve at its current state (best of two
+ * runs each) under every candidate setting of the block size of the solver
+ * in use, and for a batch of independent bubbles solved with lu16_ir64 also
+ * of its schedule (sub-classes per size class, look-ahead inside a class);
+ * for a single fp16 LU also the compute units left to its panel stream.  The
+ * fastest setting stays on the context (for all of its simulations), the
+ * simulation's state is unchanged (psi receives the last run's solution).
+ * entries (up to cap, may be null) receive every candidate; *count the number
+ * of candidates. */
+typedef struct {
+    int32_t nb;          /* LU outer block of the solver in use              */
+    int32_t split;       /* batch: sub-classes per size class (0: n/a)       */
+    int32_t lookahead;   /* batch: look-ahead inside a class (-1: n/a)       */
+    int32_t reserve;     /* fp16 LU: compute units left to the panel stream  */
+    double ms;           /* assembly + solve, best of two runs               */
+    int32_t chosen;      /* 1 for the setting kept                           */
+    int32_t pad_;
+} bemhip_tune_entry;
+int bemhip_sim_tune(bemhip_sim* sim, int32_t cap, bemhip_tune_entry* entries, int32_t* count);
+
 /* ---- potential off the surface ----
  * phi_ext(y) = 1/(4 pi) sum_T [ -sum_k G_k(y,T) psi_k + sum_k H_k(y,T) phi_k ]
  * for P points y (points: P x 3), with the disjoint-panel integrals under the

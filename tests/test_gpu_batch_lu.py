@@ -185,3 +185,52 @@ def test_class_lu_graph_replays_match_eager(ctx_graph, ctx):
     assert np.array_equal(got2, ref2) and rel(ref2, ref) > 1e-3
     s.close()
     e.close()
+
+
+@pytest.fixture
+def ctx_fresh():
+    c = bemhip.Context(0)
+    yield c
+    c.close()
+
+
+def test_tuner_batch_keeps_fastest_setting(ctx_fresh):
+    """bemhip_sim_tune on a batch: 12 candidate schedules (block 512 / 1024,
+    1-3 sub-classes, look-ahead on / off) timed, the fastest kept; the batch's
+    solution with the kept setting agrees with the default's to the mixed
+    solvers' 1e-8."""
+    cases = _class_cases(seed=13)
+    v, t, phi, parts, _, _ = meshgen.join_batch(cases)
+    s = bemhip.ColocSim(ctx_fresh, v, t, 1.0, 1.0, 0.0, 1.0, solver="lu16_ir64", independent_parts=True)
+    s.set_phi(phi)
+    before = s.compute_psi().copy()
+    table = s.tune()
+    for e in table:
+        say(f"  {e}")
+    assert len(table) == 12 and sum(e["chosen"] for e in table) == 1
+    kept = [e for e in table if e["chosen"]][0]
+    assert kept["ms"] == min(e["ms"] for e in table) and kept["ms"] > 0.0
+    assert {(e["nb"], e["split"], e["lookahead"]) for e in table} == \
+        {(nb, sp, la) for nb in (512, 1024) for sp in (1, 2, 3) for la in (0, 1)}
+    after = s.compute_psi().copy()
+    for (o, n) in parts:
+        assert rel(after[o:o + n], before[o:o + n]) < 1e-8
+    s.close()
+
+
+def test_tuner_single_fp16_lu(ctx_fresh, oracle_lib):
+    """bemhip_sim_tune on one bubble (N = 2562, lu16_ir64): block 768 / 1024 x
+    16 / 32 / 48 compute units for the panel stream; the kept setting solves
+    to the oracle's FP64 psi within 1e-8."""
+    v, t = meshgen.icosphere(16, 1.0)
+    phi = -1.0 + 0.05 * meshgen.random_sh_field(v, 1, 4, 3)
+    s = bemhip.ColocSim(ctx_fresh, v, t, 1.0, 1.0, 0.0, 1.0, solver="lu16_ir64")
+    s.set_phi(phi)
+    table = s.tune()
+    assert len(table) == 6 and sum(e["chosen"] for e in table) == 1
+    assert {(e["nb"], e["reserve"]) for e in table} == {(nb, r) for nb in (768, 1024) for r in (16, 32, 48)}
+    psi = s.compute_psi().copy()
+    p = oracle_lib.make_params(meshgen.Physics(1.0, 1.0, 0.0, 1.0), verts=v, tris=t)
+    ref = oracle_lib.compute_psi(p, v, t, phi)
+    assert rel(psi, ref) < 1e-8
+    s.close()
