@@ -446,7 +446,7 @@ void psi_device(bemhip_ctx* ctx, const DevTopology& topo, const double* d_xyz, c
         BH_CHECK(pr.n <= 65536, BEMHIP_ERR_ARG, "dense solve supports at most 65536 unknowns per problem");
     const size_t nn = (size_t)topo.gsize;
     double* G64 = dev<double>(ctx->G64, nn);
-    float* G32 = mixed(solver) ? dev<float>(ctx->G32, nn) : nullptr;
+    float* G32 = mixed(solver) ? dev<float>(ctx->G32, nn + MM_C_TAIL) : nullptr;
     hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
     if (topo.probs.size() > 1 && solver == BEMHIP_SOLVE_LU16_IR64 && ctx->batch_lu) {
         psi_classes(ctx, topo, d_xyz, d_phi, flags, tol, maxit, G64, G32, d_hphi, d_psi, st);
@@ -556,7 +556,7 @@ void psi_pinned(bemhip_sim* s, const double* xs, const double* pot, double* rhs,
     BH_CHECK(N <= 65536, BEMHIP_ERR_ARG, "dense solve supports at most 65536 unknowns per problem");
     const size_t nn = (size_t)N * N;
     double* G64 = dev<double>(ctx->G64, nn);
-    float* G32 = mixed(s->P.solver) ? dev<float>(ctx->G32, nn) : nullptr;
+    float* G32 = mixed(s->P.solver) ? dev<float>(ctx->G32, nn + MM_C_TAIL) : nullptr;
     double* H = dev<double>(ctx->H, nn);
     double* pz = dev<double>(s->pz, N);
     pin_zero_tail(ctx, N, np, pot, pz);
@@ -759,7 +759,7 @@ int bemhip_solve(bemhip_ctx* ctx, int64_t N, const double* G, const double* b, d
         h2d(dG, G, sizeof(double) * nn, ctx->stream);
         float* dG32 = nullptr;
         if (mixed(solver)) {
-            dG32 = dev<float>(ctx->G32, nn);
+            dG32 = dev<float>(ctx->G32, nn + MM_C_TAIL);
             k_d2f<<<4096, 256, 0, ctx->stream>>>(nn, dG, dG32);
             BH_HIP(hipGetLastError());
         }

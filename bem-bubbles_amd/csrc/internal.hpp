@@ -329,6 +329,10 @@ constexpr int MM_TILE = 256;
 constexpr int MM_BK = 64;
 constexpr int MM_SUB = 0, MM_STORE = 1;
 inline int mm_pad(int n, int q) { return (n + q - 1) / q * q; }
+// elements allocated past the end of an FP32 matrix that a trailing update
+// (MM_SUB) works on: with fewer than 4 rows its clamped C reads may touch up
+// to 3 elements past the block's last row
+constexpr size_t MM_C_TAIL = 64;
 // bytes of the tiles of R rows (or columns) x K
 inline size_t mm_tile_bytes(int R, int K) { return (size_t)mm_pad(R, MM_TILE) * mm_pad(K, MM_BK) * 2; }
 // fp16 tiles of scale * X; kcontig: X is K x R column-major (B operand), else R x K

@@ -312,14 +312,18 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     // C of a 32-column chunk is loaded while the one before is transposed and
     // written: 16 loads of 16 bytes per lane in flight
     float4 cv[2][16];
+    // Lanes whose column or rows fall outside C read a clamped position
+    // instead (never used: the epilogue skips such columns and reloads
+    // partial row groups), so that the loads need no exec mask or zero fill
+    // and the compiler places no wait before them; with fewer than 4 rows
+    // the clamped read may touch up to 3 elements past the block's rows.
+    const int rclamp = rows_ok ? row0 + rr : max(0, M - 4);
     auto load_chunk = [&](int j, float4* c) {
         if (EPI != MM_SUB) return;
 #pragma unroll
         for (int c2 = 0; c2 < 16; ++c2) {
-            const int col = col0 + j * 32 + 2 * c2 + (l >> 5);
-            const float* cc = C + (size_t)col * ldc + row0 + rr;
-            if (col < N && rows_ok) __builtin_memcpy(&c[c2], cc, 16);
-            else c[c2] = float4{0.f, 0.f, 0.f, 0.f};
+            const int col = min(col0 + j * 32 + 2 * c2 + (l >> 5), N - 1);
+            __builtin_memcpy(&c[c2], C + (size_t)col * ldc + rclamp, 16);
         }
     };
     f32x16 acc[4][4];
@@ -334,13 +338,33 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
 #pragma unroll
     for (int st = 0; st < NST - 1; ++st)
         if (st < nst) issue(st);
+    // the C loads of the epilogue's first two 32-column chunks go out right
+    // after the last stage's operand loads, so that the rest of the K loop
+    // hides their latency (32 more loads in flight behind the stages')
+    bool cpre = false;
+    if (EPI == MM_SUB && nst <= NST - 1) {
+        load_chunk(0, cv[0]);
+        cpre = true;
+    }
     for (int st = 0; st < nst; ++st) {
         const int ahead = min(NST - 2, nst - 1 - st);   // younger stages in flight (8 loads each)
-        if (ahead >= 2) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-        else if (ahead == 1) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
-        else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (cpre) {
+            if (ahead >= 2) asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
+            else if (ahead == 1) asm volatile("s_waitcnt vmcnt(24)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+        } else {
+            if (ahead >= 2) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+            else if (ahead == 1) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
         __syncthreads();
-        if (st + NST - 1 < nst) issue(st + NST - 1);
+        if (st + NST - 1 < nst) {
+            issue(st + NST - 1);
+            if (EPI == MM_SUB && st + NST - 1 == nst - 1) {
+                load_chunk(0, cv[0]);
+                cpre = true;
+            }
+        }
         const h8* sA = lds + (st % NST) * STAGE;
         const h8* sB = sA + STAGE / 2;
         // fragments double-buffered in registers: the eight LDS reads of k
@@ -382,10 +406,10 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     constexpr int CP = 132;
     __syncthreads();   // every wave is done with the staging buffers
     float* ep = reinterpret_cast<float*>(lds) + w * (32 * CP);
-    load_chunk(0, cv[0]);
+    if (EPI == MM_SUB && !cpre) load_chunk(0, cv[0]);
+    load_chunk(1, cv[1]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        if (j + 1 < 4) load_chunk(j + 1, cv[(j + 1) & 1]);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -486,6 +510,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 if (sub_tiles) *reinterpret_cast<float4*>(ep + cl * CP + rr) = float4{fv[0], fv[1], fv[2], fv[3]};
             }
         }
+        if (j + 2 < 4) load_chunk(j + 2, cv[j & 1]);   // chunk j + 2 into the registers chunk j freed
         if (sub_tiles && g.tkind == 1) {
             // A tiles of the result's rows from tr0 on: 4 row blocks x 2 k blocks
 #pragma unroll
