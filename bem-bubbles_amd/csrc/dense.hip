@@ -2205,53 +2205,69 @@ __global__ __launch_bounds__(2 * TS) void k_diag_inv(int N, const T* __restrict_
 
 // One work-group per TS-row block, taken in dependency order through a ticket
 // (so every block a work-group waits on belongs to an already running one).
-// x holds the right-hand side on entry and the solution on exit.  Block i
-// accumulates A_ij x_j over the blocks j it depends on: one wave polls up to 64
-// dependency flags at once, and the work-group then consumes every ready block
-// (up to TRSV_BATCH per round) before polling again.  Finally
-// x_i = inv_i (b_i - sum) is published and the block's flag raised.
-constexpr int TRSV_BATCH = 16;
+// b holds the right-hand side, x receives the solution and holds the marker
+// TRSV_PENDING in every entry on entry.  Block i accumulates A_ij x_j over the
+// blocks j it depends on as they are published: each round the four waves
+// read the next TRSV_POLL blocks' entries of x (readiness and values in one
+// round trip: a block is published once none of its entries is the marker),
+// then the work-group consumes the ready prefix.  Finally
+// x_i = inv_i (b_i - sum) is published with agent-scope stores.
+constexpr int TRSV_POLL = 8;
+constexpr unsigned long long TRSV_PENDING = 0x7FF4DEADBEEF0000ull;   // a NaN payload no arithmetic produces
+
+// x[q n + i] = the pending marker for every problem q with active[q] != 0
+// (active null: all) of a batch of problems of n entries
+__global__ void k_fill_pending(int n, int cnt, double* __restrict__ x, const int* __restrict__ active) {
+    const long long tot = (long long)n * cnt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long long)gridDim.x * blockDim.x)
+        if (!active || active[i / n]) x[i] = __longlong_as_double((long long)TRSV_PENDING);
+}
+static void fill_pending(hipStream_t st, int n, int cnt, double* x, const int* active) {
+    const long long tot = (long long)n * cnt;
+    k_fill_pending<<<(int)std::min<long long>(4096, (tot + 255) / 256), 256, 0, st>>>(n, cnt, x, active);
+    BH_HIP(hipGetLastError());
+}
 
 // Batched (tb.gticket set): the solves of a batch of problems in one launch;
 // the ticket names the problem (ticket / blocks) and its block in solve order,
-// problem q's factor, inverses, x (ld N) and sync words start q sA, q sInv,
-// q N, q sstride elements further, and the problems with active[q] == 0 are
-// skipped (all their blocks return at once, so nothing waits for them).
+// problem q's factor, inverses, b and x (ld N) and sync words start q sA,
+// q sInv, q N, q N, q sstride elements further, and the problems with
+// active[q] == 0 are skipped (all their blocks return at once, so nothing
+// waits for them).
 struct TrsvBat {
     unsigned int* gticket;
     long long sA, sInv;
     int sstride;
     const int* active;
-    const unsigned int* ebase;   // device base added to the launch's stamp (as k_np_top32)
 };
 template <class T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict__ A, const double* __restrict__ inv,
-                                                      double* __restrict__ x, unsigned int* __restrict__ sync,
-                                                      unsigned int epoch, TrsvBat tb) {
-    __shared__ double xs[TRSV_BATCH][TS];
+                                                      const double* __restrict__ b, double* __restrict__ x,
+                                                      unsigned int* __restrict__ sync, TrsvBat tb) {
+    __shared__ double xs[TRSV_POLL][TS];
     __shared__ double part[2][TS];
-    __shared__ int s_t, s_ready;
+    __shared__ int s_t, s_ready, s_ok[TRSV_POLL];
     const int nb = (N + TS - 1) / TS;
     const int tid = threadIdx.x;
     if (tid == 0) s_t = (int)atomicAdd(tb.gticket ? tb.gticket : sync, 1u);
     __syncthreads();
     int t = s_t;
-    if (tb.ebase) epoch += *tb.ebase;
     if (tb.gticket) {
         const int q = t / nb;
         t -= q * nb;
         if (tb.active && !tb.active[q]) return;   // the same for the whole work-group
         A += q * tb.sA;
         inv += q * tb.sInv;
+        b += (long long)q * N;
         x += (long long)q * N;
         sync += (long long)q * tb.sstride;
     }
     unsigned int* err = sync + 1;      // this launch: a work-group gave up waiting
     unsigned int* sticky = sync + 2;   // any launch since the host cleared it
-    unsigned int* flags = sync + 16;
     const int ib = LOWER ? t : nb - 1 - t;
     const int i0 = ib * TS, bs = min(TS, N - i0);
     const int r = tid & (TS - 1), h = tid >> 7;
+    const int wv = tid >> 6, ln = tid & 63;
     // the diagonal-block inverse does not depend on x: load this thread's half
     // row of it now so the final product is off the dependency chain
     // (held in the factor's precision: an FP32 factor only preconditions the
@@ -2278,56 +2294,57 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     double acc = 0.0;
     unsigned int spins = 0;
     for (int q = 0; q < t;) {
-        if (tid < 64) {
-            const int qq = q + tid;
-            bool ok = true;
-            if (qq < t) {
-                const int jb = LOWER ? qq : nb - 1 - qq;
-                ok = __hip_atomic_load(&flags[jb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+        // wave wv reads blocks q + wv and q + 4 + wv of x, two entries a lane
+#pragma unroll
+        for (int k = 0; k < TRSV_POLL / 4; ++k) {
+            const int slot = wv + 4 * k, pb = q + slot;
+            bool ok = pb < t;
+            if (ok) {
+                const int jb = LOWER ? pb : nb - 1 - pb;
+                const int c = jb * TS + 2 * ln;
+                double v0 = 0.0, v1 = 0.0;
+                if (c < N) v0 = __hip_atomic_load(&x[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (c + 1 < N) v1 = __hip_atomic_load(&x[c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = (unsigned long long)__double_as_longlong(v0) != TRSV_PENDING &&
+                     (unsigned long long)__double_as_longlong(v1) != TRSV_PENDING;
+                xs[slot][2 * ln] = v0;
+                xs[slot][2 * ln + 1] = v1;
             }
-            // the error word is read with the flags, every round: once one
-            // work-group has given up, the others stop waiting as well
-            const unsigned int e = tid == 0 ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-            const unsigned long long bal = __ballot(ok);
-            int nready = (~bal == 0ull) ? 64 : (__ffsll((long long)~bal) - 1);
-            nready = min(nready, min(t - q, TRSV_BATCH));
-            if (tid == 0) {
-                if (nready == 0 && !e) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if (++spins > (1u << 24))
-                        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    else
-                        nready = -1;   // poll again
-                }
-                if (nready == 0 || e) {
-                    // give up waiting: consume the next blocks as they stand and
-                    // finish (results invalid; the host reads the sticky word)
-                    __hip_atomic_fetch_or(sticky, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    nready = min(t - q, TRSV_BATCH);
-                }
-                s_ready = max(nready, 0);
+            ok = __ballot(!ok) == 0ull;
+            if (ln == 0) s_ok[slot] = ok;
+        }
+        // the error word, every round: once one work-group has given up, the
+        // others stop waiting as well
+        const unsigned int e = tid == 0 ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        __syncthreads();
+        if (tid == 0) {
+            int nready = 0;
+            while (nready < TRSV_POLL && q + nready < t && s_ok[nready]) ++nready;
+            if (nready == 0 && !e) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 24))
+                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    nready = -1;   // poll again
             }
+            if (nready == 0 || e) {
+                // give up waiting: consume the polled blocks as they stand and
+                // finish (results invalid; the host reads the sticky word)
+                __hip_atomic_fetch_or(sticky, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                nready = min(t - q, TRSV_POLL);
+            }
+            s_ready = max(nready, 0);
         }
         __syncthreads();
         const int nr = s_ready;
-        __syncthreads();   // every wave has read s_ready before wave 0 may overwrite it
-        if (nr == 0) continue;
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
-        for (int e = tid; e < nr * TS; e += 256) {
-            const int b = e / TS, c = e % TS;
-            const int jb = LOWER ? q + b : nb - 1 - (q + b);
-            const int col = jb * TS + c;
-            xs[b][c] = col < N ? __hip_atomic_load(&x[col], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
-        }
-        __syncthreads();
         if (r < bs) {
-            for (int b = 0; b < nr; ++b) {
-                if (pre && q + b == tl) {
+            for (int bb = 0; bb < nr; ++bb) {
+                if (pre && q + bb == tl) {
 #pragma unroll
-                    for (int c = 0; c < TS / 2; ++c) acc += (double)apre[c] * xs[b][h * (TS / 2) + c];
+                    for (int c = 0; c < TS / 2; ++c) acc += (double)apre[c] * xs[bb][h * (TS / 2) + c];
                     continue;
                 }
-                const int jb = LOWER ? q + b : nb - 1 - (q + b);
+                const int jb = LOWER ? q + bb : nb - 1 - (q + bb);
                 const int j0 = jb * TS, js = min(TS, N - j0);
                 const int c0 = h * (TS / 2), c1 = min(js, c0 + TS / 2);
                 const T* col = A + (size_t)(j0 + c0) * N + i0 + r;
@@ -2338,19 +2355,19 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
 #pragma unroll
                         for (int u = 0; u < 16; ++u) v[u] = col[(size_t)(cc + u) * N];
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) acc += (double)v[u] * xs[b][c0 + cc + u];
+                        for (int u = 0; u < 16; ++u) acc += (double)v[u] * xs[bb][c0 + cc + u];
                     }
                 } else {
-                    for (int c = c0; c < c1; ++c, col += N) acc += (double)col[0] * xs[b][c];
+                    for (int c = c0; c < c1; ++c, col += N) acc += (double)col[0] * xs[bb][c];
                 }
             }
         }
         q += nr;
-        __syncthreads();
+        __syncthreads();   // xs and s_ok are refilled by the next round
     }
     part[h][r] = acc;
     __syncthreads();
-    if (tid < TS) xs[0][tid] = tid < bs ? x[i0 + tid] - (part[0][tid] + part[1][tid]) : 0.0;
+    if (tid < TS) xs[0][tid] = tid < bs ? b[i0 + tid] - (part[0][tid] + part[1][tid]) : 0.0;
     __syncthreads();
     {
         // x_i = inv_i s: thread (r, h) sums its half of row r (entries beyond bs are
@@ -2361,13 +2378,9 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
         part[h][r] = v;
     }
     __syncthreads();
+    // publishing a block is storing its entries (each replaces the marker)
     if (tid < bs)
         __hip_atomic_store(&x[i0 + tid], part[0][tid] + part[1][tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (tid == 0) {
-        __atomic_thread_fence(__ATOMIC_RELEASE);
-        __hip_atomic_store(&flags[ib], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
@@ -2388,17 +2401,18 @@ template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
 template void lu_prepare_solve<double>(Solver*, int, const double*);
 template void lu_prepare_solve<float>(Solver*, int, const float*);
 
-template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, double* x, bool lower) {
+// x = L^-1 b (lower) or U^-1 b; x is overwritten (b and x distinct)
+template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, const double* b, double* x, bool lower) {
     const int nb = (N + TS - 1) / TS;
     unsigned int* sync = ctx->lu.flag.as<unsigned int>();
-    const unsigned int epoch = ++ctx->trsv_epoch;
+    fill_pending(ctx->stream, N, 1, x, nullptr);
     k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error word
     if (ctx->trsv_force_giveup) k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync + 1, 1, 1u);
-    const TrsvBat one{nullptr, 0, 0, 0, nullptr, nullptr};
+    const TrsvBat one{nullptr, 0, 0, 0, nullptr};
     if (lower)
-        k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), x, sync, epoch, one);
+        k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), b, x, sync, one);
     else
-        k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), x, sync, epoch, one);
+        k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), b, x, sync, one);
     BH_HIP(hipGetLastError());
 }
 
@@ -2423,8 +2437,8 @@ template <class T> void lu_solve(Solver* ctx, int N, const T* LU, const int* per
     BH_HIP(hipMemcpyAsync(tmp, x, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     k_gather<<<(N + 255) / 256, 256, 0, st>>>(N, perm, tmp, x);
     if (ctx->trsv_persistent && ctx->lu.inv_n == N) {
-        trsv_persist(ctx, N, LU, x, true);
-        trsv_persist(ctx, N, LU, x, false);
+        trsv_persist(ctx, N, LU, x, tmp, true);    // tmp = L^-1 P b
+        trsv_persist(ctx, N, LU, tmp, x, false);   // x = U^-1 tmp
         return;
     }
     for (int j0 = 0; j0 < N; j0 += TS) {
@@ -3113,7 +3127,7 @@ void cls_cols(const ClassPtrs& c, int k, int jb, int rest, int c0, int nc, int p
 }
 
 // the FP32 triangular solves of every problem (active: the refining ones) on x
-// (ld n per problem), in place
+// (ld n per problem), in place (through the class's fourth vector)
 void cls_trsv(const ClassPtrs& c, double* x, const int* active) {
     ClassWork* w = c.w;
     const int n = c.n, cnt = c.cnt, nbk = w->nbk;
@@ -3121,16 +3135,19 @@ void cls_trsv(const ClassPtrs& c, double* x, const int* active) {
     unsigned int* ts = w->tsync.as<unsigned int>();
     unsigned int* gt = w->tickets.as<unsigned int>() + 1;
     const long long sInv = (long long)nbk * TS * TS;
+    double* tt = w->vecs.as<double>() + 3 * (size_t)cnt * n;
     for (int lower = 1; lower >= 0; --lower) {
+        const double* in = lower ? x : tt;
+        double* out = lower ? tt : x;
+        fill_pending(c.main, n, cnt, out, active);
         k_cls_words<<<(cnt + 255) / 256, 256, 0, c.main>>>(cnt, ts, tss, 1, 0u, gt);
-        const TrsvBat tb{gt, c.sA, sInv, tss, active, w->tickets.as<unsigned int>() + CLS_EBASE};
-        BH_CHECK(w->trsv_epoch + 1 < CLS_EPOCH_STEP, BEMHIP_ERR_ARG, "batched fp16 LU: too many triangular solves");
+        const TrsvBat tb{gt, c.sA, sInv, tss, active};
         if (lower)
-            k_trsv_persist<float, true><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_lo.as<double>(), x, ts,
-                                                                      ++w->trsv_epoch, tb);
+            k_trsv_persist<float, true><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_lo.as<double>(), in, out, ts,
+                                                                      tb);
         else
-            k_trsv_persist<float, false><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_up.as<double>(), x, ts,
-                                                                       ++w->trsv_epoch, tb);
+            k_trsv_persist<float, false><<<nbk * cnt, 256, 0, c.main>>>(n, c.A, w->inv_up.as<double>(), in, out, ts,
+                                                                       tb);
         BH_HIP(hipGetLastError());
     }
 }
@@ -3167,7 +3184,7 @@ ClassPtrs class_ptrs(Solver* ctx, ClassWork* w, const ClassJob& cj, float* G32) 
     return c;
 }
 
-// class vectors: b, x, r (cnt x n doubles each)
+// class vectors: b, x, r (cnt x n doubles each; a fourth is cls_trsv's)
 double* cls_vec(ClassWork* w, int i) { return w->vecs.as<double>() + (size_t)i * w->cnt * w->n; }
 
 void class_work_ensure(Solver* ctx, ClassWork* w, const ClassJob& cj) {
@@ -3209,7 +3226,7 @@ void class_work_ensure(Solver* ctx, ClassWork* w, const ClassJob& cj) {
     w->t_u12.ensure(C * w->tb_a12);
     w->inv_lo.ensure(sizeof(double) * C * nbk * TS * TS);
     w->inv_up.ensure(sizeof(double) * C * nbk * TS * TS);
-    w->vecs.ensure(sizeof(double) * 3 * C * n);
+    w->vecs.ensure(sizeof(double) * 4 * C * n);   // b, x, r and the solves' intermediate
     w->part.ensure(sizeof(double) * C * w->nch * n);
     w->scal.ensure((2 * sizeof(double) + 3 * sizeof(int)) * C);
     w->v0.ensure(sizeof(int) * C);
