@@ -1419,7 +1419,7 @@ static void trailing_update(Solver* ctx, hipStream_t st, rocblas_handle h, int m
     }
     if constexpr (std::is_same<T, float>::value) {
         if (ctx->lu16)
-            mm16_gemm(st, MM_SUB, 0, ctx->lu.t_l21[p].p, k, ctx->lu.t_u12.p, k, col0, m, n, k, C, ldc,
+            mm16_gemm(st, MM_SUB, 0, ctx->lu.t_l21[p].p, k, ctx->lu.t_u12[0].p, k, col0, m, n, k, C, ldc,
                       std::ldexp(1.0f, ctx->lu16_e - 22),
                       ctx->lookahead && ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0, tiles);
         else if (split16) split_gemm(ctx, h, m, n, k, col0, C, ldc, p, slot);
@@ -1487,7 +1487,7 @@ static void lu16_cols(Solver* ctx, Lane L, int jb, int rest, int c0, int nc, flo
     const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;   // bytes of c0 / 32 column blocks
     float* a12 = A12 + (size_t)c0 * lda;
     if (!a12_tiled) mm16_pack(L.s, true, nc, jb, a12, lda, lu16_sa(ctx), ctx->lu.t_a12.as<char>() + off, log);
-    const Mm16Tiles tu{ctx->lu.t_u12.p, true, jb, c0, lu16_sa(ctx), log, nullptr};
+    const Mm16Tiles tu{ctx->lu.t_u12[0].p, true, jb, c0, lu16_sa(ctx), log, nullptr};
     mm16_gemm(L.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda,
               std::ldexp(1.0f, ctx->lu16_e - 8), 0, &tu);
     trailing_update<float>(ctx, L.s, L.h, rest, nc, jb, nullptr, 0, nullptr, 0, a12 + jb, lda, false, c0, p, 0, upd);
@@ -1858,6 +1858,163 @@ static void lu_two_threads(Solver* ctx, int N, T* A, int* ipiv, bool split16, in
     if (err) std::rethrow_exception(err);
 }
 
+// a timed block product (as trailing_update times its launches): event pair
+// and algorithmic flops recorded, read after the factorisation
+template <class F> static void timed_gemm(Solver* ctx, hipStream_t st, double flops, F&& launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        const size_t i = ctx->gemm_ev_used;
+        while (ctx->gemm_ev.size() < 2 * (i + 1)) {
+            hipEvent_t e;
+            BH_HIP(hipEventCreate(&e));
+            ctx->gemm_ev.push_back(e);
+        }
+        e0 = ctx->gemm_ev[2 * i];
+        e1 = ctx->gemm_ev[2 * i + 1];
+        ctx->gemm_fl.resize(i + 1);
+        ctx->gemm_fl[i] = flops;
+        ctx->gemm_ev_used = i + 1;
+        BH_HIP(hipEventRecord(e0, st));
+    }
+    launch();
+    if (ctx->timing) BH_HIP(hipEventRecord(e1, st));
+}
+
+// The fp16 LU with the trailing update of two panels merged into one launch
+// (K = 2 NB): the update reads and writes the trailing matrix once per pair of
+// panels.  For a pair (k, k + 1), k's own panel factored:
+//   main: U12_k of panel k+1's columns and their update (K = NB)  -> panel k+1
+//         on the side stream; the rest of U12_k; the block row of panel k+1
+//         updated by panel k alone (its NB rows);
+//   host: panel k+1's verdict;
+//   main: U12_{k+1}; panel k+2's columns updated by both panels (K = 2 NB)
+//         -> panel k+2 on the side stream; every other row and column beyond
+//         panel k+2 updated by both panels (K = 2 NB), writing the next block
+//         row's tiles.
+// Panels alternate over three buffer sets (two panels' products are read by
+// the merged update while the third is made), U12 over two.  Returns -1 when
+// a panel needed row interchanges: the caller redoes the factorisation step by
+// step from a fresh copy of the matrix (partial pivoting moves rows that the
+// deferred update of the previous panel still has to reach).
+static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& base) {
+    const int NB = ctx->nb16, lda = N;
+    const Lane main{ctx->stream, ctx->blas}, side{ctx->stream2, ctx->blas2};
+    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1];
+    unsigned int* log = ctx->lu.f16log.as<unsigned int>();
+    const float sa = lu16_sa(ctx);                              // tiles of A blocks: 2^(8 - e)
+    const float au = std::ldexp(1.0f, ctx->lu16_e - 8);         // U12 product
+    const float aup = std::ldexp(1.0f, ctx->lu16_e - 22);       // updates (L21 2^14, U12 2^(8 - e))
+    const int maxwg = ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0;
+    auto width = [&](int rem) { return std::min(NB, rem); };
+    // A tiles of L21_p from its row r (a multiple of 32) on
+    auto l21_rows = [&](int p, int kp, int r) {
+        return static_cast<const void*>(ctx->lu.t_l21[p].as<char>() + (size_t)(r / 32) * (mm_pad(kp, MM_BK) / 16) * 1024);
+    };
+    // U12 of the block row (jb rows at A12, ld lda) for columns [c0, c0 + nc),
+    // into the factor and as B tiles into t_u12[u] (from column c0)
+    auto u12 = [&](int jb, float* A12, int c0, int nc, int p, int u, bool tiled) {
+        if (nc <= 0) return;
+        float* a12 = A12 + (size_t)c0 * lda;
+        if (!tiled) {
+            const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;
+            mm16_pack(main.s, true, nc, jb, a12, lda, sa, ctx->lu.t_a12.as<char>() + off, log);
+        }
+        const Mm16Tiles tu{ctx->lu.t_u12[u].p, true, jb, c0, sa, log, nullptr};
+        mm16_gemm(main.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda, au, 0,
+                  &tu);
+    };
+    // an accepted panel's L21 into the factor (the products read its tiles)
+    auto l21_copy = [&](int m, int n, float* Ap, const float* l21, int ldl) {
+        copy2d(side.s, m - n, n, l21, ldl, Ap + n, lda);
+    };
+    int k = 0, jb = width(N), p = 0;
+    ctx->lu16_a21_ready = false;
+    const float* l21 = nullptr;
+    int ldl = 1;
+    panel_start(ctx, main, N, jb, A, lda, ipiv, 0, base, p, false);
+    if (panel_finish(ctx, main, N, jb, A, lda, ipiv, 0, base, p, false, &l21, &ldl)) return -1;
+    copy2d(main.s, N - jb, jb, l21, ldl, A + jb, lda);
+    bool a12_tiled = false;
+    for (;;) {
+        const int rest = N - k - jb;
+        if (rest <= 0) break;
+        const int jb2 = width(rest), rest2 = rest - jb2, p1 = (p + 1) % 3;
+        float* A12 = A + (size_t)(k + jb) * lda + k;   // panel k's block row (jb rows)
+        float* C1 = A12 + jb;                           // rows and columns beyond panel k
+        // panel k+1's columns: U12_k and their update, writing panel k+1's A21 tiles
+        u12(jb, A12, 0, jb2, p, 0, a12_tiled);
+        {
+            const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb2, 0, sa, log, nullptr, jb2};
+            timed_gemm(ctx, main.s, 2.0 * rest * jb2 * jb, [&] {
+                mm16_gemm(main.s, MM_SUB, 0, ctx->lu.t_l21[p].p, jb, ctx->lu.t_u12[0].p, jb, 0, rest, jb2, jb, C1, lda,
+                          aup, maxwg, rest2 > 0 ? &ta : nullptr);
+            });
+        }
+        ctx->lu16_a21_ready = rest2 > 0;
+        BH_HIP(hipEventRecord(ev_ready, main.s));
+        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+        panel_start(ctx, side, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false);
+        // the rest of U12_k, and panel k+1's block row updated by panel k
+        u12(jb, A12, jb2, rest2, p, 0, a12_tiled);
+        if (rest2 > 0) {
+            const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb2, 0, sa, log, nullptr};
+            timed_gemm(ctx, main.s, 2.0 * jb2 * rest2 * jb, [&] {
+                mm16_gemm(main.s, MM_SUB, 0, ctx->lu.t_l21[p].p, jb, ctx->lu.t_u12[0].p, jb, jb2, jb2, rest2, jb,
+                          C1 + (size_t)jb2 * lda, lda, aup, maxwg, &tb);
+            });
+        }
+        if (panel_finish(ctx, side, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false, &l21, &ldl))
+            return -1;
+        BH_HIP(hipEventRecord(ev_panel, side.s));
+        BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
+        l21_copy(rest, jb2, C1, l21, ldl);
+        if (rest2 <= 0) break;   // panel k+1 was the last
+        const int jb3 = width(rest2), rest3 = rest2 - jb3, p2 = (p + 2) % 3;
+        float* A12b = C1 + (size_t)jb2 * lda;   // panel k+1's block row (jb2 rows), its U12 next
+        float* C2 = A12b + jb2;                 // rows and columns beyond panel k+1
+        u12(jb2, A12b, 0, jb3, p1, 1, true);   // panel k+2's columns first: they are on its chain
+        // panel k+2's columns by both panels, writing panel k+2's A21 tiles
+        {
+            const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb3, 0, sa, log, nullptr, jb3};
+            const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[1].p, jb2, 0, jb2};
+            timed_gemm(ctx, main.s, 2.0 * rest2 * jb3 * (jb + jb2), [&] {
+                mm16_gemm(main.s, MM_SUB, 0, l21_rows(p, jb, jb2), jb, ctx->lu.t_u12[0].p, jb, jb2, rest2, jb3, jb, C2,
+                          lda, aup, maxwg, rest3 > 0 ? &ta : nullptr, nullptr, &sg);
+            });
+        }
+        ctx->lu16_a21_ready = rest3 > 0;
+        BH_HIP(hipEventRecord(ev_ready, main.s));
+        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
+        panel_start(ctx, side, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false);
+        u12(jb2, A12b, jb3, rest3, p1, 1, true);   // the rest of U12_{k+1}
+        // every other row and column beyond panel k+2, by both panels, writing
+        // panel k+2's block row tiles
+        a12_tiled = false;
+        if (rest3 > 0) {
+            const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, 0, sa, log, nullptr};
+            const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[1].p, jb2, jb3, jb2};
+            timed_gemm(ctx, main.s, 2.0 * rest2 * rest3 * (jb + jb2), [&] {
+                mm16_gemm(main.s, MM_SUB, 0, l21_rows(p, jb, jb2), jb, ctx->lu.t_u12[0].p, jb, jb2 + jb3, rest2, rest3,
+                          jb, C2 + (size_t)jb3 * lda, lda, aup, maxwg, &tb, nullptr, &sg);
+            });
+            a12_tiled = true;
+        }
+        if (panel_finish(ctx, side, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false, &l21,
+                         &ldl))
+            return -1;
+        BH_HIP(hipEventRecord(ev_panel, side.s));
+        BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
+        l21_copy(rest2, jb3, C2, l21, ldl);
+        k += jb + jb2;
+        jb = jb3;
+        p = p2;
+    }
+    BH_HIP(hipEventRecord(ev_panel, side.s));
+    BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
+    ctx->lu16_a21_ready = false;
+    return 0;
+}
+
 // Blocked right-looking LU with partial pivoting and one block of look-ahead.
 // Per outer step k (panel factored, its L21 and L11^-1 ready): the main stream
 // forms U12 = L11^-1 A12 and updates the next panel's columns first, hands them
@@ -1878,7 +2035,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     // work buffers at their largest (the first step's) before anything is
     // enqueued: growing one later would free memory under running kernels
     const size_t nb = (size_t)std::min(LU_NB, N);
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < 3; ++p) {
         ctx->lu.winv_l[p].ensure(sizeof(T) * nb * nb);
         ctx->lu.l21[p].ensure(sizeof(T) * (size_t)N * nb);
         if (split16) ctx->split_a[p].ensure(sizeof(__half) * 3 * (size_t)N * nb);
@@ -1904,12 +2061,12 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     if (f16) {
         ctx->lu.t_a21.ensure(mm_tile_bytes(N, (int)nb));
         ctx->lu.t_wu.ensure(mm_tile_bytes((int)nb, (int)nb));
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < 3; ++p) {
             ctx->lu.t_wl[p].ensure(mm_tile_bytes((int)nb, (int)nb));
             ctx->lu.t_l21[p].ensure(mm_tile_bytes(N, (int)nb));
         }
         ctx->lu.t_a12.ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
-        ctx->lu.t_u12.ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
+        for (int u = 0; u < 2; ++u) ctx->lu.t_u12[u].ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
         unsigned int* lg = static_cast<unsigned int*>(ctx->lu.f16log.ensure(256));
         // scale exponent from max |diag A|: its max maps to [2^7, 2^8)
         k_fill_u32<<<1, 64, 0, ctx->stream>>>(lg, 2, 0u);
@@ -1965,6 +2122,21 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     unsigned int base = 0;
     const int lda = N;
     auto width = [&](int rem) { return std::min(LU_NB, rem); };
+    bool merged = false;
+    if constexpr (std::is_same<T, float>::value) {
+        if (f16 && ctx->lu16_merge && ctx->lookahead && ctx->speculate && ctx->la_min_rest == 0 && N > LU_NB) {
+            merged = true;
+            if (lu16_pairs(ctx, N, A, ipiv, base) < 0) {
+                // a panel needed interchanges: the caller starts again step by step
+                BH_HIP(hipStreamSynchronize(ctx->stream2));
+                BH_HIP(hipStreamSynchronize(main.s));
+                collect_gemm_timing(ctx);
+                ctx->lu16_a21_ready = false;
+                return -2;
+            }
+        }
+    }
+    if (!merged) {
     // panel 0: nothing to overlap it with
     const int jb0 = width(N);
     int p = 0;
@@ -2055,6 +2227,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     }
     BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
     if (side_copy) BH_HIP(hipStreamWaitEvent(main.s, ev_copied, 0));
+    }
     }
     unsigned int flags[3] = {0, 0, 0};
     BH_HIP(hipMemcpyAsync(flags, ws, 12, hipMemcpyDeviceToHost, main.s));
@@ -2723,6 +2896,17 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
         BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "mixed solve needs the FP32 copy of G");
         ctx->lu16 = solver == BEMHIP_SOLVE_LU16_IR64;
         int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
+        if (info == -2) {
+            // merged fp16 steps met a panel that needs row interchanges: the
+            // FP32 copy again from G, and the factorisation step by step
+            k_d2f_mat<<<4096, 256, 0, sm>>>((size_t)N * N, G64, G32);
+            BH_HIP(hipGetLastError());
+            const bool keep = ctx->lu16_merge;
+            ctx->lu16_merge = false;
+            info = lu_factor<float>(ctx, N, G32, ipiv, false);
+            ctx->lu16_merge = keep;
+            ctx->lu16_unmerged += 1;
+        }
         ctx->lu16 = false;
         if (solver == BEMHIP_SOLVE_LU16_IR64 && ctx->lu16_overflow) {
             // an operand left the fp16 range: the FP32 copy again from G and
@@ -3439,6 +3623,7 @@ void Solver::init(const Solver* cfg) {
         nb64 = cfg->nb64; nb32 = cfg->nb32; nb_split = cfg->nb_split; nb16 = cfg->nb16;
         trsv_persistent = cfg->trsv_persistent;
         trsv_force_giveup = cfg->trsv_force_giveup;
+        lu16_merge = cfg->lu16_merge;
         batch_force_redo = cfg->batch_force_redo;
         mm_reserve = cfg->mm_reserve;
         timing = cfg->timing;

@@ -180,12 +180,12 @@ struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
     // per outer step: inverse of the panel's unit-lower diagonal block and its
     // L21 (two of each, alternating by step parity), inverse of U11, U12
-    DevBuf winv_l[2], l21[2], winv_u, u12;
+    DevBuf winv_l[3], l21[3], winv_u, u12;   // three parities: the fp16 LU's merged steps keep two panels
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
     // fp16 LU (mm16): fragment tiles of A21, U11^-1, L11^-1 and L21 (the last
     // two alternating by step parity), of the block row A12 and of U12; the
     // max |scaled value| of every pack (float bits)
-    DevBuf t_a21, t_wu, t_wl[2], t_l21[2], t_a12, t_u12, f16log;
+    DevBuf t_a21, t_wu, t_wl[3], t_l21[3], t_a12, t_u12[2], f16log;
     DevBuf tinv;   // work of the triangular inverses (2 tri_inv_tmp_elems(NB) elements)
 };
 
@@ -242,6 +242,11 @@ struct Solver {
     // factorisation (lu16_overflow), which the solver then redoes in FP32
     bool lu16 = false, lu16_overflow = false;
     int lu16_e = 0, lu16_redos = 0;
+    // fp16 LU: the trailing update of two panels in one launch (BEMHIP_LU16_MERGE=0:
+    // step by step); lu16_unmerged counts factorisations redone step by step
+    // because a panel needed row interchanges
+    bool lu16_merge = true;
+    int lu16_unmerged = 0;
     bool split_fixed_on = false, split_redo = false;
     int split_eb = 0, split_calls = 0, split_nlog = 0;
     int split_redos = 0;   // factorisations redone with per-block scales (lifetime count)
@@ -372,9 +377,18 @@ struct Mm16Tiles {
     unsigned long long* amax64;  // A only: max |x| (double bits)
     int r0 = 0;                  // A only: C's rows from r0 on (a multiple of 256) are the buffer's rows
 };
+// a second operand segment of an update: C -= alpha (A B + A2 B2), A2 the
+// tiles of M rows x Ka (first K of them used), B2 of Kb x (c0 + N) from column
+// c0 (a multiple of MM_TILE); the first segment's K must be a multiple of MM_BK
+struct Mm16Seg {
+    const void* At;
+    int Ka;
+    const void* Bt;
+    int Kb, c0, K;
+};
 void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
                int K, float* C, long ldc, float alpha, int max_wg = 0, const Mm16Tiles* tiles = nullptr,
-               const Mm16Bat* bat = nullptr);
+               const Mm16Bat* bat = nullptr, const Mm16Seg* seg = nullptr);
 
 // ---------------------------------------------------------- dense LA --
 // LU with partial pivoting (column-major, lda = N), pivots as absolute rows.

@@ -242,6 +242,13 @@ struct MmArgs {
     long long sAt, sBt, sC, sT;
     const int* ebat;
     int ek, ek2, ls;
+    // a second operand segment (At2 null: none): the k-steps from nks1 on
+    // (of 64 k) read A2 = At2 (KBa2 blocks per row block) and B2 = Bt2 (KBb2
+    // per column block, from column block cb02) from their k = 0, i.e. the
+    // product is A B + A2 B2 over the concatenated k range
+    const h8* At2;
+    const h8* Bt2;
+    int KBa2, KBb2, cb02, nks1;
 };
 
 template <int EPI, int TRI>
@@ -286,22 +293,30 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const int wm = w & 1, wn = w >> 1;
     const h8* abase = At + (size_t)tm * MM_RB * KBa * 64;
     const h8* bbase = Bt + (size_t)(cb0 + tn * MM_RB) * KBb * 64;
+    // the second segment's operands and its first stage (stages of 32 k)
+    const h8* abase2 = g.At2 ? g.At2 + (size_t)tm * MM_RB * g.KBa2 * 64 : abase;
+    const h8* bbase2 = g.At2 ? g.Bt2 + (size_t)(g.cb02 + tn * MM_RB) * g.KBb2 * 64 : bbase;
+    const int nst1 = g.At2 ? g.nks1 * (MM_BKB / SKB) : (1 << 30);
 
     // wave w stages blocks 4w .. 4w+3 of A and of B (block a: local row block
     // a >> 1, k block a & 1 of the stage; LDS slot a): 8 loads a lane
     auto issue = [&](int st) {
         h8* sA = lds + (st % NST) * STAGE;
         h8* sB = sA + STAGE / 2;
+        const bool s2 = st >= nst1;
+        const h8* ab = s2 ? abase2 : abase;
+        const h8* bb = s2 ? bbase2 : bbase;
+        const int kba = s2 ? g.KBa2 : KBa, kbb = s2 ? g.KBb2 : KBb, sk = (s2 ? st - nst1 : st) * SKB;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int a = w * 4 + u;
-            const h8* src = abase + ((size_t)(a >> 1) * KBa + st * SKB + (a & 1)) * 64;
+            const h8* src = ab + ((size_t)(a >> 1) * kba + sk + (a & 1)) * 64;
             lds_dma16(src + l, sA + a * 64);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int b = w * 4 + u;
-            const h8* src = bbase + ((size_t)(b >> 1) * KBb + st * SKB + (b & 1)) * 64;
+            const h8* src = bb + ((size_t)(b >> 1) * kbb + sk + (b & 1)) * 64;
             lds_dma16(src + l, sB + b * 64);
         }
     };
@@ -572,7 +587,8 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
 // looping over tiles, so that the remaining compute units stay free for
 // kernels of another stream
 void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const void* Bt, int Kb, int c0, int M, int N,
-               int K, float* C, long ldc, float alpha, int max_wg, const Mm16Tiles* tiles, const Mm16Bat* bat) {
+               int K, float* C, long ldc, float alpha, int max_wg, const Mm16Tiles* tiles, const Mm16Bat* bat,
+               const Mm16Seg* seg) {
     if (M <= 0 || N <= 0) return;
     BH_CHECK(c0 % MM_TILE == 0, BEMHIP_ERR_ARG, "mm16: column offset must be a multiple of 256");
     BH_CHECK(K <= Ka && K <= Kb, BEMHIP_ERR_ARG, "mm16: K exceeds the operand tiles");
@@ -591,6 +607,18 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
     g.M = M;
     g.N = N;
     g.nks = mm_pad(K, MM_BK) / MM_BK;
+    if (seg) {
+        BH_CHECK(epi == MM_SUB && tri == 0 && !bat, BEMHIP_ERR_ARG, "mm16: a second segment only for single updates");
+        BH_CHECK(K % MM_BK == 0 && seg->K > 0 && seg->K <= seg->Ka && seg->K <= seg->Kb && seg->c0 % MM_TILE == 0,
+                 BEMHIP_ERR_ARG, "mm16: bad second segment");
+        g.At2 = static_cast<const h8*>(seg->At);
+        g.Bt2 = static_cast<const h8*>(seg->Bt);
+        g.KBa2 = mm_pad(seg->Ka, MM_BK) / 16;
+        g.KBb2 = mm_pad(seg->Kb, MM_BK) / 16;
+        g.cb02 = seg->c0 / 32;
+        g.nks1 = g.nks;
+        g.nks += mm_pad(seg->K, MM_BK) / MM_BK;
+    }
     g.tiles_m = (M + MM_TILE - 1) / MM_TILE;
     g.tiles_n = (N + MM_TILE - 1) / MM_TILE;
     g.nbat = 1;
