@@ -1880,137 +1880,157 @@ template <class F> static void timed_gemm(Solver* ctx, hipStream_t st, double fl
     if (ctx->timing) BH_HIP(hipEventRecord(e1, st));
 }
 
-// The fp16 LU with the trailing update of two panels merged into one launch
-// (K = 2 NB): the update reads and writes the trailing matrix once per pair of
-// panels.  For a pair (k, k + 1), k's own panel factored:
-//   main: U12_k of panel k+1's columns and their update (K = NB)  -> panel k+1
-//         on the side stream; the rest of U12_k; the block row of panel k+1
-//         updated by panel k alone (its NB rows);
-//   host: panel k+1's verdict;
-//   main: U12_{k+1}; panel k+2's columns updated by both panels (K = 2 NB)
-//         -> panel k+2 on the side stream; every other row and column beyond
-//         panel k+2 updated by both panels (K = 2 NB), writing the next block
-//         row's tiles.
-// Panels alternate over three buffer sets (two panels' products are read by
-// the merged update while the third is made), U12 over two.  Returns -1 when
-// a panel needed row interchanges: the caller redoes the factorisation step by
-// step from a fresh copy of the matrix (partial pivoting moves rows that the
-// deferred update of the previous panel still has to reach).
+// The fp16 LU with the trailing updates of two panels merged into one launch
+// (K = 2 NB): the big update reads and writes the trailing matrix once per
+// pair of panels.  Three streams: S factors the panels, X runs the narrow
+// products (U12, the updates of the next panels' columns and of a panel's
+// block row), M the big merged updates.  For a pair (k, k+1), panel k
+// factored:
+//   X: U12_k of panel k+1's columns, their update (K = NB) -> S: panel k+1;
+//      once M's previous big update is done, the rest of U12_k and panel
+//      k+1's block row updated by panel k;
+//   X: after panel k+1: U12_{k+1} of panel k+2's columns, panel k+2's
+//      columns by both panels (K = 2 NB) -> S: panel k+2; the rest of
+//      U12_{k+1}; panel k+3's columns by both panels (K = 2 NB);
+//   M: every other row and column beyond panel k+3 by both panels (K = 2 NB),
+//      while S factors panels k+2 and k+3 (the next pair's first products
+//      only need panel k+3's columns, which X has updated already).
+// Panels rotate over four buffer sets, U12 over four (a big update reads two
+// panels' products while the next pair's are made).  Only the bulk updates on
+// M are timed (the gemm statistics): the narrow products belong to the chain
+// from panel to panel.  Returns -1 when a panel
+// needed row interchanges: the caller redoes the factorisation step by step
+// from a fresh copy of the matrix (partial pivoting moves rows that a deferred
+// update still has to reach).
 static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& base) {
     const int NB = ctx->nb16, lda = N;
-    const Lane main{ctx->stream, ctx->blas}, side{ctx->stream2, ctx->blas2};
-    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1];
+    const Lane lm{ctx->stream, ctx->blas}, ls{ctx->stream2, ctx->blas2}, lx{ctx->stream3, ctx->blas};
+    hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1], ev_x = ctx->ev_m[0], ev_big = ctx->ev_m[1];
     unsigned int* log = ctx->lu.f16log.as<unsigned int>();
     const float sa = lu16_sa(ctx);                              // tiles of A blocks: 2^(8 - e)
     const float au = std::ldexp(1.0f, ctx->lu16_e - 8);         // U12 product
     const float aup = std::ldexp(1.0f, ctx->lu16_e - 22);       // updates (L21 2^14, U12 2^(8 - e))
     const int maxwg = ctx->mm_reserve > 0 ? std::max(8, ctx->ncu - ctx->mm_reserve) : 0;
     auto width = [&](int rem) { return std::min(NB, rem); };
-    // A tiles of L21_p from its row r (a multiple of 32) on
+    // X and M start after everything enqueued on M so far
+    BH_HIP(hipEventRecord(ev_big, lm.s));
+    BH_HIP(hipStreamWaitEvent(lx.s, ev_big, 0));
+    BH_HIP(hipStreamWaitEvent(ls.s, ev_big, 0));
+    // A tiles of L21 of panel buffer p (k extent kp) from its row r (a multiple of 32) on
     auto l21_rows = [&](int p, int kp, int r) {
         return static_cast<const void*>(ctx->lu.t_l21[p].as<char>() + (size_t)(r / 32) * (mm_pad(kp, MM_BK) / 16) * 1024);
     };
     // U12 of the block row (jb rows at A12, ld lda) for columns [c0, c0 + nc),
-    // into the factor and as B tiles into t_u12[u] (from column c0)
+    // into the factor and as B tiles into t_u12[u] (from column c0), on X
     auto u12 = [&](int jb, float* A12, int c0, int nc, int p, int u, bool tiled) {
         if (nc <= 0) return;
         float* a12 = A12 + (size_t)c0 * lda;
         if (!tiled) {
             const size_t off = (size_t)(c0 / 32) * (mm_pad(jb, MM_BK) / 16) * 1024;
-            mm16_pack(main.s, true, nc, jb, a12, lda, sa, ctx->lu.t_a12.as<char>() + off, log);
+            mm16_pack(lx.s, true, nc, jb, a12, lda, sa, ctx->lu.t_a12.as<char>() + off, log);
         }
         const Mm16Tiles tu{ctx->lu.t_u12[u].p, true, jb, c0, sa, log, nullptr};
-        mm16_gemm(main.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda, au, 0,
+        mm16_gemm(lx.s, MM_STORE, 1, ctx->lu.t_wl[p].p, jb, ctx->lu.t_a12.p, jb, c0, jb, nc, jb, a12, lda, au, 0,
                   &tu);
     };
-    // an accepted panel's L21 into the factor (the products read its tiles)
-    auto l21_copy = [&](int m, int n, float* Ap, const float* l21, int ldl) {
-        copy2d(side.s, m - n, n, l21, ldl, Ap + n, lda);
-    };
-    int k = 0, jb = width(N), p = 0;
-    ctx->lu16_a21_ready = false;
+    // a factored panel: the verdict (host), its L21 into the factor on S, and
+    // X continues after it
     const float* l21 = nullptr;
     int ldl = 1;
-    panel_start(ctx, main, N, jb, A, lda, ipiv, 0, base, p, false);
-    if (panel_finish(ctx, main, N, jb, A, lda, ipiv, 0, base, p, false, &l21, &ldl)) return -1;
-    copy2d(main.s, N - jb, jb, l21, ldl, A + jb, lda);
+    auto finish = [&](int m, int n, float* Ap, int* ip, int roff, int p) {
+        if (panel_finish(ctx, ls, m, n, Ap, lda, ip, roff, base, p, false, &l21, &ldl)) return false;
+        BH_HIP(hipEventRecord(ev_panel, ls.s));
+        BH_HIP(hipStreamWaitEvent(lx.s, ev_panel, 0));
+        copy2d(ls.s, m - n, n, l21, ldl, Ap + n, lda);
+        return true;
+    };
+    int k = 0, jb = width(N), pk = 0;
+    ctx->lu16_a21_ready = false;
+    panel_start(ctx, ls, N, jb, A, lda, ipiv, 0, base, pk, false);
+    if (!finish(N, jb, A, ipiv, 0, pk)) return -1;
     bool a12_tiled = false;
+    bool big = false;   // a big update is on M (ev_big marks its end)
     for (;;) {
         const int rest = N - k - jb;
         if (rest <= 0) break;
-        const int jb2 = width(rest), rest2 = rest - jb2, p1 = (p + 1) % 3;
+        const int jb2 = width(rest), rest2 = rest - jb2, p1 = (pk + 1) % 4;
         float* A12 = A + (size_t)(k + jb) * lda + k;   // panel k's block row (jb rows)
         float* C1 = A12 + jb;                           // rows and columns beyond panel k
-        // panel k+1's columns: U12_k and their update, writing panel k+1's A21 tiles
-        u12(jb, A12, 0, jb2, p, 0, a12_tiled);
+        // panel k+1's columns (updated by the previous pair on X already)
+        u12(jb, A12, 0, jb2, pk, pk, a12_tiled);
         {
             const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb2, 0, sa, log, nullptr, jb2};
-            timed_gemm(ctx, main.s, 2.0 * rest * jb2 * jb, [&] {
-                mm16_gemm(main.s, MM_SUB, 0, ctx->lu.t_l21[p].p, jb, ctx->lu.t_u12[0].p, jb, 0, rest, jb2, jb, C1, lda,
-                          aup, maxwg, rest2 > 0 ? &ta : nullptr);
-            });
+            mm16_gemm(lx.s, MM_SUB, 0, ctx->lu.t_l21[pk].p, jb, ctx->lu.t_u12[pk].p, jb, 0, rest, jb2, jb, C1,
+                          lda, aup, 0, rest2 > 0 ? &ta : nullptr);
         }
         ctx->lu16_a21_ready = rest2 > 0;
-        BH_HIP(hipEventRecord(ev_ready, main.s));
-        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-        panel_start(ctx, side, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false);
-        // the rest of U12_k, and panel k+1's block row updated by panel k
-        u12(jb, A12, jb2, rest2, p, 0, a12_tiled);
+        BH_HIP(hipEventRecord(ev_ready, lx.s));
+        BH_HIP(hipStreamWaitEvent(ls.s, ev_ready, 0));
+        panel_start(ctx, ls, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false);
+        // the rest of the block row and panel k+1's block row: after M's big update
+        if (big) BH_HIP(hipStreamWaitEvent(lx.s, ev_big, 0));
+        u12(jb, A12, jb2, rest2, pk, pk, a12_tiled);
         if (rest2 > 0) {
             const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb2, 0, sa, log, nullptr};
-            timed_gemm(ctx, main.s, 2.0 * jb2 * rest2 * jb, [&] {
-                mm16_gemm(main.s, MM_SUB, 0, ctx->lu.t_l21[p].p, jb, ctx->lu.t_u12[0].p, jb, jb2, jb2, rest2, jb,
-                          C1 + (size_t)jb2 * lda, lda, aup, maxwg, &tb);
-            });
+            mm16_gemm(lx.s, MM_SUB, 0, ctx->lu.t_l21[pk].p, jb, ctx->lu.t_u12[pk].p, jb, jb2, jb2, rest2, jb,
+                          C1 + (size_t)jb2 * lda, lda, aup, 0, &tb);
         }
-        if (panel_finish(ctx, side, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false, &l21, &ldl))
-            return -1;
-        BH_HIP(hipEventRecord(ev_panel, side.s));
-        BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
-        l21_copy(rest, jb2, C1, l21, ldl);
+        if (!finish(rest, jb2, C1, ipiv + k + jb, k + jb, p1)) return -1;
         if (rest2 <= 0) break;   // panel k+1 was the last
-        const int jb3 = width(rest2), rest3 = rest2 - jb3, p2 = (p + 2) % 3;
-        float* A12b = C1 + (size_t)jb2 * lda;   // panel k+1's block row (jb2 rows), its U12 next
+        const int jb3 = width(rest2), rest3 = rest2 - jb3, p2 = (pk + 2) % 4;
+        float* A12b = C1 + (size_t)jb2 * lda;   // panel k+1's block row (jb2 rows)
         float* C2 = A12b + jb2;                 // rows and columns beyond panel k+1
-        u12(jb2, A12b, 0, jb3, p1, 1, true);   // panel k+2's columns first: they are on its chain
+        const Mm16Seg sg_n{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, 0, jb2};
+        u12(jb2, A12b, 0, jb3, p1, p1, true);
         // panel k+2's columns by both panels, writing panel k+2's A21 tiles
         {
             const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb3, 0, sa, log, nullptr, jb3};
-            const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[1].p, jb2, 0, jb2};
-            timed_gemm(ctx, main.s, 2.0 * rest2 * jb3 * (jb + jb2), [&] {
-                mm16_gemm(main.s, MM_SUB, 0, l21_rows(p, jb, jb2), jb, ctx->lu.t_u12[0].p, jb, jb2, rest2, jb3, jb, C2,
-                          lda, aup, maxwg, rest3 > 0 ? &ta : nullptr, nullptr, &sg);
-            });
+            mm16_gemm(lx.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2, rest2, jb3, jb, C2,
+                          lda, aup, 0, rest3 > 0 ? &ta : nullptr, nullptr, &sg_n);
         }
         ctx->lu16_a21_ready = rest3 > 0;
-        BH_HIP(hipEventRecord(ev_ready, main.s));
-        BH_HIP(hipStreamWaitEvent(side.s, ev_ready, 0));
-        panel_start(ctx, side, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false);
-        u12(jb2, A12b, jb3, rest3, p1, 1, true);   // the rest of U12_{k+1}
-        // every other row and column beyond panel k+2, by both panels, writing
-        // panel k+2's block row tiles
+        BH_HIP(hipEventRecord(ev_ready, lx.s));
+        BH_HIP(hipStreamWaitEvent(ls.s, ev_ready, 0));
+        panel_start(ctx, ls, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false);
+        u12(jb2, A12b, jb3, rest3, p1, p1, true);   // the rest of U12_{k+1}
         a12_tiled = false;
+        big = false;
         if (rest3 > 0) {
-            const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, 0, sa, log, nullptr};
-            const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[1].p, jb2, jb3, jb2};
-            timed_gemm(ctx, main.s, 2.0 * rest2 * rest3 * (jb + jb2), [&] {
-                mm16_gemm(main.s, MM_SUB, 0, l21_rows(p, jb, jb2), jb, ctx->lu.t_u12[0].p, jb, jb2 + jb3, rest2, rest3,
-                          jb, C2 + (size_t)jb3 * lda, lda, aup, maxwg, &tb, nullptr, &sg);
-            });
+            const int jb4 = width(rest3), rest4 = rest3 - jb4;
+            // panel k+3's columns by both panels (panel k+2's block row tiles
+            // of those columns into t_a12 from column 0)
+            {
+                const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, 0, sa, log, nullptr};
+                const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3, jb2};
+                mm16_gemm(lx.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3, rest2,
+                              jb4, jb, C2 + (size_t)jb3 * lda, lda, aup, 0, &tb, nullptr, &sg);
+            }
+            // every other row and column, by both panels, on M (the rest of
+            // panel k+2's block row tiles from column jb4 of t_a12)
+            if (rest4 > 0) {
+                BH_HIP(hipEventRecord(ev_x, lx.s));
+                BH_HIP(hipStreamWaitEvent(lm.s, ev_x, 0));
+                const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, jb4, sa, log, nullptr};
+                const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
+                timed_gemm(ctx, lm.s, 2.0 * rest2 * rest4 * (jb + jb2), [&] {
+                    mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4,
+                              rest2, rest4, jb, C2 + (size_t)(jb3 + jb4) * lda, lda, aup, maxwg, &tb, nullptr, &sg);
+                });
+                BH_HIP(hipEventRecord(ev_big, lm.s));
+                big = true;
+            }
             a12_tiled = true;
         }
-        if (panel_finish(ctx, side, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false, &l21,
-                         &ldl))
-            return -1;
-        BH_HIP(hipEventRecord(ev_panel, side.s));
-        BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
-        l21_copy(rest2, jb3, C2, l21, ldl);
+        if (!finish(rest2, jb3, C2, ipiv + k + jb + jb2, k + jb + jb2, p2)) return -1;
         k += jb + jb2;
         jb = jb3;
-        p = p2;
+        pk = p2;
     }
-    BH_HIP(hipEventRecord(ev_panel, side.s));
-    BH_HIP(hipStreamWaitEvent(main.s, ev_panel, 0));
+    // M continues after all three streams
+    BH_HIP(hipEventRecord(ev_x, lx.s));
+    BH_HIP(hipStreamWaitEvent(lm.s, ev_x, 0));
+    BH_HIP(hipEventRecord(ev_panel, ls.s));
+    BH_HIP(hipStreamWaitEvent(lm.s, ev_panel, 0));
     ctx->lu16_a21_ready = false;
     return 0;
 }
@@ -2035,7 +2055,7 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     // work buffers at their largest (the first step's) before anything is
     // enqueued: growing one later would free memory under running kernels
     const size_t nb = (size_t)std::min(LU_NB, N);
-    for (int p = 0; p < 3; ++p) {
+    for (int p = 0; p < 4; ++p) {
         ctx->lu.winv_l[p].ensure(sizeof(T) * nb * nb);
         ctx->lu.l21[p].ensure(sizeof(T) * (size_t)N * nb);
         if (split16) ctx->split_a[p].ensure(sizeof(__half) * 3 * (size_t)N * nb);
@@ -2061,12 +2081,12 @@ template <class T> int lu_factor(Solver* ctx, int N, T* A, int* ipiv, bool split
     if (f16) {
         ctx->lu.t_a21.ensure(mm_tile_bytes(N, (int)nb));
         ctx->lu.t_wu.ensure(mm_tile_bytes((int)nb, (int)nb));
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < 4; ++p) {
             ctx->lu.t_wl[p].ensure(mm_tile_bytes((int)nb, (int)nb));
             ctx->lu.t_l21[p].ensure(mm_tile_bytes(N, (int)nb));
         }
         ctx->lu.t_a12.ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
-        for (int u = 0; u < 2; ++u) ctx->lu.t_u12[u].ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
+        for (int u = 0; u < 4; ++u) ctx->lu.t_u12[u].ensure(mm_tile_bytes(N + MM_TILE, (int)nb));
         unsigned int* lg = static_cast<unsigned int*>(ctx->lu.f16log.ensure(256));
         // scale exponent from max |diag A|: its max maps to [2^7, 2^8)
         k_fill_u32<<<1, 64, 0, ctx->stream>>>(lg, 2, 0u);
@@ -3639,6 +3659,8 @@ void Solver::init(const Solver* cfg) {
     BH_BLAS(rocblas_create_handle(&blas));
     BH_BLAS(rocblas_set_stream(blas, stream));
     BH_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    BH_HIP(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
+    for (auto& e : ev_m) BH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     BH_BLAS(rocblas_create_handle(&blas2));
     BH_BLAS(rocblas_set_stream(blas2, stream2));
     BH_HIP(hipHostMalloc((void**)&host_scal, 64 * sizeof(double), hipHostMallocDefault));
@@ -3648,11 +3670,14 @@ void Solver::init(const Solver* cfg) {
 void Solver::release() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (stream2) (void)hipStreamSynchronize(stream2);
+    if (stream3) (void)hipStreamSynchronize(stream3);
+    for (auto& e : ev_m) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (auto& e : ev_la) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (auto& e : gemm_ev) if (e) (void)hipEventDestroy(e);
     gemm_ev.clear();
     if (blas2) { (void)rocblas_destroy_handle(blas2); blas2 = nullptr; }
     if (stream2) { (void)hipStreamDestroy(stream2); stream2 = nullptr; }
+    if (stream3) { (void)hipStreamDestroy(stream3); stream3 = nullptr; }
     if (blas) { (void)rocblas_destroy_handle(blas); blas = nullptr; }
     if (host_scal) { (void)hipHostFree(host_scal); host_scal = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }

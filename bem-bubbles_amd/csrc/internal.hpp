@@ -180,12 +180,12 @@ struct LuWork {
     DevBuf ipiv, panel_ws, flag, inv_lo, inv_up, backup, maxl, diag;   // maxl: max |l|, then top-block sync words
     // per outer step: inverse of the panel's unit-lower diagonal block and its
     // L21 (two of each, alternating by step parity), inverse of U11, U12
-    DevBuf winv_l[3], l21[3], winv_u, u12;   // three parities: the fp16 LU's merged steps keep two panels
+    DevBuf winv_l[4], l21[4], winv_u, u12;   // four parities: the fp16 LU's merged steps keep two panel pairs
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
     // fp16 LU (mm16): fragment tiles of A21, U11^-1, L11^-1 and L21 (the last
     // two alternating by step parity), of the block row A12 and of U12; the
     // max |scaled value| of every pack (float bits)
-    DevBuf t_a21, t_wu, t_wl[3], t_l21[3], t_a12, t_u12[2], f16log;
+    DevBuf t_a21, t_wu, t_wl[4], t_l21[4], t_a12, t_u12[4], f16log;
     DevBuf tinv;   // work of the triangular inverses (2 tri_inv_tmp_elems(NB) elements)
 };
 
@@ -197,6 +197,8 @@ struct Solver {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     hipStream_t stream2 = nullptr;      // LU look-ahead: panels of the next block column
+    hipStream_t stream3 = nullptr;      // fp16 LU with merged updates: the narrow products
+    hipEvent_t ev_m[4] = {};            // its hand-overs: narrow products done, big update done
     rocblas_handle blas2 = nullptr;
     // look-ahead only while the trailing update of a step has at least this
     // many columns besides the next panel's (0: always; measured best at

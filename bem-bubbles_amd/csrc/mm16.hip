@@ -251,7 +251,9 @@ struct MmArgs {
     int KBa2, KBb2, cb02, nks1;
 };
 
-template <int EPI, int TRI>
+// BULK: the same kernel for the LU's bulk update of two panels (persistent,
+// two operand segments), instantiated apart so that profiles list it apart
+template <int EPI, int TRI, bool BULK = false>
 __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const long ldc = g.ldc;
     const int KBa = g.KBa, KBb = g.KBb, cb0 = g.cb0, M = g.M, N = g.N, nks = g.nks;
@@ -651,7 +653,8 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
     const dim3 grid(8 * g.wg_xcd);
     if (epi == MM_SUB) {
         BH_CHECK(tri == 0, BEMHIP_ERR_ARG, "mm16: triangular operands only for stores");
-        k_mm16<MM_SUB, 0><<<grid, 256, 0, st>>>(g);
+        if (seg && max_wg > 0) k_mm16<MM_SUB, 0, true><<<grid, 256, 0, st>>>(g);
+        else k_mm16<MM_SUB, 0><<<grid, 256, 0, st>>>(g);
     } else if (tri == 1) {
         k_mm16<MM_STORE, 1><<<grid, 256, 0, st>>>(g);
     } else if (tri == 2) {

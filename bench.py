@@ -434,9 +434,11 @@ def main():
     if gms > 0:
         achieved = gf / (gms * 1e-3) / 1e12
         launches = acc["gemm_launches"]
+        merged = os.environ.get("BEMHIP_LU16_MERGE", "1") != "0"
         rl = {"bound": "mfma", "kernel": "LU trailing update " + (
-                  "bemhip::k_mm16<0,0> (fp16 MFMA, FP32 accumulation)" if args.solver == "lu16_ir64" else
-                  "(rocBLAS GEMM)"),
+                  ("bemhip::k_mm16<0,0,true> (bulk update of two panels, K = 2048" if merged else
+                   "bemhip::k_mm16<0,0> (K = 1024") + ", fp16 MFMA, FP32 accumulation)"
+                  if args.solver == "lu16_ir64" else "(rocBLAS GEMM)"),
               "achieved": round(achieved, 2), "peak": round(PEAK[args.solver], 1), "unit": "TFLOP/s",
               "frac": round(achieved / PEAK[args.solver], 4),
               "traffic": TRAFFIC_NU64.get(args.solver) if args.nu == 64 else None,
@@ -445,7 +447,7 @@ def main():
             # algorithmic bytes: C read and written in FP32, the fp16 L21 and U12
             # tiles read once, over the factorisation's update launches
             nb = int(os.environ.get("BEMHIP_NB16", "1024"))
-            shapes = lu_update_shapes(N, nb)
+            shapes = lu_update_shapes(N, nb, os.environ.get("BEMHIP_LU16_MERGE", "1") != "0")
             ab = sum(8.0 * m * n + 2.0 * (m + n) * k for m, n, k in shapes)
             fl = sum(2.0 * m * n * k for m, n, k in shapes)
             if abs(fl - gf / stages) < 1e-6 * fl:
@@ -538,21 +540,44 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def lu_update_shapes(N, nb):
-    """(m, n, k) of the fp16 LU's trailing-update launches: per step, the next
-    panel's columns first, then the rest (the look-ahead schedule)."""
+def lu_update_shapes(N, nb, merged=True):
+    """(m, n, k) of the fp16 LU's timed update launches.  Merged (the library's
+    default): per pair of panels (k, k+1) the bulk update of every row and
+    column beyond panel k+3 by both panels (K = 2 nb; the narrow products
+    before it are not timed); step by step: per panel the next panel's
+    columns, then the rest."""
     out = []
     k = 0
-    while k < N:
-        jb = min(nb, N - k)
+    w = lambda r: min(nb, r)
+    if not merged:
+        while k < N:
+            jb = w(N - k)
+            rest = N - k - jb
+            if rest <= 0:
+                break
+            jb2 = w(rest)
+            out.append((rest, jb2, jb))
+            if rest > jb2:
+                out.append((rest, rest - jb2, jb))
+            k += jb
+        return out
+    jb = w(N)
+    while True:
         rest = N - k - jb
         if rest <= 0:
             break
-        jb2 = min(nb, rest)
-        out.append((rest, jb2, jb))
-        if rest > jb2:
-            out.append((rest, rest - jb2, jb))
-        k += jb
+        jb2 = w(rest)
+        rest2 = rest - jb2
+        if rest2 <= 0:
+            break
+        jb3 = w(rest2)
+        rest3 = rest2 - jb3
+        if rest3 > 0:
+            jb4 = w(rest3)
+            if rest3 - jb4 > 0:
+                out.append((rest2, rest3 - jb4, jb + jb2))
+        k += jb + jb2
+        jb = jb3
     return out
 
 
