@@ -2433,7 +2433,14 @@ struct TrsvBat {
     int sstride;
     const int* active;
 };
-template <class T, bool LOWER>
+// TRANS: the system is the transpose of the triangle named by LOWER's
+// counterpart: LOWER solves U^T x = b (U^T is lower), !LOWER solves L^T x = b,
+// both on the factor stored in A, with inv holding the diagonal-block inverses
+// of U (resp. L) themselves (the inverse of a transposed block is the
+// transposed inverse).  Row i of U^T is column i of U, contiguous in A: a
+// half-wave reads 16 consecutive entries of each of 4 such rows per load and
+// keeps per-row partial sums, reduced over the 16 lanes once at the end.
+template <class T, bool LOWER, bool TRANS = false>
 __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict__ A, const double* __restrict__ inv,
                                                       const double* __restrict__ b, double* __restrict__ x,
                                                       unsigned int* __restrict__ sync, TrsvBat tb) {
@@ -2467,24 +2474,42 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
     // FP64 refinement; with the prefetched tile below the FP32 kernel runs one
     // work-group per compute unit, which measured faster than two with spills)
     T ivr[TS / 2];
-    {
+    if constexpr (TRANS) {
+        const double* iv = inv + (size_t)ib * TS * TS + (size_t)r * TS + h * (TS / 2);
+#pragma unroll
+        for (int c = 0; c < TS / 2; ++c) ivr[c] = (T)iv[c];
+    } else {
         const double* iv = inv + (size_t)ib * TS * TS + (size_t)(h * (TS / 2)) * TS + r;
 #pragma unroll
         for (int c = 0; c < TS / 2; ++c) ivr[c] = (T)iv[(size_t)c * TS];
     }
+    // TRANS: lane group g4 of wave wv holds rows wv * 32 + 4 rg + g4 (rg < 8)
+    // and columns u16 + 16 k (k < 8) of every block it consumes
+    const int g4 = ln >> 4, u16 = ln & 15;
     // FP32: the tile of the last block this one depends on: once that block
     // is published its product is all that is left on the dependency chain,
     // so its half row is loaded now as well (FP64 lacks the registers)
     const int tl = t - 1;                      // that block's position in the solve order
     const int jl = LOWER ? tl : nb - 1 - tl;   // and its index
-    const bool pre = sizeof(T) == 4 && t > 0 && r < bs && N - jl * TS >= TS;
+    const bool pre = sizeof(T) == 4 && t > 0 && (TRANS || r < bs) && N - jl * TS >= TS;
     T apre[TS / 2];
     if (pre) {
-        const T* col = A + (size_t)(jl * TS + h * (TS / 2)) * N + i0 + r;
+        if constexpr (TRANS) {
 #pragma unroll
-        for (int c = 0; c < TS / 2; ++c) apre[c] = col[(size_t)c * N];
+            for (int rg = 0; rg < 8; ++rg) {
+                const int rr = wv * 32 + 4 * rg + g4;
+                const T* row = A + (size_t)(i0 + rr) * N + jl * TS + u16;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) apre[8 * rg + k] = rr < bs ? row[16 * k] : (T)0;
+            }
+        } else {
+            const T* col = A + (size_t)(jl * TS + h * (TS / 2)) * N + i0 + r;
+#pragma unroll
+            for (int c = 0; c < TS / 2; ++c) apre[c] = col[(size_t)c * N];
+        }
     }
     double acc = 0.0;
+    double acct[8] = {};   // TRANS: the partial sums of rows wv * 32 + 4 rg + g4
     unsigned int spins = 0;
     for (int q = 0; q < t;) {
         // wave wv reads blocks q + wv and q + 4 + wv of x, two entries a lane
@@ -2530,7 +2555,38 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
         }
         __syncthreads();
         const int nr = s_ready;
-        if (r < bs) {
+        if constexpr (TRANS) {
+            for (int bb = 0; bb < nr; ++bb) {
+                double xv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) xv[k] = xs[bb][u16 + 16 * k];
+                if (pre && q + bb == tl) {
+#pragma unroll
+                    for (int rg = 0; rg < 8; ++rg)
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) acct[rg] += (double)apre[8 * rg + k] * xv[k];
+                    continue;
+                }
+                const int jb = LOWER ? q + bb : nb - 1 - (q + bb);
+                const int j0 = jb * TS, js = min(TS, N - j0);
+#pragma unroll
+                for (int rg = 0; rg < 8; ++rg) {
+                    const int rr = wv * 32 + 4 * rg + g4;
+                    if (rr >= bs) continue;
+                    const T* row = A + (size_t)(i0 + rr) * N + j0 + u16;
+                    if (js == TS) {
+                        T v[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) v[k] = row[16 * k];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) acct[rg] += (double)v[k] * xv[k];
+                    } else {
+                        for (int k = 0; k < 8; ++k)
+                            if (u16 + 16 * k < js) acct[rg] += (double)row[16 * k] * xv[k];
+                    }
+                }
+            }
+        } else if (r < bs) {
             for (int bb = 0; bb < nr; ++bb) {
                 if (pre && q + bb == tl) {
 #pragma unroll
@@ -2558,7 +2614,21 @@ __global__ __launch_bounds__(256) void k_trsv_persist(int N, const T* __restrict
         q += nr;
         __syncthreads();   // xs and s_ok are refilled by the next round
     }
-    part[h][r] = acc;
+    if constexpr (TRANS) {
+        // sum each row's partials over its 16 lanes (xor 1..8 stays in the group)
+#pragma unroll
+        for (int rg = 0; rg < 8; ++rg) {
+            double v = acct[rg];
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (u16 == 0) {
+                part[0][wv * 32 + 4 * rg + g4] = v;
+                part[1][wv * 32 + 4 * rg + g4] = 0.0;
+            }
+        }
+    } else {
+        part[h][r] = acc;
+    }
     __syncthreads();
     if (tid < TS) xs[0][tid] = tid < bs ? b[i0 + tid] - (part[0][tid] + part[1][tid]) : 0.0;
     __syncthreads();
@@ -2590,22 +2660,33 @@ template <class T> void lu_prepare_solve(Solver* ctx, int N, const T* LU) {
         ctx->trsv_epoch = 0;
     }
     ctx->lu.inv_n = N;
+    ctx->lu.inv_src = LU;
 }
 template void lu_prepare_solve<double>(Solver*, int, const double*);
 template void lu_prepare_solve<float>(Solver*, int, const float*);
 
-// x = L^-1 b (lower) or U^-1 b; x is overwritten (b and x distinct)
-template <class T> static void trsv_persist(Solver* ctx, int N, const T* LU, const double* b, double* x, bool lower) {
+// x = L^-1 b (lower) or U^-1 b; trans: x = U^-T b (lower) or L^-T b; x is
+// overwritten (b and x distinct)
+template <class T>
+static void trsv_persist(Solver* ctx, int N, const T* LU, const double* b, double* x, bool lower, bool trans = false) {
     const int nb = (N + TS - 1) / TS;
     unsigned int* sync = ctx->lu.flag.as<unsigned int>();
     fill_pending(ctx->stream, N, 1, x, nullptr);
     k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync, 2, 0u);   // ticket, error word
     if (ctx->trsv_force_giveup) k_fill_u32<<<1, 64, 0, ctx->stream>>>(sync + 1, 1, 1u);
     const TrsvBat one{nullptr, 0, 0, 0, nullptr};
-    if (lower)
-        k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_lo.as<double>(), b, x, sync, one);
-    else
-        k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, ctx->lu.inv_up.as<double>(), b, x, sync, one);
+    const double* il = ctx->lu.inv_lo.as<double>();
+    const double* iu = ctx->lu.inv_up.as<double>();
+    if (trans) {
+        if (lower)
+            k_trsv_persist<T, true, true><<<nb, 256, 0, ctx->stream>>>(N, LU, iu, b, x, sync, one);
+        else
+            k_trsv_persist<T, false, true><<<nb, 256, 0, ctx->stream>>>(N, LU, il, b, x, sync, one);
+    } else if (lower) {
+        k_trsv_persist<T, true><<<nb, 256, 0, ctx->stream>>>(N, LU, il, b, x, sync, one);
+    } else {
+        k_trsv_persist<T, false><<<nb, 256, 0, ctx->stream>>>(N, LU, iu, b, x, sync, one);
+    }
     BH_HIP(hipGetLastError());
 }
 
@@ -3012,8 +3093,68 @@ __global__ void k_scatter_perm(int n, const int* __restrict__ perm, const T* __r
     if (i < n) dst[perm[i]] = (double)src[i];
 }
 
+// r = b - G^T x in FP64: entry i is column i of G (contiguous) against x, a
+// wave per column; v2: 16-byte loads (N even, G and x 16-byte aligned)
+__global__ __launch_bounds__(256) void k_gemv_t(int N, const double* __restrict__ G, const double* __restrict__ x,
+                                               const double* __restrict__ b, double* __restrict__ r, int v2) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (int i = blockIdx.x * 4 + w; i < N; i += gridDim.x * 4) {
+        const double* col = G + (size_t)i * N;
+        double s0 = 0.0, s1 = 0.0;
+        if (v2) {
+            const double2* c2 = reinterpret_cast<const double2*>(col);
+            const double2* x2 = reinterpret_cast<const double2*>(x);
+            const int n2 = N >> 1;
+            int j = l;
+            for (; j + 192 < n2; j += 256) {
+                double2 a[4], y[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] = c2[j + 64 * u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) y[u] = x2[j + 64 * u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    s0 += a[u].x * y[u].x;
+                    s1 += a[u].y * y[u].y;
+                }
+            }
+            for (; j < n2; j += 64) {
+                const double2 a = c2[j], y = x2[j];
+                s0 += a.x * y.x;
+                s1 += a.y * y.y;
+            }
+        } else {
+            for (int j = l; j < N; j += 64) s0 += col[j] * x[j];
+        }
+        double s = s0 + s1;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (l == 0) r[i] = b[i] - s;
+    }
+}
+static void gemv_t_residual(Solver* ctx, int N, const double* G, const double* x, const double* b, double* r) {
+    const int v2 = (N % 2 == 0) && ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(x)) % 16 == 0);
+    k_gemv_t<<<std::min(4096, (N + 3) / 4), 256, 0, ctx->stream>>>(N, G, x, b, r, v2);
+    BH_HIP(hipGetLastError());
+}
+
+// x = G^-T b with the persistent triangular solves on the factor whose
+// diagonal-block inverses are current (FP64 vectors); false: not prepared
+template <class T>
+static bool lu_solve_t_persist(Solver* ctx, int N, const T* LU, const int* perm, const double* b, double* x) {
+    if (!ctx->trsv_persistent || ctx->lu.inv_n != N || ctx->lu.inv_src != (const void*)LU) return false;
+    double* y = static_cast<double*>(ctx->vec[8].ensure(sizeof(double) * N));
+    double* z = static_cast<double*>(ctx->vec[9].ensure(sizeof(double) * N));
+    trsv_persist(ctx, N, LU, b, y, true, true);    // y = U^-T b
+    trsv_persist(ctx, N, LU, y, z, false, true);   // z = L^-T y
+    k_scatter_perm<double><<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, perm, z, x);
+    BH_HIP(hipGetLastError());
+    return true;
+}
+
 // x = G^-T b from FP64 factors
 static void lu_solve_t64(Solver* ctx, int N, const double* LU, const int* perm, const double* b, double* x) {
+    if (lu_solve_t_persist(ctx, N, LU, perm, b, x)) return;
     hipStream_t st = ctx->stream;
     double* t = static_cast<double*>(ctx->vec[8].ensure(sizeof(double) * N));
     BH_HIP(hipMemcpyAsync(t, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
@@ -3027,6 +3168,7 @@ static void lu_solve_t64(Solver* ctx, int N, const double* LU, const int* perm, 
 
 // x = G^-T b from FP32 factors (b rounded to FP32; refinement recovers it)
 static void lu_solve_t32(Solver* ctx, int N, const float* LU, const int* perm, const double* b, double* x) {
+    if (lu_solve_t_persist(ctx, N, LU, perm, b, x)) return;
     hipStream_t st = ctx->stream;
     float* t = static_cast<float*>(ctx->vec[9].ensure(sizeof(float) * N));
     k_d2f_vec<<<(N + 255) / 256, 256, 0, st>>>(N, b, t);
@@ -3045,8 +3187,10 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
     BH_CHECK(!(ctx->small_single && N <= SMALL_MAX_N), BEMHIP_ERR_ARG,
              "transposed solves need the blocked LU's factors (BEMHIP_SMALL_SINGLE routes this size elsewhere)");
     st->iterations = 0; st->converged = 1; st->correction = 0.0; st->residual = 0.0;
+    trsv_sticky_clear(ctx);
     if (solver == BEMHIP_SOLVE_LU64) {
         lu_solve_t64(ctx, N, G64, perm, b, x);
+        trsv_sticky_check(ctx);
         return;
     }
     BH_CHECK(solver == BEMHIP_SOLVE_LU32_IR64 || solver == BEMHIP_SOLVE_LU16X3_IR64 ||
@@ -3057,14 +3201,11 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
     lu_solve_t32(ctx, N, G32, perm, b, x);
     double* r = static_cast<double*>(ctx->vec[6].ensure(sizeof(double) * N));
     double* d = static_cast<double*>(ctx->vec[5].ensure(sizeof(double) * N));
-    const double one = 1.0, mone = -1.0;
     int it = 0;
     bool conv = false;
     double cn = 0.0;
     while (it < maxit) {
-        // r = b - G^T x in FP64
-        BH_HIP(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
-        BH_BLAS(rocblas_dgemv(ctx->blas, rocblas_operation_transpose, N, N, &mone, G64, N, x, 1, &one, r, 1));
+        gemv_t_residual(ctx, N, G64, x, b, r);   // r = b - G^T x in FP64
         lu_solve_t32(ctx, N, G32, perm, r, d);
         axpby(ctx, N, 1.0, d, 1.0, x);
         ++it;
@@ -3073,6 +3214,7 @@ void solve_transposed_device(Solver* ctx, int N, const double* G64, const float*
         if (!std::isfinite(cn)) break;
         if (cn <= tol) { conv = true; break; }
     }
+    trsv_sticky_check(ctx);
     st->iterations = it;
     st->converged = conv ? 1 : 0;
     st->correction = cn;

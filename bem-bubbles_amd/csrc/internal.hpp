@@ -182,6 +182,7 @@ struct LuWork {
     // L21 (two of each, alternating by step parity), inverse of U11, U12
     DevBuf winv_l[4], l21[4], winv_u, u12;   // four parities: the fp16 LU's merged steps keep two panel pairs
     int inv_n = 0;   // N of the factor whose diagonal-block inverses are current
+    const void* inv_src = nullptr;   // and that factor
     // fp16 LU (mm16): fragment tiles of A21, U11^-1, L11^-1 and L21 (the last
     // two alternating by step parity), of the block row A12 and of U12; the
     // max |scaled value| of every pack (float bits)

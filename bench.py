@@ -37,16 +37,16 @@ import meshgen  # noqa: E402
 
 METRIC = "ms/RK4 stage (assembly+solve) at N=40962; mixed-prec vs FP64 GPU @1e-8 rel"
 PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "lu64": 78.6}   # dense MFMA TFLOP/s per algorithmic flop
-# HBM bytes per trailing-update GEMM launch at nu = 64 (average over the
-# factorisation's launches), from two rocprofv3 --pmc runs (FETCH_SIZE, then
+# HBM bytes per timed trailing-update launch at nu = 64 (fp16 LU: the bulk
+# two-panel updates; average over the factorisation's launches), from two rocprofv3 --pmc runs (FETCH_SIZE, then
 # WRITE_SIZE, each scaled by what it reads for a plain copy of known size):
 # profiles/lu16_update_gemm_nu64_hbm_traffic.md (fp16 LU),
 # profiles/split16_update_gemm_nu64_counters.md (split-fp16 LU)
-TRAFFIC_NU64 = {"lu16_ir64": 3.767e9, "lu16x3_ir64": 6.21e9}
+TRAFFIC_NU64 = {"lu16_ir64": 8.475e9, "lu16x3_ir64": 6.21e9}
 # MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
 # GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
 # profiles/lu16_update_gemm_nu64_mfma_utilisation.md, profiles/split16_update_gemm_nu64_counters.md
-MFMA_BUSY_NU64 = {"lu16_ir64": (0.252, 2.05), "lu16x3_ir64": (0.517, 1.82)}
+MFMA_BUSY_NU64 = {"lu16_ir64": (0.333, 1.99), "lu16x3_ir64": (0.517, 1.82)}
 DTYPE = {"lu16_ir64": "fp16-mfma lu (fp32 acc, fp32 factor)+f64-ir", "lu16x3_ir64": "f32-lu(split-f16 mfma)+f64-ir",
          "lu32_ir64": "f32-lu+f64-ir", "lu64": "f64", "bicgstab64": "f64"}
 
