@@ -673,6 +673,7 @@ int bemhip_create(int device, bemhip_ctx** out) {
         if (const char* v = std::getenv("BEMHIP_BATCH_SPLIT")) ctx->batch_split = std::max(1, std::atoi(v));
         const char* lm = std::getenv("BEMHIP_LU16_MERGE");
         if (lm && lm[0] == '0') ctx->lu16_merge = false;
+        if (const char* v = std::getenv("BEMHIP_LU16_BULK_FIRST")) ctx->lu16_bulk_first = std::atoll(v);
         const char* bla = std::getenv("BEMHIP_BATCH_LOOKAHEAD");
         if (bla && bla[0] == '0') ctx->batch_lookahead = false;
         const char* bg = std::getenv("BEMHIP_BATCH_GRAPH");

@@ -392,34 +392,44 @@ __device__ inline floatx16 mfma_rows32(const float* __restrict__ W, const float 
     return acc;
 }
 
-// Unit-lower inverse (lane = column c < 32 of L^-1, forward substitution) of
-// the 32 x 32 LU block in Lb (row stride NPB + 1), written transposed to
-// W[c * NPB + i] = (L^-1)[i][c] through `st`.
+// Unit-lower inverse (lane = column c < 32 of L^-1) of the 32 x 32 LU block
+// in Lb (row stride NPB + 1), written transposed to W[c * NPB + i] =
+// (L^-1)[i][c] through `st`.  Column oriented: once x[k] is final, step k
+// subtracts L[i][k] x[k] from every x[i] below it, so the 31 - k updates of a
+// step are independent and the chain from step to step is one FMA (each x[i]
+// still receives its terms in increasing k, as forward substitution adds
+// them).  The fence after each step keeps the later steps' LDS reads from
+// piling up in registers.
 template <class T, class St>
 __device__ inline void unit_lower_inverse(const T* __restrict__ Lb, int c, St st) {
     T x[NPB];
 #pragma unroll
-    for (int i = 0; i < NPB; ++i) {
-        T v = i == c ? T(1) : T(0);
+    for (int i = 0; i < NPB; ++i) x[i] = i == c ? T(1) : T(0);
 #pragma unroll
-        for (int k = 0; k < i; ++k) v -= Lb[i * (NPB + 1) + k] * x[k];
-        x[i] = v;
+    for (int k = 0; k < NPB - 1; ++k) {
+        const T xk = x[k];
+#pragma unroll
+        for (int i = k + 1; i < NPB; ++i) x[i] -= Lb[i * (NPB + 1) + k] * xk;
         step_fence(x);
     }
 #pragma unroll
     for (int i = 0; i < NPB; ++i) st(c * NPB + i, x[i]);
 }
-// Upper inverse (lane = column c < 32 of U^-1, back substitution with the
-// reciprocal pivots), written to W[i * NPB + c] = (U^-1)[i][c].
+// Upper inverse (lane = column c < 32 of U^-1, with the reciprocal pivots),
+// written to W[i * NPB + c] = (U^-1)[i][c].  Column oriented from the last
+// column: x[k] = acc[k] / u_kk is final once the columns beyond k are applied,
+// then U[i][k] x[k] leaves every acc[i] above it (independent updates).
 template <class T>
 __device__ inline void upper_inverse(const T* __restrict__ Lb, const T* __restrict__ invd, int c, T* __restrict__ W) {
     T x[NPB];
 #pragma unroll
-    for (int i = NPB - 1; i >= 0; --i) {
-        T v = i == c ? T(1) : T(0);
+    for (int i = 0; i < NPB; ++i) x[i] = i == c ? T(1) : T(0);
 #pragma unroll
-        for (int k = i + 1; k < NPB; ++k) v -= Lb[i * (NPB + 1) + k] * x[k];
-        x[i] = v * invd[i];
+    for (int k = NPB - 1; k >= 0; --k) {
+        x[k] *= invd[k];
+        const T xk = x[k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] -= Lb[i * (NPB + 1) + k] * xk;
         step_fence(x);
     }
 #pragma unroll
@@ -1633,9 +1643,23 @@ static void panel_products(Solver* ctx, Lane L, int m, int n, const T* A, int ld
 // panel_finish, which keeps the result when max |l| <= 1 (identical to GEPP's
 // choices) and otherwise restores the top block and factors the panel with
 // partial pivoting.
+// verdict set: the verdict is not read back; the largest multiplier and the
+// error word are merged into verdict (max |l| bits in words 0-1, error in
+// word 2) on the lane's stream, for the caller to read once at the end.
+__global__ void k_verdict_merge(const unsigned int* __restrict__ sync, unsigned int* __restrict__ verdict) {
+    if (threadIdx.x == 0) {
+        const unsigned long long ml = (unsigned long long)sync[0] | ((unsigned long long)sync[1] << 32);
+        const unsigned long long vm = (unsigned long long)verdict[0] | ((unsigned long long)verdict[1] << 32);
+        if (ml > vm) {
+            verdict[0] = (unsigned int)ml;
+            verdict[1] = (unsigned int)(ml >> 32);
+        }
+        verdict[2] |= sync[3];
+    }
+}
 template <class T>
 static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base,
-                        int p, bool split16) {
+                        int p, bool split16, unsigned int* verdict = nullptr) {
     if (!ctx->speculate) {
         panel_rec(ctx, L, m, n, A, lda, ipiv, roff, base);
         panel_products(ctx, L, m, n, A, lda, A + n, lda, p, split16, nullptr);
@@ -1648,6 +1672,11 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
     T* l21 = ctx->lu.l21[p].as<T>();
     panel_np(ctx, L, m, n, A, lda, sync, l21, p);
     panel_products(ctx, L, m, n, A, lda, l21, m - n, p, split16, reinterpret_cast<unsigned long long*>(sync));
+    if (verdict) {
+        k_verdict_merge<<<1, 64, 0, L.s>>>(sync, verdict);
+        BH_HIP(hipGetLastError());
+        return;
+    }
     BH_HIP(hipMemcpyAsync(ctx->host_scal + 32, sync, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, L.s));
     BH_HIP(hipEventRecord(ctx->ev_la[2], L.s));
 }
@@ -1658,7 +1687,7 @@ static void panel_start(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* i
 // over) or A itself.
 template <class T>
 static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* ipiv, int roff, unsigned int& base,
-                         int p, bool split16, const T** l21, int* ldl) {
+                         int p, bool split16, const T** l21, int* ldl, bool deferred = false) {
     ctx->panels += 1;
     *l21 = A + n;
     *ldl = lda;
@@ -1666,10 +1695,15 @@ static bool panel_finish(Solver* ctx, Lane L, int m, int n, T* A, int lda, int* 
         ctx->panels_pivoted += 1;
         return true;
     }
-    BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
-    const double ml = ctx->host_scal[32];
-    BH_CHECK(reinterpret_cast<const unsigned int*>(ctx->host_scal + 33)[1] == 0u, BEMHIP_ERR_HIP,
-             "panel top-block factorisation gave up waiting");
+    double ml = 0.0;
+    if (!deferred) {
+        // (deferred: the panel is taken as accepted; the caller checks the
+        // merged verdict at the end and redoes the factorisation otherwise)
+        BH_HIP(hipEventSynchronize(ctx->ev_la[2]));
+        ml = ctx->host_scal[32];
+        BH_CHECK(reinterpret_cast<const unsigned int*>(ctx->host_scal + 33)[1] == 0u, BEMHIP_ERR_HIP,
+                 "panel top-block factorisation gave up waiting");
+    }
     if (ml <= 1.0) {
         k_iota<<<(n + 255) / 256, 256, 0, L.s>>>(ipiv, n, roff);
         BH_HIP(hipGetLastError());
@@ -1906,6 +1940,7 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
     const int NB = ctx->nb16, lda = N;
     const Lane lm{ctx->stream, ctx->blas}, ls{ctx->stream2, ctx->blas2}, lx{ctx->stream3, ctx->blas};
     hipEvent_t ev_ready = ctx->ev_la[0], ev_panel = ctx->ev_la[1], ev_x = ctx->ev_m[0], ev_big = ctx->ev_m[1];
+    hipEvent_t ev_b1 = ctx->ev_m[2];
     unsigned int* log = ctx->lu.f16log.as<unsigned int>();
     const float sa = lu16_sa(ctx);                              // tiles of A blocks: 2^(8 - e)
     const float au = std::ldexp(1.0f, ctx->lu16_e - 8);         // U12 product
@@ -1938,7 +1973,7 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
     const float* l21 = nullptr;
     int ldl = 1;
     auto finish = [&](int m, int n, float* Ap, int* ip, int roff, int p) {
-        if (panel_finish(ctx, ls, m, n, Ap, lda, ip, roff, base, p, false, &l21, &ldl)) return false;
+        if (panel_finish(ctx, ls, m, n, Ap, lda, ip, roff, base, p, false, &l21, &ldl, true)) return false;
         BH_HIP(hipEventRecord(ev_panel, ls.s));
         BH_HIP(hipStreamWaitEvent(lx.s, ev_panel, 0));
         copy2d(ls.s, m - n, n, l21, ldl, Ap + n, lda);
@@ -1946,10 +1981,17 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
     };
     int k = 0, jb = width(N), pk = 0;
     ctx->lu16_a21_ready = false;
-    panel_start(ctx, ls, N, jb, A, lda, ipiv, 0, base, pk, false);
+    // the panels' verdicts are merged on the device and read once at the end
+    // (no host round trip per panel); a panel that needed interchanges makes
+    // the whole factorisation a redo either way
+    unsigned int* verdict = static_cast<unsigned int*>(ctx->lu.verdict.ensure(16));
+    k_fill_u32<<<1, 64, 0, ls.s>>>(verdict, 4, 0u);
+    panel_start(ctx, ls, N, jb, A, lda, ipiv, 0, base, pk, false, verdict);
     if (!finish(N, jb, A, ipiv, 0, pk)) return -1;
     bool a12_tiled = false;
-    bool big = false;   // a big update is on M (ev_big marks its end)
+    // the previous pair's bulk update is on M in two launches: the rows of the
+    // next two panels (ev_b1 marks its end), then the rows beyond (ev_big)
+    bool big = false;
     for (;;) {
         const int rest = N - k - jb;
         if (rest <= 0) break;
@@ -1966,9 +2008,10 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
         ctx->lu16_a21_ready = rest2 > 0;
         BH_HIP(hipEventRecord(ev_ready, lx.s));
         BH_HIP(hipStreamWaitEvent(ls.s, ev_ready, 0));
-        panel_start(ctx, ls, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false);
-        // the rest of the block row and panel k+1's block row: after M's big update
-        if (big) BH_HIP(hipStreamWaitEvent(lx.s, ev_big, 0));
+        panel_start(ctx, ls, rest, jb2, C1, lda, ipiv + k + jb, k + jb, base, p1, false, verdict);
+        // the rest of the block row and panel k+1's block row: their rows are
+        // the first part of the previous pair's bulk update
+        if (big) BH_HIP(hipStreamWaitEvent(lx.s, ev_b1, 0));
         u12(jb, A12, jb2, rest2, pk, pk, a12_tiled);
         if (rest2 > 0) {
             const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb2, 0, sa, log, nullptr};
@@ -1981,44 +2024,70 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
         float* A12b = C1 + (size_t)jb2 * lda;   // panel k+1's block row (jb2 rows)
         float* C2 = A12b + jb2;                 // rows and columns beyond panel k+1
         const Mm16Seg sg_n{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, 0, jb2};
-        u12(jb2, A12b, 0, jb3, p1, p1, true);
-        // panel k+2's columns by both panels, writing panel k+2's A21 tiles
-        {
+        // every column beyond panel k+3, by both panels, on M: first the rows
+        // of panels k+2 and k+3 (with panel k+2's block row tiles of those
+        // columns, from column jb4 of t_a12), then the rows beyond; it needs
+        // nothing from X beyond U12_{k+1}
+        auto bulk = [&] {
+            BH_HIP(hipEventRecord(ev_x, lx.s));
+            const int jb4 = rest3 > 0 ? width(rest3) : 0, rest4 = rest3 - jb4;
+            if (rest3 <= 0 || rest4 <= 0) return;
+            BH_HIP(hipStreamWaitEvent(lm.s, ev_x, 0));
+            const int r1 = jb3 + jb4;
+            const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, jb4, sa, log, nullptr};
+            const Mm16Seg sg1{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
+            timed_gemm(ctx, lm.s, 2.0 * r1 * rest4 * (jb + jb2), [&] {
+                mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4, r1,
+                          rest4, jb, C2 + (size_t)r1 * lda, lda, aup, maxwg, &tb, nullptr, &sg1);
+            });
+            BH_HIP(hipEventRecord(ev_b1, lm.s));
+            const Mm16Seg sg2{l21_rows(p1, jb2, r1), jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
+            timed_gemm(ctx, lm.s, 2.0 * rest4 * rest4 * (jb + jb2), [&] {
+                mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2 + r1), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4,
+                          rest4, rest4, jb, C2 + r1 + (size_t)r1 * lda, lda, aup, maxwg, nullptr, nullptr, &sg2);
+            });
+            BH_HIP(hipEventRecord(ev_big, lm.s));
+            big = true;
+        };
+        // panel k+2's columns by both panels (writing panel k+2's A21 tiles),
+        // then panel k+2 on S; its rows beyond panel k+1 are the previous bulk
+        // update's second launch
+        auto narrow2 = [&] {
             const Mm16Tiles ta{ctx->lu.t_a21.p, false, jb3, 0, sa, log, nullptr, jb3};
             mm16_gemm(lx.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2, rest2, jb3, jb, C2,
                           lda, aup, 0, rest3 > 0 ? &ta : nullptr, nullptr, &sg_n);
-        }
-        ctx->lu16_a21_ready = rest3 > 0;
-        BH_HIP(hipEventRecord(ev_ready, lx.s));
-        BH_HIP(hipStreamWaitEvent(ls.s, ev_ready, 0));
-        panel_start(ctx, ls, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false);
-        u12(jb2, A12b, jb3, rest3, p1, p1, true);   // the rest of U12_{k+1}
-        a12_tiled = false;
+            ctx->lu16_a21_ready = rest3 > 0;
+            BH_HIP(hipEventRecord(ev_ready, lx.s));
+            BH_HIP(hipStreamWaitEvent(ls.s, ev_ready, 0));
+            panel_start(ctx, ls, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false, verdict);
+        };
+        u12(jb2, A12b, 0, jb3, p1, p1, true);
+        // while the trailing matrix is large the bulk updates are the longest
+        // chain: U12_{k+1} is finished first so that M goes on at once; later
+        // the panels are, and panel k+2 comes first
+        const bool bulk_first = (long long)rest2 >= ctx->lu16_bulk_first;
+        const bool had_big = big;
         big = false;
+        if (bulk_first) {
+            u12(jb2, A12b, jb3, rest3, p1, p1, true);   // the rest of U12_{k+1}
+            if (had_big) BH_HIP(hipStreamWaitEvent(lx.s, ev_big, 0));
+            bulk();
+            narrow2();
+        } else {
+            if (had_big) BH_HIP(hipStreamWaitEvent(lx.s, ev_big, 0));
+            narrow2();
+            u12(jb2, A12b, jb3, rest3, p1, p1, true);
+            bulk();
+        }
+        a12_tiled = false;
         if (rest3 > 0) {
-            const int jb4 = width(rest3), rest4 = rest3 - jb4;
+            const int jb4 = width(rest3);
             // panel k+3's columns by both panels (panel k+2's block row tiles
             // of those columns into t_a12 from column 0)
-            {
-                const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, 0, sa, log, nullptr};
-                const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3, jb2};
-                mm16_gemm(lx.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3, rest2,
-                              jb4, jb, C2 + (size_t)jb3 * lda, lda, aup, 0, &tb, nullptr, &sg);
-            }
-            // every other row and column, by both panels, on M (the rest of
-            // panel k+2's block row tiles from column jb4 of t_a12)
-            if (rest4 > 0) {
-                BH_HIP(hipEventRecord(ev_x, lx.s));
-                BH_HIP(hipStreamWaitEvent(lm.s, ev_x, 0));
-                const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, jb4, sa, log, nullptr};
-                const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
-                timed_gemm(ctx, lm.s, 2.0 * rest2 * rest4 * (jb + jb2), [&] {
-                    mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4,
-                              rest2, rest4, jb, C2 + (size_t)(jb3 + jb4) * lda, lda, aup, maxwg, &tb, nullptr, &sg);
-                });
-                BH_HIP(hipEventRecord(ev_big, lm.s));
-                big = true;
-            }
+            const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, 0, sa, log, nullptr};
+            const Mm16Seg sg{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3, jb2};
+            mm16_gemm(lx.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3, rest2, jb4, jb,
+                          C2 + (size_t)jb3 * lda, lda, aup, 0, &tb, nullptr, &sg);
             a12_tiled = true;
         }
         if (!finish(rest2, jb3, C2, ipiv + k + jb + jb2, k + jb + jb2, p2)) return -1;
@@ -2032,6 +2101,12 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
     BH_HIP(hipEventRecord(ev_panel, ls.s));
     BH_HIP(hipStreamWaitEvent(lm.s, ev_panel, 0));
     ctx->lu16_a21_ready = false;
+    BH_HIP(hipMemcpyAsync(ctx->host_scal + 34, verdict, 16, hipMemcpyDeviceToHost, ls.s));
+    BH_HIP(hipStreamSynchronize(ls.s));
+    BH_CHECK(reinterpret_cast<const unsigned int*>(ctx->host_scal + 35)[0] == 0u, BEMHIP_ERR_HIP,
+             "panel top-block factorisation gave up waiting");
+    const double ml = ctx->host_scal[34];
+    if (!(ml <= 1.0)) return -1;
     return 0;
 }
 
@@ -2996,10 +3071,14 @@ void solve_device(Solver* ctx, int N, double* G64, float* G32, const double* b, 
                solver == BEMHIP_SOLVE_LU16_IR64) {
         BH_CHECK(G32 != nullptr, BEMHIP_ERR_ARG, "mixed solve needs the FP32 copy of G");
         ctx->lu16 = solver == BEMHIP_SOLVE_LU16_IR64;
+        const int panels0 = ctx->panels, pivoted0 = ctx->panels_pivoted;
         int info = lu_factor<float>(ctx, N, G32, ipiv, solver == BEMHIP_SOLVE_LU16X3_IR64);
         if (info == -2) {
             // merged fp16 steps met a panel that needs row interchanges: the
-            // FP32 copy again from G, and the factorisation step by step
+            // FP32 copy again from G, and the factorisation step by step (the
+            // discarded pass's panels are not counted)
+            ctx->panels = panels0;
+            ctx->panels_pivoted = pivoted0;
             k_d2f_mat<<<4096, 256, 0, sm>>>((size_t)N * N, G64, G32);
             BH_HIP(hipGetLastError());
             const bool keep = ctx->lu16_merge;
@@ -3786,6 +3865,7 @@ void Solver::init(const Solver* cfg) {
         trsv_persistent = cfg->trsv_persistent;
         trsv_force_giveup = cfg->trsv_force_giveup;
         lu16_merge = cfg->lu16_merge;
+        lu16_bulk_first = cfg->lu16_bulk_first;
         batch_force_redo = cfg->batch_force_redo;
         mm_reserve = cfg->mm_reserve;
         timing = cfg->timing;

@@ -542,10 +542,11 @@ def main():
 
 def lu_update_shapes(N, nb, merged=True):
     """(m, n, k) of the fp16 LU's timed update launches.  Merged (the library's
-    default): per pair of panels (k, k+1) the bulk update of every row and
-    column beyond panel k+3 by both panels (K = 2 nb; the narrow products
-    before it are not timed); step by step: per panel the next panel's
-    columns, then the rest."""
+    default): per pair of panels (k, k+1) the bulk update of every column
+    beyond panel k+3 by both panels (K = 2 nb) in two launches, the rows of
+    panels k+2 and k+3 first, then the rows beyond (the narrow products are
+    not timed); step by step: per panel the next panel's columns, then the
+    rest."""
     out = []
     k = 0
     w = lambda r: min(nb, r)
@@ -574,8 +575,10 @@ def lu_update_shapes(N, nb, merged=True):
         rest3 = rest2 - jb3
         if rest3 > 0:
             jb4 = w(rest3)
-            if rest3 - jb4 > 0:
-                out.append((rest2, rest3 - jb4, jb + jb2))
+            rest4 = rest3 - jb4
+            if rest4 > 0:
+                out.append((jb3 + jb4, rest4, jb + jb2))
+                out.append((rest4, rest4, jb + jb2))
         k += jb + jb2
         jb = jb3
     return out

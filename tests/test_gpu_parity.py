@@ -61,7 +61,9 @@ def test_lu_accepted_then_pivoted_panels(ctx):
     """Columns [0, 1536) diagonally dominant, the rest random: the first panels
     are accepted without interchanges and later ones fall back to partial
     pivoting, whose row swaps reach into L21 blocks that the second stream
-    copied into the factor."""
+    copied into the factor.  The fp16 LU's merged schedule checks its panels'
+    verdicts once at the end and redoes the factorisation step by step, where
+    the same per-panel fallback applies (the discarded pass is not counted)."""
     say("lu accepted then pivoted N=3000")
     rng = np.random.default_rng(11)
     n = 3000
@@ -69,7 +71,7 @@ def test_lu_accepted_then_pivoted_panels(ctx):
     A[np.arange(1536), np.arange(1536)] += 100.0
     b = rng.standard_normal(n)
     ref = np.linalg.solve(A, b)
-    for solver in ("lu64", "lu32_ir64", "lu16x3_ir64"):
+    for solver in ("lu64", "lu32_ir64", "lu16x3_ir64", "lu16_ir64"):
         x, st = ctx.solve(A, b, solver)
         say(f"  {solver}: panels {st['panels']} pivoted {st['panels_pivoted']}")
         assert rel(x, ref) < 1e-10
