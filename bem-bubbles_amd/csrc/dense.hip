@@ -436,6 +436,58 @@ __device__ inline void upper_inverse(const T* __restrict__ Lb, const T* __restri
     for (int i = 0; i < NPB; ++i) W[i * NPB + c] = x[i];
 }
 
+// The same two inverses from the block stored by columns (LT[k * NPB + i] =
+// entry (i, k), 16-byte aligned, in LDS): a step reads its whole column with
+// eight 16-byte loads and waits for them once, not for one load per FMA.
+__device__ inline void lds_col(const float* LT, int k, float (&v)[NPB]) {
+    const float4* p = reinterpret_cast<const float4*>(LT + k * NPB);
+#pragma unroll
+    for (int q = 0; q < NPB / 4; ++q) {
+        const float4 t = p[q];
+        v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+    }
+}
+// (LT is not __restrict__: the fences must keep its reads in their steps;
+// x[k] is pinned once final, or the compiler keeps the steps' products
+// apart and spills)
+template <class St>
+__device__ inline void unit_lower_inverse_cols(const float* LT, int c, St st) {
+    float x[NPB];
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) x[i] = i == c ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NPB - 1; ++k) {
+        float cur[NPB];   // column k
+        lds_col(LT, k, cur);
+        step_fence(cur);
+        const float xk = x[k];
+#pragma unroll
+        for (int i = k + 1; i < NPB; ++i) x[i] -= cur[i] * xk;
+        __asm__ volatile("" : "+v"(x[k + 1]));
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) st(c * NPB + i, x[i]);
+}
+__device__ inline void upper_inverse_cols(const float* LT, const float* __restrict__ invd, int c,
+                                          float* __restrict__ W) {
+    float x[NPB];
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) x[i] = i == c ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = NPB - 1; k >= 0; --k) {
+        float cur[NPB];   // column k
+        lds_col(LT, k, cur);
+        step_fence(cur);
+        x[k] *= invd[k];
+        __asm__ volatile("" : "+v"(x[k]));
+        const float xk = x[k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] -= cur[i] * xk;
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) W[i * NPB + c] = x[i];
+}
+
 // FP32 row block in slot h of wave wv (nw waves), or -1 past the block
 __device__ inline int top_rb(int wv, int h, int nw, int n) {
     const int rb = wv + h * nw;
@@ -769,6 +821,7 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
     __shared__ float Wt[NPB * NPB];      // published L^-1 of stripe j (transposed) / own U^-1
     __shared__ float Xs[NPB][NPB + 1];   // a row block of this stripe handed between waves: Xs[i][c]
     __shared__ float Ld[NPB][NPB + 1];   // own factored diagonal block
+    __shared__ __align__(16) float LdT[NPB * NPB];   // the same by columns, for the inverses
     __shared__ float invd[NPB];
     __shared__ unsigned long long red[MAXW];
     __shared__ int s_stripe, s_go;
@@ -786,6 +839,8 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
         linv += q * slinv;
     }
     unsigned int* err = sync + 3;
+    // flags[j]: stripe j's L^-1 and the multipliers of row block j + 1 are
+    // published; flags[32 + j]: all of its multipliers are
     unsigned int* flags = sync + 16;
     const int c0 = s * NPB, w = min(NPB, n - c0);
     float* S = A + (size_t)c0 * lda;
@@ -816,33 +871,43 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
         }
     };
     unsigned long long lbits = 0;
+    // wait for flag f of this launch (lane 0 spins, bounded; false: the wait
+    // gave up or another work-group did, the error word is raised)
+    auto wait_flag = [&](const unsigned int* fl) {
+        int go = 1;
+        unsigned int spins = 0;
+        while (__hip_atomic_load(fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1u << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                go = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return go;
+    };
     for (int j = 0; j < s; ++j) {
         const int r0 = j * NPB;
-        if (tid == 0) {
-            int go = 1;
-            unsigned int spins = 0;
-            while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1u << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    go = 0;
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            s_go = go;
-        }
+        if (tid == 0) s_go = wait_flag(&flags[j]);
         __syncthreads();
         if (!s_go) return;   // error raised: results invalid, every work-group leaves
         const float* Lj = A + (size_t)r0 * lda;   // stripe j (published)
-        float xb[TOP_RBW][16];
+        // first row block j + 1 alone (its multipliers come with the first
+        // flag); the step before this stripe's own leaves the rows below its
+        // diagonal block for while that block is factored, the others apply
+        // the step to them after the second flag
+        const bool last = j == s - 1;
+        // the slot of row block j + 1 in this wave (-1: another wave holds it)
+        int h1 = -1;
 #pragma unroll
-        for (int h = 0; h < TOP_RBW; ++h) {
-            const int rb = top_rb(wv, h, nw, n), i = min(max(rb, 0) * NPB + li, n - 1);
-            const bool act = rb > j;
+        for (int h = 0; h < TOP_RBW; ++h)
+            if (top_rb(wv, h, nw, n) == j + 1) h1 = h;
+        float xb[16];
+        {
+            const int i = min((j + 1) * NPB + li, n - 1);
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk)
-                xb[h][kk] = act ? ld_agent(Lj + (size_t)(2 * kk + kh) * lda + i) : 0.0f;
+            for (int kk = 0; kk < 16; ++kk) xb[kk] = h1 >= 0 ? ld_agent(Lj + (size_t)(2 * kk + kh) * lda + i) : 0.0f;
         }
         for (int e = tid; e < NPB * NPB; e += nt) Wt[e] = ld_agent(linv + (size_t)j * NPB * NPB + e);
         hand_over(j);
@@ -861,51 +926,101 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
             }
         }
         __syncthreads();
-        // rank-32 update of this stripe's row blocks below, in registers
+        // rank-32 update of this stripe's row blocks below, in registers:
+        // row block j + 1 now
+        if (h1 >= 0) {
+            const floatx16 acc = mfma_rows32(Us, xb, lane);
+#pragma unroll
+            for (int h = 0; h < TOP_RBW; ++h)
+                if (h == h1) sv[h] -= acc;
+        }
+        if (last) break;   // the last step's Us stays for the deferred rows
+        // the rows below it once stripe j's other multipliers are published
+        if (tid == 0) s_go = wait_flag(&flags[32 + j]);
+        __syncthreads();
+        if (!s_go) return;
 #pragma unroll
         for (int h = 0; h < TOP_RBW; ++h) {
             const int rb = top_rb(wv, h, nw, n);
-            if (rb > j) {
-                const floatx16 acc = mfma_rows32(Us, xb[h], lane);
+            if (rb > j + 1) {
+                const int i = min(rb * NPB + li, n - 1);
+                float xv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) xv[kk] = ld_agent(Lj + (size_t)(2 * kk + kh) * lda + i);
+                const floatx16 acc = mfma_rows32(Us, xv, lane);
                 sv[h] -= acc;
             }
         }
         __syncthreads();   // Us / Wt / Xs are refilled by the next step
     }
+    // the deferred rows: the last step (stripe s - 1, its U rows still in Us)
+    // applied to this wave's row blocks below the diagonal one, multipliers
+    // read again (they were published before that step's flag)
+    auto deferred = [&] {
+        if (s == 0) return;
+        // stripe s - 1's multipliers below row block s (this wave waits for
+        // them itself; a wait that gives up leaves the rows as they are, the
+        // error word is raised)
+        if (!__builtin_amdgcn_readfirstlane(lane == 0 ? wait_flag(&flags[32 + s - 1]) : 0)) return;
+        const float* Lj = A + (size_t)(s - 1) * NPB * lda;
+#pragma unroll
+        for (int h = 0; h < TOP_RBW; ++h) {
+            const int rb = top_rb(wv, h, nw, n);
+            if (rb > s) {
+                const int i = min(rb * NPB + li, n - 1);
+                float xv[16];
+#pragma unroll
+                for (int kk = 0; kk < 16; ++kk) xv[kk] = ld_agent(Lj + (size_t)(2 * kk + kh) * lda + i);
+                const floatx16 acc = mfma_rows32(Us, xv, lane);
+                sv[h] -= acc;
+            }
+        }
+    };
     // own diagonal block: its row block from the holding wave, factored by
-    // wave 0 in registers (pivot row by v_readlane)
+    // wave 0 in registers (pivot row by v_readlane); meanwhile the waves on
+    // the other three SIMDs (waves are dealt to SIMDs round robin) take the
+    // deferred rows, the ones on wave 0's SIMD (and wave 0) during the
+    // inverses below
     hand_over(s);
     __syncthreads();
+    if (tid >= 64 && (wv & 3) != 0) deferred();
     if (tid < 64) {
         const bool row = lane < w;
         float rw[NPB];
         unsigned int fbits = 0;
+        float myinv = 0.0f;   // lane j: the reciprocal of pivot j (written to LDS once at the end)
 #pragma unroll
         for (int k = 0; k < NPB; ++k) rw[k] = (row && k < w) ? Xs[lane & 31][k] : 0.0f;
+        auto pivot_step = [&](int j) {
+            const float piv = lane_value(rw[j], j);
+            const float inv = __builtin_amdgcn_rcpf(piv);
+            const bool below = row && lane > j;
+            const float l = below ? rw[j] * inv : 0.0f;
+            if (below) rw[j] = l;
+            const unsigned int lb = __float_as_uint(fabsf(l));
+            fbits = lb > fbits ? lb : fbits;
+            float pk[NPB];
 #pragma unroll
-        for (int j = 0; j < NPB; ++j) {
-            if (j < w) {
-                const float piv = lane_value(rw[j], j);
-                const float inv = __builtin_amdgcn_rcpf(piv);
-                const bool below = row && lane > j;
-                const float l = below ? rw[j] * inv : 0.0f;
-                if (below) rw[j] = l;
-                const unsigned int lb = __float_as_uint(fabsf(l));
-                fbits = lb > fbits ? lb : fbits;
-                float pk[NPB];
+            for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
 #pragma unroll
-                for (int k = j + 1; k < NPB; ++k) pk[k] = lane_value(rw[k], j);
+            for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
+            myinv = lane == j ? inv : myinv;
+        };
+        if (w == NPB) {   // a full stripe: no per-step test
 #pragma unroll
-                for (int k = j + 1; k < NPB; ++k) rw[k] -= l * pk[k];
-                if (lane == j) invd[j] = inv;
-            } else if (lane == j) {
-                invd[j] = 0.0f;
-            }
+            for (int j = 0; j < NPB; ++j) pivot_step(j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NPB; ++j)
+                if (j < w) pivot_step(j);
         }
         if (lane < NPB) {
+            invd[lane] = myinv;
 #pragma unroll
             for (int k = 0; k < NPB; ++k) {
-                Ld[lane][k] = (row && k < w) ? rw[k] : 0.0f;
+                const float v = (row && k < w) ? rw[k] : 0.0f;
+                Ld[lane][k] = v;
+                LdT[k * NPB + lane] = v;
                 if (row && k < w) st_agent(&S[(size_t)k * lda + c0 + lane], rw[k]);
             }
         }
@@ -916,32 +1031,42 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
     __syncthreads();
     const bool below = c0 + NPB < n;
     {
-        const float* Lb = &Ld[0][0] + opaque_zero();
+        const float* LT = LdT + opaque_zero();
         if (wv == (1 % nw) && lane < NPB && w == NPB)
-            unit_lower_inverse(Lb, lane, [&](int e, float v) { st_agent(linv + (size_t)s * NPB * NPB + e, v); });
-        if (wv == (2 % nw) && lane < NPB && below) upper_inverse(Lb, invd, lane, Wt);
+            unit_lower_inverse_cols(LT, lane, [&](int e, float v) { st_agent(linv + (size_t)s * NPB * NPB + e, v); });
+        if (wv == (2 % nw) && lane < NPB && below) upper_inverse_cols(LT, invd, lane, Wt);
     }
+    if ((wv & 3) == 0) deferred();
     __syncthreads();
-    // multipliers of the row blocks below: L = A U^-1, the row operand from registers
+    // multipliers of the row blocks below: L = A U^-1, the row operand from
+    // registers; row block s + 1 first, published with the first flag by
+    // the wave that holds it (wave 0 when there is none)
     unsigned int fb = 0;
-#pragma unroll
-    for (int h = 0; h < TOP_RBW; ++h) {
+    auto mult = [&](int h) {
         const int rb = top_rb(wv, h, nw, n);
-        if (below && rb > s) {
-            const int i = rb * NPB + li;
-            const floatx16 acc = mfma_rows32p(Wt, sv[h], lane);
+        const int i = rb * NPB + li;
+        const floatx16 acc = mfma_rows32p(Wt, sv[h], lane);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int c = 8 * (q >> 2) + 4 * kh + (q & 3);
-                if (i < n) {
-                    const float lq = acc[q];
-                    st_agent(S + (size_t)c * lda + i, lq);
-                    const unsigned int lb = __float_as_uint(fabsf(lq));
-                    fb = lb > fb ? lb : fb;
-                }
+        for (int q = 0; q < 16; ++q) {
+            const int c = 8 * (q >> 2) + 4 * kh + (q & 3);
+            if (i < n) {
+                const float lq = acc[q];
+                st_agent(S + (size_t)c * lda + i, lq);
+                const unsigned int lb = __float_as_uint(fabsf(lq));
+                fb = lb > fb ? lb : fb;
             }
         }
+    };
+#pragma unroll
+    for (int h = 0; h < TOP_RBW; ++h)
+        if (below && top_rb(wv, h, nw, n) == s + 1) mult(h);
+    if (wv == (below ? (s + 1) % nw : 0) && lane == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+#pragma unroll
+    for (int h = 0; h < TOP_RBW; ++h)
+        if (below && top_rb(wv, h, nw, n) > s + 1) mult(h);
     const unsigned long long fbd = fb > 0x7F800000u ? 0x7FF8000000000000ull
                                                     : (unsigned long long)__double_as_longlong((double)__uint_as_float(fb));
     lbits = fbd > lbits ? fbd : lbits;
@@ -957,7 +1082,7 @@ __global__ __launch_bounds__(512) void k_np_top32(int n, float* __restrict__ A, 
         for (int q = 1; q < nw; ++q) bits = red[q] > bits ? red[q] : bits;
         atomicMax(reinterpret_cast<unsigned long long*>(sync), bits);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(&flags[s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&flags[32 + s], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
