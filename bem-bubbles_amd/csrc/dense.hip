@@ -2153,23 +2153,35 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
         // of panels k+2 and k+3 (with panel k+2's block row tiles of those
         // columns, from column jb4 of t_a12), then the rows beyond; it needs
         // nothing from X beyond U12_{k+1}
+        // while the trailing matrix is large the bulk updates are the longest
+        // chain: U12_{k+1} is finished first so that M goes on at once; later
+        // the panels are, and panel k+2 comes first
+        const bool bulk_first = (long long)rest2 >= ctx->lu16_bulk_first;
         auto bulk = [&] {
             BH_HIP(hipEventRecord(ev_x, lx.s));
             const int jb4 = rest3 > 0 ? width(rest3) : 0, rest4 = rest3 - jb4;
             if (rest3 <= 0 || rest4 <= 0) return;
             BH_HIP(hipStreamWaitEvent(lm.s, ev_x, 0));
-            const int r1 = jb3 + jb4;
+            // once the panels are the longer chain, nothing waits for the
+            // first part alone: one launch over all the rows
+            const int r1 = bulk_first ? jb3 + jb4 : rest2;
             const Mm16Tiles tb{ctx->lu.t_a12.p, true, jb3, jb4, sa, log, nullptr};
             const Mm16Seg sg1{ctx->lu.t_l21[p1].p, jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
             timed_gemm(ctx, lm.s, 2.0 * r1 * rest4 * (jb + jb2), [&] {
                 mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4, r1,
-                          rest4, jb, C2 + (size_t)r1 * lda, lda, aup, maxwg, &tb, nullptr, &sg1);
+                          rest4, jb, C2 + (size_t)(jb3 + jb4) * lda, lda, aup, maxwg, &tb, nullptr, &sg1);
             });
             BH_HIP(hipEventRecord(ev_b1, lm.s));
+            if (r1 == rest2) {
+                BH_HIP(hipEventRecord(ev_big, lm.s));
+                big = true;
+                return;
+            }
             const Mm16Seg sg2{l21_rows(p1, jb2, r1), jb2, ctx->lu.t_u12[p1].p, jb2, jb3 + jb4, jb2};
             timed_gemm(ctx, lm.s, 2.0 * rest4 * rest4 * (jb + jb2), [&] {
                 mm16_gemm(lm.s, MM_SUB, 0, l21_rows(pk, jb, jb2 + r1), jb, ctx->lu.t_u12[pk].p, jb, jb2 + jb3 + jb4,
-                          rest4, rest4, jb, C2 + r1 + (size_t)r1 * lda, lda, aup, maxwg, nullptr, nullptr, &sg2);
+                          rest4, rest4, jb, C2 + r1 + (size_t)(jb3 + jb4) * lda, lda, aup, maxwg, nullptr, nullptr,
+                          &sg2);
             });
             BH_HIP(hipEventRecord(ev_big, lm.s));
             big = true;
@@ -2187,10 +2199,6 @@ static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& bas
             panel_start(ctx, ls, rest2, jb3, C2, lda, ipiv + k + jb + jb2, k + jb + jb2, base, p2, false, verdict);
         };
         u12(jb2, A12b, 0, jb3, p1, p1, true);
-        // while the trailing matrix is large the bulk updates are the longest
-        // chain: U12_{k+1} is finished first so that M goes on at once; later
-        // the panels are, and panel k+2 comes first
-        const bool bulk_first = (long long)rest2 >= ctx->lu16_bulk_first;
         const bool had_big = big;
         big = false;
         if (bulk_first) {

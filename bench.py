@@ -447,7 +447,8 @@ def main():
             # algorithmic bytes: C read and written in FP32, the fp16 L21 and U12
             # tiles read once, over the factorisation's update launches
             nb = int(os.environ.get("BEMHIP_NB16", "1024"))
-            shapes = lu_update_shapes(N, nb, os.environ.get("BEMHIP_LU16_MERGE", "1") != "0")
+            shapes = lu_update_shapes(N, nb, os.environ.get("BEMHIP_LU16_MERGE", "1") != "0",
+                                      int(os.environ.get("BEMHIP_LU16_BULK_FIRST", "25000")))
             ab = sum(8.0 * m * n + 2.0 * (m + n) * k for m, n, k in shapes)
             fl = sum(2.0 * m * n * k for m, n, k in shapes)
             if abs(fl - gf / stages) < 1e-6 * fl:
@@ -540,13 +541,13 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def lu_update_shapes(N, nb, merged=True):
+def lu_update_shapes(N, nb, merged=True, bulk_first=25000):
     """(m, n, k) of the fp16 LU's timed update launches.  Merged (the library's
     default): per pair of panels (k, k+1) the bulk update of every column
-    beyond panel k+3 by both panels (K = 2 nb) in two launches, the rows of
-    panels k+2 and k+3 first, then the rows beyond (the narrow products are
-    not timed); step by step: per panel the next panel's columns, then the
-    rest."""
+    beyond panel k+3 by both panels (K = 2 nb); while at least `bulk_first`
+    rows are left after panel k+1 in two launches, the rows of panels k+2 and
+    k+3 first, then the rows beyond, later in one (the narrow products are not
+    timed); step by step: per panel the next panel's columns, then the rest."""
     out = []
     k = 0
     w = lambda r: min(nb, r)
@@ -577,8 +578,11 @@ def lu_update_shapes(N, nb, merged=True):
             jb4 = w(rest3)
             rest4 = rest3 - jb4
             if rest4 > 0:
-                out.append((jb3 + jb4, rest4, jb + jb2))
-                out.append((rest4, rest4, jb + jb2))
+                if rest2 >= bulk_first:
+                    out.append((jb3 + jb4, rest4, jb + jb2))
+                    out.append((rest4, rest4, jb + jb2))
+                else:
+                    out.append((rest2, rest4, jb + jb2))
         k += jb + jb2
         jb = jb3
     return out
