@@ -799,6 +799,12 @@ __device__ inline floatx16 mfma_rows32p(const float* __restrict__ W, const float
 // step.  A row block that another wave needs (the step's own row block, the
 // diagonal block) goes through LDS.  Only final values are stored: the U rows
 // of every step, the factored diagonal block and the multipliers below it.
+// Stripe j publishes under two flags: its L^-1 and the multipliers of row
+// block j + 1 (all that stripe j + 1 needs before factoring its diagonal
+// block), then its other multipliers.  Stripe j + 1 applies step j to its
+// diagonal row block alone, factors it, and applies step j to its other row
+// blocks while that runs; the stripes further right apply step j to their
+// row blocks below j + 1 after the second flag.
 // backup (optional, ld n) receives the block's original values.
 // Batched (gticket set): one launch factors the top blocks of a batch of
 // problems, `stripes` stripes each; the ticket taken from gticket names the
