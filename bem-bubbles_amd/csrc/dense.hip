@@ -2045,28 +2045,32 @@ template <class F> static void timed_gemm(Solver* ctx, hipStream_t st, double fl
     if (ctx->timing) BH_HIP(hipEventRecord(e1, st));
 }
 
-// The fp16 LU with the trailing updates of two panels merged into one launch
-// (K = 2 NB): the big update reads and writes the trailing matrix once per
-// pair of panels.  Three streams: S factors the panels, X runs the narrow
-// products (U12, the updates of the next panels' columns and of a panel's
-// block row), M the big merged updates.  For a pair (k, k+1), panel k
-// factored:
+// The fp16 LU with the trailing updates of two panels merged (K = 2 NB): the
+// bulk update reads and writes the trailing matrix once per pair of panels.
+// Three streams: S factors the panels, X runs the narrow products (U12, the
+// updates of the next panels' columns and of a panel's block row), M the bulk
+// updates.  For a pair (k, k+1), panel k factored:
 //   X: U12_k of panel k+1's columns, their update (K = NB) -> S: panel k+1;
-//      once M's previous big update is done, the rest of U12_k and panel
-//      k+1's block row updated by panel k;
-//   X: after panel k+1: U12_{k+1} of panel k+2's columns, panel k+2's
-//      columns by both panels (K = 2 NB) -> S: panel k+2; the rest of
-//      U12_{k+1}; panel k+3's columns by both panels (K = 2 NB);
-//   M: every other row and column beyond panel k+3 by both panels (K = 2 NB),
-//      while S factors panels k+2 and k+3 (the next pair's first products
-//      only need panel k+3's columns, which X has updated already).
-// Panels rotate over four buffer sets, U12 over four (a big update reads two
+//      once the rows of panels k and k+1 have the previous pair's bulk
+//      update, the rest of U12_k and panel k+1's block row updated by panel
+//      k; after panel k+1, U12_{k+1} of panel k+2's columns;
+//   then, in either order (below), the rest of U12_{k+1} and, once the
+//      previous bulk update is done, panel k+2's columns by both panels
+//      (K = 2 NB) -> S: panel k+2; last panel k+3's columns by both panels;
+//   M: every column beyond panel k+3 by both panels (K = 2 NB), while S
+//      factors panels k+2 and k+3.  While at least lu16_bulk_first rows are
+//      left, M is the longest chain: U12_{k+1} is finished first, and the
+//      bulk update is two launches, the rows of panels k+2 and k+3 first
+//      (all that the next pair's U12 and block row wait for), then the rows
+//      beyond; later panel k+2's columns come first and the bulk update is
+//      one launch.
+// Panels rotate over four buffer sets, U12 over four (a bulk update reads two
 // panels' products while the next pair's are made).  Only the bulk updates on
 // M are timed (the gemm statistics): the narrow products belong to the chain
-// from panel to panel.  Returns -1 when a panel
-// needed row interchanges: the caller redoes the factorisation step by step
-// from a fresh copy of the matrix (partial pivoting moves rows that a deferred
-// update still has to reach).
+// from panel to panel.  The panels' verdicts are merged on the device and read
+// once at the end; returns -1 when a panel needed row interchanges: the caller
+// redoes the factorisation step by step from a fresh copy of the matrix
+// (partial pivoting moves rows that a deferred update still has to reach).
 static int lu16_pairs(Solver* ctx, int N, float* A, int* ipiv, unsigned int& base) {
     const int NB = ctx->nb16, lda = N;
     const Lane lm{ctx->stream, ctx->blas}, ls{ctx->stream2, ctx->blas2}, lx{ctx->stream3, ctx->blas};
