@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "internal.hpp"
 
@@ -211,10 +212,14 @@ void mm16_pack(hipStream_t st, bool kcontig, int R, int K, const float* X, long 
 // is lower triangular (tile row tm only needs k < 256 (tm + 1)); TRI = 2: B is
 // upper triangular (tile column tn only needs k < 256 (tn + 1)).
 //
-// A work-group (4 waves) owns a 256 x 256 tile of C; wave (wm, wn) a 128 x 128
-// quarter as 4 x 4 MFMA 32x32 accumulators (256 FP32 per lane).  Per stage of
-// 32 k the work-group stages 16 KB of A and 16 KB of B in LDS by direct-to-LDS
-// loads, four stages deep.  The launch covers
+// A work-group (4 waves) owns a 256 x TN tile of C; wave (wm, wn) a 128 x TN/2
+// quarter as 4 x TN/64 MFMA 32x32 accumulators (TN FP32 per lane).  Per stage
+// of 32 k the work-group stages 16 KB of A and TN/16 KB of B in LDS by
+// direct-to-LDS loads, four stages deep (TN = 256, one work-group per compute
+// unit) or three (TN = 128: two work-groups share a compute unit, two waves a
+// SIMD, so that one work-group's waits and epilogue overlap the other's
+// MFMAs; about twice the rate per compute unit where the launch does not fill
+// the chip, as the LU's narrow products beside the bulk update).  The launch covers
 // the tiles in groups of 8 tile rows, and consecutive work-group ids (which
 // the hardware deals round-robin to the 8 XCDs) are remapped so that every XCD
 // works through a contiguous range of groups: the work-groups sharing an L2
@@ -253,17 +258,40 @@ struct MmArgs {
 
 // BULK: the same kernel for the LU's bulk update of two panels (persistent,
 // two operand segments), instantiated apart so that profiles list it apart
-template <int EPI, int TRI, bool BULK = false>
-__global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
+// s_waitcnt vmcnt with a count known only at run time (one of the values the
+// stage schedules below produce)
+__device__ inline void vm_wait(int n) {
+    switch (n) {
+    case 40: asm volatile("s_waitcnt vmcnt(40)" ::: "memory"); break;
+    case 32: asm volatile("s_waitcnt vmcnt(32)" ::: "memory"); break;
+    case 24: asm volatile("s_waitcnt vmcnt(24)" ::: "memory"); break;
+    case 16: asm volatile("s_waitcnt vmcnt(16)" ::: "memory"); break;
+    case 12: asm volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+    case 8: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+    case 6: asm volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+    default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
+template <int EPI, int TRI, bool BULK = false, int TN = 256>
+__global__ __launch_bounds__(256, TN == 256 ? 1 : 2) void k_mm16(const MmArgs g) {
+    static_assert(TN == 256 || TN == 128, "k_mm16: tiles of 256 x 256 or 256 x 128");
     const long ldc = g.ldc;
     const int KBa = g.KBa, KBb = g.KBb, cb0 = g.cb0, M = g.M, N = g.N, nks = g.nks;
     const int tiles_m = g.tiles_m, tiles_n = g.tiles_n, per_xcd = g.per_xcd, wg_xcd = g.wg_xcd;
     // four stages of [A 16 KB | B 16 KB], 32 k each: three stages' loads are
     // in flight while one is computed (two 64-k stages left the next stage's
-    // loads a single stage of compute to arrive in, which they often missed)
+    // loads a single stage of compute to arrive in, which they often missed);
+    // TN = 128: three stages of [A 16 KB | B 8 KB], 72 KB per work-group
     constexpr int SKB = 2;                          // 16-k blocks per stage
-    constexpr int NST = 4;                          // stages
-    constexpr int STAGE = 2 * MM_RB * SKB * 64;     // h8 per stage
+    constexpr int NST = TN == 256 ? 4 : 3;          // stages
+    constexpr int NJ = TN / 64;                     // accumulator columns per wave
+    constexpr int ABL = MM_RB * SKB, BBL = (TN / 32) * SKB;   // 1 KB blocks per stage
+    constexpr int STAGE = (ABL + BBL) * 64;         // h8 per stage
+    constexpr int LPS = (ABL + BBL) / 4;            // loads per wave and stage
+    // the epilogue's first C chunk is loaded behind the last stage (TN = 256;
+    // with two work-groups a compute unit the registers are not there)
+    constexpr bool CPRE = TN == 256;
     __shared__ h8 lds[NST * STAGE];
     const int xcd = blockIdx.x & 7;
     float tmax = 0.0f;    // max |tscale x| of the fused tiles
@@ -287,24 +315,25 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     const int tm = fm + q % gs, tn = q / gs;
     int kend = nks;
     if (TRI == 1) kend = min(nks, (tm + 1) * (MM_TILE / MM_BK));
-    if (TRI == 2) kend = min(nks, (tn + 1) * (MM_TILE / MM_BK));
+    if (TRI == 2) kend = min(nks, (tn + 1) * (TN / MM_BK));
     const int nst = kend * (MM_BKB / SKB);          // stages of this tile
     // the wave index as a scalar: every staging address is then a scalar base
     // plus the lane's 16 bytes
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
     const int wm = w & 1, wn = w >> 1;
     const h8* abase = At + (size_t)tm * MM_RB * KBa * 64;
-    const h8* bbase = Bt + (size_t)(cb0 + tn * MM_RB) * KBb * 64;
+    const h8* bbase = Bt + (size_t)(cb0 + tn * (TN / 32)) * KBb * 64;
     // the second segment's operands and its first stage (stages of 32 k)
     const h8* abase2 = g.At2 ? g.At2 + (size_t)tm * MM_RB * g.KBa2 * 64 : abase;
-    const h8* bbase2 = g.At2 ? g.Bt2 + (size_t)(g.cb02 + tn * MM_RB) * g.KBb2 * 64 : bbase;
+    const h8* bbase2 = g.At2 ? g.Bt2 + (size_t)(g.cb02 + tn * (TN / 32)) * g.KBb2 * 64 : bbase;
     const int nst1 = g.At2 ? g.nks1 * (MM_BKB / SKB) : (1 << 30);
 
-    // wave w stages blocks 4w .. 4w+3 of A and of B (block a: local row block
-    // a >> 1, k block a & 1 of the stage; LDS slot a): 8 loads a lane
+    // wave w stages blocks 4w .. 4w+3 of A and TN/64 blocks of B (block a:
+    // local row block a >> 1, k block a & 1 of the stage; LDS slot a): LPS
+    // loads a lane
     auto issue = [&](int st) {
         h8* sA = lds + (st % NST) * STAGE;
-        h8* sB = sA + STAGE / 2;
+        h8* sB = sA + ABL * 64;
         const bool s2 = st >= nst1;
         const h8* ab = s2 ? abase2 : abase;
         const h8* bb = s2 ? bbase2 : bbase;
@@ -316,19 +345,20 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
             lds_dma16(src + l, sA + a * 64);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int b = w * 4 + u;
+        for (int u = 0; u < NJ; ++u) {
+            const int b = w * NJ + u;
             const h8* src = bb + ((size_t)(b >> 1) * kbb + sk + (b & 1)) * 64;
             lds_dma16(src + l, sB + b * 64);
         }
     };
     // the epilogue's view of this wave's 128 x 128 quarter of C
-    const int row0 = tm * MM_TILE + wm * 128, col0 = tn * MM_TILE + wn * 128;
+    const int row0 = tm * MM_TILE + wm * 128, col0 = tn * TN + wn * (TN / 2);
     const int rr = 4 * (l & 31);   // this lane's 4 rows (of 128) when reading back
     const bool rows_ok = row0 + rr + 3 < M;
     // C of a 32-column chunk is loaded while the one before is transposed and
     // written: 16 loads of 16 bytes per lane in flight
-    float4 cv[2][16];
+    constexpr int NCV = CPRE ? 2 : 1;   // C chunks in registers
+    float4 cv[NCV][16];
     // Lanes whose column or rows fall outside C read a clamped position
     // instead (never used: the epilogue skips such columns and reloads
     // partial row groups), so that the loads need no exec mask or zero fill
@@ -343,11 +373,11 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
             __builtin_memcpy(&c[c2], C + (size_t)col * ldc + rclamp, 16);
         }
     };
-    f32x16 acc[4][4];
+    f32x16 acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x16{};
+        for (int j = 0; j < NJ; ++j) acc[i][j] = f32x16{};
     // Per stage: wait for its loads (the younger stages' may stay in flight)
     // and for every wave to be done with the oldest slot, start the loads of
     // the stage three ahead into it, then read this stage's fragments and run
@@ -359,39 +389,31 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     // after the last stage's operand loads, so that the rest of the K loop
     // hides their latency (32 more loads in flight behind the stages')
     bool cpre = false;
-    if (EPI == MM_SUB && nst <= NST - 1) {
+    if (CPRE && EPI == MM_SUB && nst <= NST - 1) {
         load_chunk(0, cv[0]);
         cpre = true;
     }
     for (int st = 0; st < nst; ++st) {
-        const int ahead = min(NST - 2, nst - 1 - st);   // younger stages in flight (8 loads each)
-        if (cpre) {
-            if (ahead >= 2) asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
-            else if (ahead == 1) asm volatile("s_waitcnt vmcnt(24)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-        } else {
-            if (ahead >= 2) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-            else if (ahead == 1) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
-            else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        }
+        const int ahead = min(NST - 2, nst - 1 - st);   // younger stages in flight (LPS loads each)
+        vm_wait(LPS * ahead + (cpre ? 16 : 0));
         __syncthreads();
         if (st + NST - 1 < nst) {
             issue(st + NST - 1);
-            if (EPI == MM_SUB && st + NST - 1 == nst - 1) {
+            if (CPRE && EPI == MM_SUB && st + NST - 1 == nst - 1) {
                 load_chunk(0, cv[0]);
                 cpre = true;
             }
         }
         const h8* sA = lds + (st % NST) * STAGE;
-        const h8* sB = sA + STAGE / 2;
+        const h8* sB = sA + ABL * 64;
         // fragments double-buffered in registers: the eight LDS reads of k
         // block s + 1 are in flight while block s's 16 MFMAs run (read one
         // at a time, each read's latency stalled the single wave of a SIMD)
-        h8 fa[2][4], fb[2][4];
+        h8 fa[2][4], fb[2][NJ];
 #pragma unroll
         for (int i = 0; i < 4; ++i) fa[0][i] = sA[((wm * 4 + i) * SKB) * 64 + l];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) fb[0][j] = sB[((wn * 4 + j) * SKB) * 64 + l];
+        for (int j = 0; j < NJ; ++j) fb[0][j] = sB[((wn * NJ + j) * SKB) * 64 + l];
 #pragma unroll
         for (int s = 0; s < SKB; ++s) {
             const int cur = s & 1, nxt = cur ^ 1;
@@ -399,7 +421,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) fa[nxt][i] = sA[((wm * 4 + i) * SKB + s + 1) * 64 + l];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) fb[nxt][j] = sB[((wn * 4 + j) * SKB + s + 1) * 64 + l];
+                for (int j = 0; j < NJ; ++j) fb[nxt][j] = sB[((wn * NJ + j) * SKB + s + 1) * 64 + l];
             }
             // keep the reads ahead of the MFMAs (the scheduler would otherwise
             // move each read down to its first use)
@@ -407,7 +429,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < NJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[cur][i], fb[cur][j], acc[i][j], 0, 0, 0);
         }
     }
@@ -424,9 +446,9 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     __syncthreads();   // every wave is done with the staging buffers
     float* ep = reinterpret_cast<float*>(lds) + w * (32 * CP);
     if (EPI == MM_SUB && !cpre) load_chunk(0, cv[0]);
-    load_chunk(1, cv[1]);
+    if constexpr (NCV == 2) load_chunk(1, cv[1]);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NJ; ++j) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -501,7 +523,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
             if (rows_ok) {
                 float4 v;
                 if (EPI == MM_SUB) {
-                    v = cv[j & 1][c2];
+                    v = cv[j % NCV][c2];
                     v.x -= alpha * a.x;
                     v.y -= alpha * a.y;
                     v.z -= alpha * a.z;
@@ -527,7 +549,7 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
                 if (sub_tiles) *reinterpret_cast<float4*>(ep + cl * CP + rr) = float4{fv[0], fv[1], fv[2], fv[3]};
             }
         }
-        if (j + 2 < 4) load_chunk(j + 2, cv[j & 1]);   // chunk j + 2 into the registers chunk j freed
+        if (j + NCV < NJ) load_chunk(j + NCV, cv[j % NCV]);   // the next chunk into the registers chunk j freed
         if (sub_tiles && g.tkind == 1) {
             // A tiles of the result's rows from tr0 on: 4 row blocks x 2 k blocks
 #pragma unroll
@@ -585,6 +607,16 @@ __global__ __launch_bounds__(256, 1) void k_mm16(const MmArgs g) {
     if (g.tkind && g.nbat == 1) pack_logs(tmax, tamax, g.maxlog, EPI == MM_STORE && g.tkind == 1 ? g.amax64 : nullptr);
 }
 
+// 256 x 128 tiles (two work-groups a compute unit) for every product but the
+// LU's persistent bulk update; BEMHIP_MM_TN128=0 keeps 256 x 256 everywhere
+static bool mm_tn128() {
+    static const bool on = [] {
+        const char* v = std::getenv("BEMHIP_MM_TN128");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+
 // max_wg > 0: at most that many work-groups (one per compute unit), each
 // looping over tiles, so that the remaining compute units stay free for
 // kernels of another stream
@@ -621,8 +653,10 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
         g.nks1 = g.nks;
         g.nks += mm_pad(seg->K, MM_BK) / MM_BK;
     }
+    const bool bulk = seg && max_wg > 0;
+    const int tn = bulk || !mm_tn128() ? 256 : 128;
     g.tiles_m = (M + MM_TILE - 1) / MM_TILE;
-    g.tiles_n = (N + MM_TILE - 1) / MM_TILE;
+    g.tiles_n = (N + tn - 1) / tn;
     g.nbat = 1;
     if (bat) {
         BH_CHECK(bat->n >= 1 && bat->sa % 16 == 0 && bat->sb % 16 == 0 && bat->st % 16 == 0, BEMHIP_ERR_ARG,
@@ -638,7 +672,9 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
         g.ls = bat->ls;
     }
     g.per_xcd = (int)(((long long)g.tiles_m * g.tiles_n * g.nbat + 7) / 8);
-    g.wg_xcd = max_wg > 0 ? std::max(1, std::min(g.per_xcd, max_wg / 8)) : g.per_xcd;
+    // max_wg counts compute units: two 256 x 128 work-groups share one
+    const int wg_cap = tn == 128 ? 2 * max_wg : max_wg;
+    g.wg_xcd = max_wg > 0 ? std::max(1, std::min(g.per_xcd, wg_cap / 8)) : g.per_xcd;
     if (tiles) {
         g.tout = static_cast<h8*>(tiles->out);
         g.tkind = tiles->b_operand ? 2 : 1;
@@ -653,14 +689,18 @@ void mm16_gemm(hipStream_t st, int epi, int tri, const void* At, int Ka, const v
     const dim3 grid(8 * g.wg_xcd);
     if (epi == MM_SUB) {
         BH_CHECK(tri == 0, BEMHIP_ERR_ARG, "mm16: triangular operands only for stores");
-        if (seg && max_wg > 0) k_mm16<MM_SUB, 0, true><<<grid, 256, 0, st>>>(g);
+        if (bulk) k_mm16<MM_SUB, 0, true><<<grid, 256, 0, st>>>(g);
+        else if (tn == 128) k_mm16<MM_SUB, 0, false, 128><<<grid, 256, 0, st>>>(g);
         else k_mm16<MM_SUB, 0><<<grid, 256, 0, st>>>(g);
     } else if (tri == 1) {
-        k_mm16<MM_STORE, 1><<<grid, 256, 0, st>>>(g);
+        if (tn == 128) k_mm16<MM_STORE, 1, false, 128><<<grid, 256, 0, st>>>(g);
+        else k_mm16<MM_STORE, 1><<<grid, 256, 0, st>>>(g);
     } else if (tri == 2) {
-        k_mm16<MM_STORE, 2><<<grid, 256, 0, st>>>(g);
+        if (tn == 128) k_mm16<MM_STORE, 2, false, 128><<<grid, 256, 0, st>>>(g);
+        else k_mm16<MM_STORE, 2><<<grid, 256, 0, st>>>(g);
     } else {
-        k_mm16<MM_STORE, 0><<<grid, 256, 0, st>>>(g);
+        if (tn == 128) k_mm16<MM_STORE, 0, false, 128><<<grid, 256, 0, st>>>(g);
+        else k_mm16<MM_STORE, 0><<<grid, 256, 0, st>>>(g);
     }
     BH_HIP(hipGetLastError());
 }
