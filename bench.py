@@ -42,11 +42,11 @@ PEAK = {"lu16_ir64": 2500.0, "lu16x3_ir64": 2500.0 / 3.0, "lu32_ir64": 157.3, "l
 # WRITE_SIZE, each scaled by what it reads for a plain copy of known size):
 # profiles/lu16_update_gemm_nu64_hbm_traffic.md (fp16 LU),
 # profiles/split16_update_gemm_nu64_counters.md (split-fp16 LU)
-TRAFFIC_NU64 = {"lu16_ir64": 6.225e9, "lu16x3_ir64": 6.21e9}
+TRAFFIC_NU64 = {"lu16_ir64": 6.081e9, "lu16x3_ir64": 6.21e9}
 # MFMA busy fraction of the same launches (SQ_VALU_MFMA_BUSY_CYCLES over
 # GRBM_GUI_ACTIVE x SIMDs, rocprofv3's MfmaUtil) and the engine clock it implies:
 # profiles/lu16_update_gemm_nu64_mfma_utilisation.md, profiles/split16_update_gemm_nu64_counters.md
-MFMA_BUSY_NU64 = {"lu16_ir64": (0.332, 1.94), "lu16x3_ir64": (0.517, 1.82)}
+MFMA_BUSY_NU64 = {"lu16_ir64": (0.329, 1.98), "lu16x3_ir64": (0.517, 1.82)}
 DTYPE = {"lu16_ir64": "fp16-mfma lu (fp32 acc, fp32 factor)+f64-ir", "lu16x3_ir64": "f32-lu(split-f16 mfma)+f64-ir",
          "lu32_ir64": "f32-lu+f64-ir", "lu64": "f64", "bicgstab64": "f64"}
 
